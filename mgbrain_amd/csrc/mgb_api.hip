@@ -1,0 +1,1187 @@
+// mgb_api.hip -- the C ABI (include/mgbrain_hip.h) over the HIP kernels.
+//
+// Host-side orchestration of one single-GPU simulator: device network build,
+// format choice, constant upload, the windowed step loop (optionally replayed
+// from a hipGraph) and read-back.  Reference counterparts are cited per entry
+// point in the header.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mgb_build.h"
+#include "mgb_internal.h"
+#include "mgb_rng.h"
+
+namespace mgb {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+}  // namespace mgb
+
+using namespace mgb;
+
+struct mgb_sim {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    int32_t n = 0, npad = 0, D = 1, min_delay = 0, window = 1;
+    int64_t S = 0;
+    int mode = kRingClass;
+    int wf = MGB_WF_CLASS;
+    int wide_kind = kWideF32;
+    int stdp = 0;
+    int use_graph = 0;
+    int prop_grid = 0, prop_block = 256;
+    int64_t rec_cap = 0;
+    int64_t launches = 0;
+    float scale_e = 1.f, scale_i = 1.f;
+    float sc[6] = {0, 0, 0, 0, 0, 0};
+    int64_t n_plastic = 0;
+    // device
+    Ctl* ctl = nullptr;
+    uint8_t *type = nullptr, *cset = nullptr;
+    uint16_t *tbl = nullptr, *stream_idx = nullptr;
+    float *rate = nullptr, *V = nullptr, *Ie = nullptr, *Ii = nullptr;
+    int32_t *refr = nullptr, *last = nullptr, *cnt = nullptr;
+    CSet* csets = nullptr;
+    int32_t* stream_base = nullptr;
+    uint64_t* stream_seed = nullptr;
+    int64_t* row_ptr = nullptr;
+    uint32_t* syn = nullptr;
+    int32_t* col = nullptr;
+    uint8_t* dly = nullptr;
+    float* w32 = nullptr;
+    uint16_t* w16 = nullptr;
+    uint8_t* w8 = nullptr;
+    float* row_scale = nullptr;
+    int64_t* in_ptr = nullptr;
+    uint32_t* in_pos = nullptr;
+    int32_t* in_src = nullptr;
+    uint32_t* ring = nullptr;
+    int32_t *ring_e = nullptr, *ring_i = nullptr;
+    uint32_t* tbl_off = nullptr;
+    float* seq = nullptr;
+    float *ltp = nullptr, *ltd = nullptr;
+    int32_t lut_n = 1;
+    uint32_t* win_buf = nullptr;
+    uint32_t win_stride = 0;
+    uint64_t* rec = nullptr;
+    // host mirrors for read-back
+    std::vector<float> class_w;        // [n_tbl][2]
+    std::vector<uint16_t> h_tbl;       // [n]
+    int32_t n_tbl = 0;
+    // graph
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int32_t graph_windows = 0;
+    // bookkeeping
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+};
+
+namespace {
+
+mgb_status dalloc(mgb_sim* s, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        *p = nullptr;
+        return e == hipErrorOutOfMemory ? MGB_ERR_OOM : MGB_ERR_HIP;
+    }
+    s->allocs.push_back(*p);
+    s->bytes += (int64_t)bytes;
+    return MGB_OK;
+}
+
+template <class T>
+mgb_status dalloc_n(mgb_sim* s, T** p, size_t count) {
+    return dalloc(s, reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+void dfree(mgb_sim* s, void* p) {
+    if (!p) return;
+    auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
+    if (it != s->allocs.end()) s->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+// Temporary device buffers freed on scope exit.
+struct Scratch {
+    std::vector<void*> ptrs;
+    ~Scratch() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    mgb_status get(T** p, size_t count) {
+        void* q = nullptr;
+        size_t bytes = std::max<size_t>(count * sizeof(T), 4);
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) {
+            set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+            return e == hipErrorOutOfMemory ? MGB_ERR_OOM : MGB_ERR_HIP;
+        }
+        ptrs.push_back(q);
+        *p = static_cast<T*>(q);
+        return MGB_OK;
+    }
+    void release(void* p) {
+        auto it = std::find(ptrs.begin(), ptrs.end(), p);
+        if (it != ptrs.end()) {
+            (void)hipFree(p);
+            ptrs.erase(it);
+        }
+    }
+};
+
+#define MGB_TRY(x)                       \
+    do {                                 \
+        mgb_status st_ = (x);            \
+        if (st_ != MGB_OK) return st_;   \
+    } while (0)
+
+int bits_for(uint64_t v) {   // bits to represent 0..v
+    int b = 1;
+    while (b < 32 && (v >> b) != 0) ++b;
+    return b;
+}
+
+void free_all(mgb_sim* s) {
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (void* p : s->allocs) (void)hipFree(p);
+    s->allocs.clear();
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+// Per-neuron host inputs common to both construction paths.
+struct HostNeurons {
+    int32_t n = 0;
+    std::vector<uint8_t> type, cset;
+    std::vector<uint16_t> stream;
+    std::vector<float> rate, v0;
+    std::vector<float> lifconst;       // [n_csets][30]
+    std::vector<int32_t> stream_base;
+    std::vector<uint64_t> stream_seed;
+    std::vector<float> fp8_scale;      // [n] or empty
+    float stdpconst[6] = {0.1f, -0.01f, 17.f, 34.f, 40.f, 0.f};
+    float step = 1e-4f;
+    int stdp = 0;
+};
+
+// Device list of synapses in reference list order.
+struct DevList {
+    uint64_t S = 0;
+    int32_t* src = nullptr;
+    int32_t* tar = nullptr;
+    float* weight = nullptr;
+    float* delay = nullptr;
+    uint8_t* plastic = nullptr;   // may be null (no STDP)
+};
+
+mgb_status init_device(mgb_sim* s, const mgb_opts* o) {
+    int ndev = 0;
+    MGB_HIP_TRY(hipGetDeviceCount(&ndev));
+    if (ndev <= 0) {
+        set_error("no HIP device visible");
+        return MGB_ERR_HIP;
+    }
+    s->device = o ? o->device : 0;
+    if (s->device < 0 || s->device >= ndev) {
+        set_error("device %d out of range (%d devices)", s->device, ndev);
+        return MGB_ERR_INVALID;
+    }
+    MGB_HIP_TRY(hipSetDevice(s->device));
+    MGB_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    MGB_HIP_TRY(hipEventCreate(&s->ev0));
+    MGB_HIP_TRY(hipEventCreate(&s->ev1));
+    return MGB_OK;
+}
+
+mgb_status validate_neurons(const HostNeurons& h) {
+    if (h.n <= 0 || h.n > kMaxNeurons) {
+        set_error("neuron count %d outside [1, %d]", h.n, kMaxNeurons);
+        return MGB_ERR_INVALID;
+    }
+    if (!(h.step > 0.f)) {
+        set_error("time step must be positive");
+        return MGB_ERR_INVALID;
+    }
+    const size_t ncs = h.lifconst.size() / 30;
+    if (ncs == 0 || ncs > 255) {
+        set_error("constant sets must number 1..255");
+        return MGB_ERR_INVALID;
+    }
+    for (int32_t i = 0; i < h.n; ++i) {
+        if (h.type[i] > MGB_LIFB) {
+            set_error("neuron %d has unknown type %d", i, (int)h.type[i]);
+            return MGB_ERR_INVALID;
+        }
+        if (h.cset[i] >= ncs) {
+            set_error("neuron %d uses constant set %d of %zu", i, (int)h.cset[i], ncs);
+            return MGB_ERR_INVALID;
+        }
+        if (h.stream[i] >= h.stream_seed.size()) {
+            set_error("neuron %d uses Poisson stream %d of %zu", i, (int)h.stream[i],
+                      h.stream_seed.size());
+            return MGB_ERR_INVALID;
+        }
+    }
+    for (size_t k = 0; k < ncs; ++k) {
+        const float* c = &h.lifconst[30 * k];
+        if (!(c[0] > 0.f)) {
+            set_error("constant set %zu has non-positive DT (LIFECONSTS[0])", k);
+            return MGB_ERR_INVALID;
+        }
+    }
+    return MGB_OK;
+}
+
+// STDP tables: ltp[k] = A_LTP*expf(-k/TAU_LTP), ltd[k] = A_LTD*expf(-k/TAU_LTD)
+// (mgcore.cu:351,355 evaluated on the host); the last entry is the zero plateau.
+mgb_status build_stdp_lut(const float* sc, std::vector<float>& ltp, std::vector<float>& ltd) {
+    const int32_t cap = 1 << 22;
+    ltp.assign(1, 0.f);
+    ltd.assign(1, 0.f);
+    for (int32_t k = 1; k < cap; ++k) {
+        const float x = (float)(-k);
+        const float e1 = expf(x / sc[2]);
+        const float e2 = expf(x / sc[3]);
+        ltp.push_back(sc[0] * e1);
+        ltd.push_back(sc[1] * e2);
+        if (e1 == 0.f && e2 == 0.f) return MGB_OK;
+    }
+    set_error("STDP time constants too long for the exponential table (%d steps)", cap);
+    return MGB_ERR_UNSUPPORTED;
+}
+
+uint64_t host_u64(const void* p) {
+    uint64_t v;
+    std::memcpy(&v, p, 8);
+    return v;
+}
+
+// Builds device CSR + tables from a device synapse list; frees nothing of `L`.
+mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int wf_req) {
+    const int32_t n = h.n;
+    const uint64_t S = L.S;
+    hipStream_t st = s->stream;
+    s->S = (int64_t)S;
+    if (S >= 0xFFFFFFFFull) {
+        set_error("%llu synapses exceed the 2^32-1 limit of one simulator",
+                  (unsigned long long)S);
+        return MGB_ERR_UNSUPPORTED;
+    }
+    Scratch X;
+    // ---- statistics + delay quantisation --------------------------------------
+    uint8_t* dq = nullptr;
+    uint32_t *flags = nullptr, *wcls = nullptr, *indeg = nullptr, *in_pl = nullptr, *absmax = nullptr;
+    int32_t* mm = nullptr;
+    unsigned long long* npl = nullptr;
+    MGB_TRY(X.get(&dq, S));
+    MGB_TRY(X.get(&flags, 1));
+    MGB_TRY(X.get(&mm, 2));
+    MGB_TRY(X.get(&wcls, 2 * (size_t)n));
+    MGB_TRY(X.get(&indeg, 2 * (size_t)n));
+    MGB_TRY(X.get(&in_pl, (size_t)n));
+    MGB_TRY(X.get(&absmax, 1));
+    MGB_TRY(X.get(&npl, 1));
+    const int32_t mm_init[2] = {0x7FFFFFFF, 0};
+    MGB_HIP_TRY(hipMemsetAsync(flags, 0, 4, st));
+    MGB_HIP_TRY(hipMemcpyAsync(mm, mm_init, 8, hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemsetAsync(wcls, 0xFF, 8 * (size_t)n, st));
+    MGB_HIP_TRY(hipMemsetAsync(indeg, 0, 8 * (size_t)n, st));
+    MGB_HIP_TRY(hipMemsetAsync(in_pl, 0, 4 * (size_t)n, st));
+    MGB_HIP_TRY(hipMemsetAsync(absmax, 0, 4, st));
+    MGB_HIP_TRY(hipMemsetAsync(npl, 0, 8, st));
+    if (S > 0) {
+        StatArgs a{};
+        a.S = S; a.n = n; a.step = h.step;
+        a.src = L.src; a.tar = L.tar; a.weight = L.weight; a.delay = L.delay;
+        a.plastic = h.stdp ? L.plastic : nullptr;
+        a.dq = dq; a.flags = flags; a.min_d = mm; a.max_d = mm + 1;
+        a.wcls = wcls; a.indeg = indeg; a.in_pl = in_pl; a.absmax = absmax; a.n_plastic = npl;
+        MGB_HIP_TRY(run_stats(a, st));
+    }
+    uint32_t hflags = 0, habs = 0;
+    int32_t hmm[2];
+    unsigned long long hnpl = 0;
+    MGB_HIP_TRY(hipMemcpyAsync(&hflags, flags, 4, hipMemcpyDeviceToHost, st));
+    MGB_HIP_TRY(hipMemcpyAsync(hmm, mm, 8, hipMemcpyDeviceToHost, st));
+    MGB_HIP_TRY(hipMemcpyAsync(&habs, absmax, 4, hipMemcpyDeviceToHost, st));
+    MGB_HIP_TRY(hipMemcpyAsync(&hnpl, npl, 8, hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> h_wcls(2 * (size_t)n), h_indeg(2 * (size_t)n);
+    MGB_HIP_TRY(hipMemcpyAsync(h_wcls.data(), wcls, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
+    MGB_HIP_TRY(hipMemcpyAsync(h_indeg.data(), indeg, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
+    MGB_HIP_TRY(hipStreamSynchronize(st));
+    if (hflags & kFlagBadId) {
+        set_error("a synapse references a neuron id outside [0, %d)", n);
+        return MGB_ERR_INVALID;
+    }
+    if (hflags & kFlagDelayLow) {
+        set_error("a synapse delay rounds to less than one step (the reference divides by "
+                  "min_delay = 0, mgsim.cpp:802)");
+        return MGB_ERR_INVALID;
+    }
+    if (hflags & kFlagDelayHigh) {
+        set_error("a synapse delay exceeds %d steps", kWideMaxDelay);
+        return MGB_ERR_UNSUPPORTED;
+    }
+    const int32_t max_d = S ? hmm[1] : 0;
+    s->min_delay = S ? hmm[0] : 0;
+    s->D = max_d + 1;                                   // mgsim.cpp:71
+    s->n_plastic = (int64_t)hnpl;
+    uint32_t max_deg_cls = 0, max_deg_tot = 0;
+    for (int32_t t = 0; t < n; ++t) {
+        max_deg_cls = std::max({max_deg_cls, h_indeg[t], h_indeg[(size_t)n + t]});
+        max_deg_tot = std::max(max_deg_tot, h_indeg[t] + h_indeg[(size_t)n + t]);
+    }
+    // ---- format choice ----------------------------------------------------------
+    const bool class_ok = !(hflags & kFlagMixed) && !h.stdp && max_d <= kClassMaxDelay &&
+                          max_deg_cls <= 0xFFFFu;
+    int wf = wf_req;
+    if (wf == MGB_WF_AUTO) wf = class_ok ? MGB_WF_CLASS : MGB_WF_F32;
+    if (wf == MGB_WF_CLASS && !class_ok) {
+        set_error("CLASS weight format needs one weight per (target, sign), no STDP, delays <= %d "
+                  "steps and in-degree <= 65535", kClassMaxDelay);
+        return MGB_ERR_UNSUPPORTED;
+    }
+    if (wf < MGB_WF_AUTO || wf > MGB_WF_FP8) {
+        set_error("unknown weight format %d", wf);
+        return MGB_ERR_INVALID;
+    }
+    s->wf = wf;
+    s->mode = (wf == MGB_WF_CLASS) ? kRingClass : kRingFixed;
+    s->wide_kind = wf == MGB_WF_BF16 ? kWideBF16 : (wf == MGB_WF_FP8 ? kWideFP8 : kWideF32);
+
+    // ---- stable sort by source ---------------------------------------------------
+    uint32_t *kin = nullptr, *kout = nullptr, *vin = nullptr, *vout = nullptr;
+    MGB_TRY(X.get(&kin, S));
+    MGB_TRY(X.get(&kout, S));
+    MGB_TRY(X.get(&vin, S));
+    MGB_TRY(X.get(&vout, S));
+    const bool cls = s->mode == kRingClass;
+    MGB_HIP_TRY(make_sort_input(L.src, L.tar, L.weight, dq, cls, kin, vin, S, st));
+    const int kbits = bits_for((uint64_t)n);
+    size_t tb = 0;
+    void* temp = nullptr;
+    if (S > 0) {
+        MGB_HIP_TRY(sort_pairs(nullptr, &tb, kin, kout, vin, vout, S, kbits, st));
+        MGB_TRY(X.get(reinterpret_cast<uint8_t**>(&temp), tb));
+        MGB_HIP_TRY(sort_pairs(temp, &tb, kin, kout, vin, vout, S, kbits, st));
+    }
+    MGB_TRY(dalloc_n(s, &s->row_ptr, (size_t)n + 1));
+    MGB_HIP_TRY(row_bounds(kout, S, n, s->row_ptr, st));
+
+    if (cls) {
+        MGB_TRY(dalloc_n(s, &s->syn, S));
+        if (S) MGB_HIP_TRY(hipMemcpyAsync(s->syn, vout, S * 4, hipMemcpyDeviceToDevice, st));
+        // class tables: unique (exc weight, inh weight) per target
+        std::map<std::pair<uint32_t, uint32_t>, int32_t> ids;
+        std::vector<uint32_t> tmax_e, tmax_i;
+        s->h_tbl.assign((size_t)n, 0);
+        std::vector<std::pair<uint32_t, uint32_t>> keys;
+        for (int32_t t = 0; t < n; ++t) {
+            const auto key = std::make_pair(h_wcls[t], h_wcls[(size_t)n + t]);
+            auto it = ids.find(key);
+            int32_t id;
+            if (it == ids.end()) {
+                id = (int32_t)keys.size();
+                ids.emplace(key, id);
+                keys.push_back(key);
+                tmax_e.push_back(0);
+                tmax_i.push_back(0);
+            } else {
+                id = it->second;
+            }
+            s->h_tbl[t] = (uint16_t)id;
+            tmax_e[id] = std::max(tmax_e[id], h_indeg[t]);
+            tmax_i[id] = std::max(tmax_i[id], h_indeg[(size_t)n + t]);
+        }
+        if (keys.size() > 0xFFFFu) {
+            set_error("too many weight classes (%zu)", keys.size());
+            return MGB_ERR_UNSUPPORTED;
+        }
+        s->n_tbl = (int32_t)keys.size();
+        std::vector<uint32_t> offs(2 * keys.size());
+        std::vector<float> seq;
+        s->class_w.assign(2 * keys.size(), 0.f);
+        for (size_t c = 0; c < keys.size(); ++c) {
+            for (int side = 0; side < 2; ++side) {
+                const uint32_t bits = side ? keys[c].second : keys[c].first;
+                const uint32_t len = (side ? tmax_i[c] : tmax_e[c]) + 1;
+                float w = 0.f;
+                if (bits != kWSentinel) std::memcpy(&w, &bits, 4);
+                s->class_w[2 * c + side] = w;
+                offs[2 * c + side] = (uint32_t)seq.size();
+                // seq[k] = k-fold sequential FP32 sum of w: the value k float
+                // atomicAdds of w into a zeroed buffer leave (mgcore.cu:112-115)
+                float acc = 0.f;
+                seq.push_back(acc);
+                for (uint32_t k = 1; k < len; ++k) {
+                    acc = acc + w;
+                    seq.push_back(acc);
+                }
+            }
+        }
+        MGB_TRY(dalloc_n(s, &s->tbl, (size_t)n));
+        MGB_TRY(dalloc_n(s, &s->tbl_off, offs.size()));
+        MGB_TRY(dalloc_n(s, &s->seq, seq.size()));
+        MGB_HIP_TRY(hipMemcpyAsync(s->tbl, s->h_tbl.data(), 2 * (size_t)n, hipMemcpyHostToDevice, st));
+        MGB_HIP_TRY(hipMemcpyAsync(s->tbl_off, offs.data(), 4 * offs.size(), hipMemcpyHostToDevice, st));
+        MGB_HIP_TRY(hipMemcpyAsync(s->seq, seq.data(), 4 * seq.size(), hipMemcpyHostToDevice, st));
+        MGB_HIP_TRY(hipStreamSynchronize(st));
+    } else {
+        // fixed-point scale: |ring value| <= in-degree * max |w| < 2^31
+        float wabs = 0.f;
+        std::memcpy(&wabs, &habs, 4);
+        if (h.stdp) wabs = std::max({wabs, std::fabs(h.stdpconst[4]), std::fabs(h.stdpconst[5])});
+        if (s->wide_kind == kWideFP8) wabs *= 1.0625f;   // FP8 rounding headroom
+        const double bound = (double)std::max<uint32_t>(max_deg_tot, 1) * (double)wabs;
+        int e = 30;
+        if (bound > 0) e = (int)std::floor(std::log2(2147483520.0 / bound));
+        e = std::max(-100, std::min(e, 100));
+        s->scale_e = s->scale_i = std::ldexp(1.0f, e);
+        MGB_TRY(dalloc_n(s, &s->col, S));
+        MGB_TRY(dalloc_n(s, &s->dly, S));
+        MGB_TRY(dalloc_n(s, &s->w32, S));
+        if (s->wide_kind == kWideBF16) MGB_TRY(dalloc_n(s, &s->w16, S));
+        if (s->wide_kind == kWideFP8) {
+            MGB_TRY(dalloc_n(s, &s->w8, S));
+            MGB_TRY(dalloc_n(s, &s->row_scale, (size_t)n));
+            if (!h.fp8_scale.empty()) {
+                MGB_HIP_TRY(hipMemcpyAsync(s->row_scale, h.fp8_scale.data(), 4 * (size_t)n,
+                                           hipMemcpyHostToDevice, st));
+            } else {
+                // per-source-row scale: the row's largest |w| maps onto 448
+                uint32_t* rmax = nullptr;
+                MGB_TRY(X.get(&rmax, (size_t)n));
+                MGB_HIP_TRY(fp8_row_scales(L.src, L.weight, S, n, rmax, s->row_scale, st));
+            }
+        }
+        uint32_t* pos_of = nullptr;
+        if (h.stdp) MGB_TRY(X.get(&pos_of, S));
+        GatherArgs g{};
+        g.S = S; g.idx = vout; g.src = L.src; g.tar = L.tar; g.weight = L.weight; g.dq = dq;
+        g.plastic = h.stdp ? L.plastic : nullptr;
+        g.col = s->col; g.dly = s->dly; g.w32 = s->w32; g.w16 = s->w16; g.w8 = s->w8;
+        g.row_scale = s->row_scale; g.pos_of = pos_of;
+        MGB_HIP_TRY(gather_wide(g, st));
+        if (h.stdp) {
+            // incoming CSR of plastic synapses in list order (dend_offs/dend_refs)
+            const int ibits = bits_for((uint64_t)n);
+            MGB_HIP_TRY(make_in_keys(L.tar, L.plastic, n, kin, vin, S, st));
+            if (S > 0) {
+                size_t tb2 = 0;
+                MGB_HIP_TRY(sort_pairs(nullptr, &tb2, kin, kout, vin, vout, S, ibits, st));
+                if (tb2 > tb) {
+                    X.release(temp);
+                    MGB_TRY(X.get(reinterpret_cast<uint8_t**>(&temp), tb2));
+                    tb = tb2;
+                }
+                MGB_HIP_TRY(sort_pairs(temp, &tb2, kin, kout, vin, vout, S, ibits, st));
+            }
+            MGB_TRY(dalloc_n(s, &s->in_ptr, (size_t)n + 1));
+            MGB_HIP_TRY(row_bounds(kout, S, n, s->in_ptr, st));
+            MGB_TRY(dalloc_n(s, &s->in_pos, (size_t)s->n_plastic));
+            MGB_TRY(dalloc_n(s, &s->in_src, (size_t)s->n_plastic));
+            MGB_HIP_TRY(in_fill(vout, pos_of, L.src, (uint64_t)s->n_plastic, s->in_pos, s->in_src, st));
+        }
+        MGB_HIP_TRY(hipStreamSynchronize(st));
+    }
+    return MGB_OK;
+}
+
+mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
+    const int32_t n = h.n;
+    hipStream_t st = s->stream;
+    s->n = n;
+    s->npad = (n + kNeuBlock - 1) / kNeuBlock * kNeuBlock;
+    s->stdp = h.stdp;
+    std::memcpy(s->sc, h.stdpconst, sizeof(s->sc));
+    const size_t np = (size_t)s->npad;
+    MGB_TRY(dalloc_n(s, &s->ctl, 1));
+    MGB_TRY(dalloc_n(s, &s->type, np));
+    MGB_TRY(dalloc_n(s, &s->cset, np));
+    MGB_TRY(dalloc_n(s, &s->stream_idx, np));
+    MGB_TRY(dalloc_n(s, &s->rate, np));
+    MGB_TRY(dalloc_n(s, &s->V, np));
+    MGB_TRY(dalloc_n(s, &s->Ie, np));
+    MGB_TRY(dalloc_n(s, &s->Ii, np));
+    MGB_TRY(dalloc_n(s, &s->refr, np));
+    MGB_TRY(dalloc_n(s, &s->last, np));
+    MGB_TRY(dalloc_n(s, &s->cnt, np));
+    MGB_HIP_TRY(hipMemsetAsync(s->ctl, 0, sizeof(Ctl), st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->type, h.type.data(), (size_t)n, hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->cset, h.cset.data(), (size_t)n, hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->stream_idx, h.stream.data(), 2 * (size_t)n, hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->rate, h.rate.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->V, h.v0.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemsetAsync(s->Ie, 0, 4 * np, st));
+    MGB_HIP_TRY(hipMemsetAsync(s->Ii, 0, 4 * np, st));
+    MGB_HIP_TRY(hipMemsetAsync(s->refr, 0, 4 * np, st));
+    MGB_HIP_TRY(hipMemsetAsync(s->last, 0, 4 * np, st));   // mgnet.cpp:16 zero-initialised
+    MGB_HIP_TRY(hipMemsetAsync(s->cnt, 0, 4 * np, st));
+    // constants: LIFECONSTS + refractory steps (mgcore.cu:153)
+    const size_t ncs = h.lifconst.size() / 30;
+    std::vector<CSet> cs(ncs);
+    for (size_t k = 0; k < ncs; ++k) {
+        std::memset(&cs[k], 0, sizeof(CSet));
+        std::memcpy(cs[k].c, &h.lifconst[30 * k], 30 * sizeof(float));
+        const int32_t refr = (int32_t)roundf(cs[k].c[8] / cs[k].c[0]) - 1;
+        std::memcpy(&cs[k].c[30], &refr, 4);
+    }
+    MGB_TRY(dalloc_n(s, &s->csets, ncs));
+    MGB_HIP_TRY(hipMemcpyAsync(s->csets, cs.data(), sizeof(CSet) * ncs, hipMemcpyHostToDevice, st));
+    MGB_TRY(dalloc_n(s, &s->stream_base, h.stream_base.size()));
+    MGB_TRY(dalloc_n(s, &s->stream_seed, h.stream_seed.size()));
+    MGB_HIP_TRY(hipMemcpyAsync(s->stream_base, h.stream_base.data(), 4 * h.stream_base.size(),
+                               hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->stream_seed, h.stream_seed.data(), 8 * h.stream_seed.size(),
+                               hipMemcpyHostToDevice, st));
+    // ring
+    if (s->mode == kRingClass) {
+        MGB_TRY(dalloc_n(s, &s->ring, (size_t)s->D * np));
+        MGB_HIP_TRY(hipMemsetAsync(s->ring, 0, 4 * (size_t)s->D * np, st));
+    } else {
+        MGB_TRY(dalloc_n(s, &s->ring_e, (size_t)s->D * np));
+        MGB_TRY(dalloc_n(s, &s->ring_i, (size_t)s->D * np));
+        MGB_HIP_TRY(hipMemsetAsync(s->ring_e, 0, 4 * (size_t)s->D * np, st));
+        MGB_HIP_TRY(hipMemsetAsync(s->ring_i, 0, 4 * (size_t)s->D * np, st));
+    }
+    // window
+    int32_t win = s->S ? s->min_delay : kMaxWindow;
+    if (o && o->window > 0) win = std::min(win, o->window);
+    win = std::min(win, kMaxWindow);
+    if (h.stdp) win = 1;
+    s->window = std::max(win, 1);
+    s->win_stride = (uint32_t)(np * (size_t)s->window);
+    MGB_TRY(dalloc_n(s, &s->win_buf, 2 * (size_t)s->win_stride));
+    // record
+    s->rec_cap = o ? std::max<int64_t>(o->record_capacity, 0) : 0;
+    if (s->rec_cap > 0) MGB_TRY(dalloc_n(s, &s->rec, (size_t)s->rec_cap));
+    // STDP tables
+    if (h.stdp) {
+        std::vector<float> ltp, ltd;
+        MGB_TRY(build_stdp_lut(h.stdpconst, ltp, ltd));
+        s->lut_n = (int32_t)ltp.size();
+        MGB_TRY(dalloc_n(s, &s->ltp, ltp.size()));
+        MGB_TRY(dalloc_n(s, &s->ltd, ltd.size()));
+        MGB_HIP_TRY(hipMemcpyAsync(s->ltp, ltp.data(), 4 * ltp.size(), hipMemcpyHostToDevice, st));
+        MGB_HIP_TRY(hipMemcpyAsync(s->ltd, ltd.data(), 4 * ltd.size(), hipMemcpyHostToDevice, st));
+    }
+    s->use_graph = o ? o->use_graph : 0;
+    s->prop_block = (o && o->prop_block >= 64 && o->prop_block <= 1024) ? (o->prop_block / 64) * 64 : 256;
+    s->prop_grid = (o && o->prop_grid > 0) ? o->prop_grid : 0;
+    MGB_HIP_TRY(hipStreamSynchronize(st));
+    return MGB_OK;
+}
+
+NeuArgs neu_args(const mgb_sim* s) {
+    NeuArgs a{};
+    a.n = s->n; a.npad = s->npad; a.D = s->D;
+    a.type = s->type; a.cset = s->cset; a.tbl = s->tbl; a.stream = s->stream_idx;
+    a.rate = s->rate; a.V = s->V; a.Ie = s->Ie; a.Ii = s->Ii;
+    a.refr = s->refr; a.last = s->last; a.cnt = s->cnt;
+    a.csets = s->csets; a.stream_base = s->stream_base; a.stream_seed = s->stream_seed;
+    a.ring = s->ring; a.tbl_off = s->tbl_off; a.seq = s->seq;
+    a.ring_e = s->ring_e; a.ring_i = s->ring_i;
+    a.inv_e = 1.0f / s->scale_e; a.inv_i = 1.0f / s->scale_i;
+    a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
+    return a;
+}
+
+PropArgs prop_args(const mgb_sim* s) {
+    PropArgs a{};
+    a.n = s->n; a.npad = s->npad; a.D = s->D;
+    a.row_ptr = s->row_ptr; a.syn = s->syn; a.col = s->col; a.dly = s->dly;
+    a.w32 = s->w32; a.w16 = s->w16; a.w8 = s->w8; a.row_scale = s->row_scale;
+    a.ring = s->ring; a.ring_e = s->ring_e; a.ring_i = s->ring_i;
+    a.scale_e = s->scale_e; a.scale_i = s->scale_i;
+    a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
+    a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
+    return a;
+}
+
+StdpArgs stdp_args(const mgb_sim* s) {
+    StdpArgs a{};
+    a.n = s->n; a.row_ptr = s->row_ptr; a.col = s->col; a.dly = s->dly; a.w32 = s->w32;
+    a.in_ptr = s->in_ptr; a.in_pos = s->in_pos; a.in_src = s->in_src; a.last = s->last;
+    a.ltp = s->ltp; a.ltd = s->ltd; a.lut_n = s->lut_n;
+    a.w_max = s->sc[4]; a.w_min = s->sc[5];
+    a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
+    return a;
+}
+
+int prop_grid(const mgb_sim* s, int32_t nst) {
+    if (s->prop_grid > 0) return s->prop_grid;
+    // waves >= expected spikes; capped so an idle window costs little
+    const int64_t waves_per_block = s->prop_block / 64;
+    int64_t spikes_max = (int64_t)s->n * (int64_t)nst;
+    int64_t g = (spikes_max + waves_per_block - 1) / waves_per_block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
+// Issues the kernels of `windows` windows starting `off` steps after base_step.
+mgb_status issue_windows(mgb_sim* s, int32_t nsteps, bool finalize) {
+    const NeuArgs na = neu_args(s);
+    const PropArgs pa = prop_args(s);
+    const StdpArgs sa = stdp_args(s);
+    int32_t off = 0, wk = 0;
+    while (off < nsteps) {
+        const int32_t nst = std::min(s->window, nsteps - off);
+        MGB_HIP_TRY(launch_neurons(s->mode, na, off, wk, nst, s->stream));
+        MGB_HIP_TRY(launch_prop(s->mode, s->wide_kind, pa, off, wk, prop_grid(s, nst),
+                                s->prop_block, s->stream));
+        s->launches += 2;
+        if (s->stdp) {
+            MGB_HIP_TRY(launch_stdp(sa, off, wk, prop_grid(s, nst), s->stream));
+            s->launches += 1;
+        }
+        off += nst;
+        ++wk;
+    }
+    if (finalize) {
+        MGB_HIP_TRY(launch_finalize(s->ctl, nsteps, wk, s->stream));
+        s->launches += 1;
+    }
+    return MGB_OK;
+}
+
+mgb_status ensure_graph(mgb_sim* s, int32_t steps) {
+    if (s->gexec) return MGB_OK;
+    MGB_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    const int64_t before = s->launches;
+    mgb_status st = issue_windows(s, steps, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    s->launches = before;
+    if (st != MGB_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) {
+        set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        return MGB_ERR_HIP;
+    }
+    s->graph = g;
+    MGB_HIP_TRY(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+    s->graph_windows = steps;
+    return MGB_OK;
+}
+
+mgb_status copy_from_host(mgb_sim* s, Scratch& X, void** dst, const void* src, size_t bytes) {
+    uint8_t* p = nullptr;
+    MGB_TRY(X.get(&p, bytes));
+    if (bytes) MGB_HIP_TRY(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s->stream));
+    *dst = p;
+    return MGB_OK;
+}
+
+mgb_status create_common(mgb_sim* s, const HostNeurons& h, const DevList& L, const mgb_opts* o) {
+    MGB_TRY(validate_neurons(h));
+    MGB_TRY(build_network(s, h, L, o ? o->weight_format : MGB_WF_AUTO));
+    MGB_TRY(setup_state(s, h, o));
+    return MGB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mgb_abi_version(void) { return MGB_ABI_VERSION; }
+
+const char* mgb_last_error(void) { return g_err.c_str(); }
+
+mgb_status mgb_create(const mgb_net_desc* d, const mgb_opts* o, mgb_sim** out) {
+    if (!d || !out) {
+        set_error("null argument");
+        return MGB_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (d->n_neurons <= 0 || !d->type || !d->lifconst || d->n_csets < 1 ||
+        (d->n_synapses > 0 && (!d->syn_src || !d->syn_tar || !d->syn_weight || !d->syn_delay)) ||
+        d->n_synapses < 0) {
+        set_error("incomplete network description");
+        return MGB_ERR_INVALID;
+    }
+    HostNeurons h;
+    const int32_t n = d->n_neurons;
+    h.n = n;
+    h.step = d->step;
+    h.stdp = d->stdp ? 1 : 0;
+    h.lifconst.assign(d->lifconst, d->lifconst + 30 * (size_t)d->n_csets);
+    if (d->stdpconst) std::memcpy(h.stdpconst, d->stdpconst, sizeof(h.stdpconst));
+    h.type.resize(n);
+    h.cset.assign(n, 0);
+    h.stream.assign(n, 0);
+    h.rate.assign(n, 0.f);
+    h.v0.resize(n);
+    for (int32_t i = 0; i < n; ++i) {
+        if (d->type[i] < 0 || d->type[i] > MGB_LIFB) {
+            set_error("neuron %d has unknown type %d", i, d->type[i]);
+            return MGB_ERR_INVALID;
+        }
+        h.type[i] = (uint8_t)d->type[i];
+        if (d->cset) h.cset[i] = d->cset[i];
+        if (d->stream) h.stream[i] = d->stream[i];
+        if (d->rate) h.rate[i] = d->rate[i];
+    }
+    const int32_t ns = std::max(d->n_streams, 1);
+    h.stream_base.assign(ns, 0);
+    h.stream_seed.assign(ns, d->seed);
+    for (int32_t k = 0; k < ns; ++k) {
+        if (d->stream_base && k < d->n_streams) h.stream_base[k] = d->stream_base[k];
+        if (d->stream_seed && k < d->n_streams) h.stream_seed[k] = d->stream_seed[k];
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        if (h.cset[i] >= (uint32_t)d->n_csets) {
+            set_error("neuron %d uses constant set %d of %d", i, (int)h.cset[i], d->n_csets);
+            return MGB_ERR_INVALID;
+        }
+        h.v0[i] = d->v_init ? d->v_init[i] : h.lifconst[30 * (size_t)h.cset[i] + 1];
+    }
+    if (d->fp8_scale) h.fp8_scale.assign(d->fp8_scale, d->fp8_scale + n);
+
+    mgb_sim* s = new (std::nothrow) mgb_sim();
+    if (!s) {
+        set_error("host allocation failed");
+        return MGB_ERR_OOM;
+    }
+    mgb_status st = init_device(s, o);
+    if (st == MGB_OK) {
+        Scratch X;
+        DevList L;
+        L.S = (uint64_t)d->n_synapses;
+        const size_t S = (size_t)L.S;
+        void *a = nullptr, *b = nullptr, *c = nullptr, *e = nullptr, *p = nullptr;
+        st = copy_from_host(s, X, &a, d->syn_src, 4 * S);
+        if (st == MGB_OK) st = copy_from_host(s, X, &b, d->syn_tar, 4 * S);
+        if (st == MGB_OK) st = copy_from_host(s, X, &c, d->syn_weight, 4 * S);
+        if (st == MGB_OK) st = copy_from_host(s, X, &e, d->syn_delay, 4 * S);
+        std::vector<uint8_t> pl;
+        if (st == MGB_OK && h.stdp) {
+            if (d->syn_plastic) pl.assign(d->syn_plastic, d->syn_plastic + S);
+            else pl.assign(S, 1);
+            st = copy_from_host(s, X, &p, pl.data(), S);
+        }
+        if (st == MGB_OK) {
+            L.src = (int32_t*)a; L.tar = (int32_t*)b; L.weight = (float*)c; L.delay = (float*)e;
+            L.plastic = (uint8_t*)p;
+            st = create_common(s, h, L, o);
+        }
+    }
+    if (st != MGB_OK) {
+        free_all(s);
+        delete s;
+        return st;
+    }
+    *out = s;
+    return MGB_OK;
+}
+
+mgb_status mgb_create_generated(const mgb_gen_desc* g, const mgb_opts* o, mgb_sim** out) {
+    if (!g || !out || g->n_pops <= 0 || !g->pops || (g->n_projs > 0 && !g->projs) || !g->lifconst ||
+        g->n_csets < 1) {
+        set_error("incomplete generator description");
+        return MGB_ERR_INVALID;
+    }
+    *out = nullptr;
+    HostNeurons h;
+    h.step = g->step;
+    h.stdp = g->stdp ? 1 : 0;
+    h.lifconst.assign(g->lifconst, g->lifconst + 30 * (size_t)g->n_csets);
+    if (g->stdpconst) std::memcpy(h.stdpconst, g->stdpconst, sizeof(h.stdpconst));
+    const int32_t nst = std::max(g->n_streams, 1);
+    h.stream_seed.assign(nst, g->seed);
+    if (g->stream_seed)
+        for (int32_t k = 0; k < g->n_streams; ++k) h.stream_seed[k] = g->stream_seed[k];
+    h.stream_base.assign(nst, -1);
+    std::vector<int32_t> first(g->n_pops + 1, 0);
+    int64_t total = 0;
+    for (int32_t p = 0; p < g->n_pops; ++p) {
+        const mgb_gen_pop& P = g->pops[p];
+        if (P.num < 0 || P.stream < 0 || P.stream >= nst || P.cset < 0 || P.cset >= g->n_csets ||
+            P.type < 0 || P.type > MGB_LIFB) {
+            set_error("population %d is invalid", p);
+            return MGB_ERR_INVALID;
+        }
+        first[p] = (int32_t)total;
+        total += P.num;
+        if (total > kMaxNeurons) {
+            set_error("too many neurons");
+            return MGB_ERR_UNSUPPORTED;
+        }
+    }
+    first[g->n_pops] = (int32_t)total;
+    const int32_t n = (int32_t)total;
+    h.n = n;
+    h.type.resize(n);
+    h.cset.resize(n);
+    h.stream.resize(n);
+    h.rate.resize(n);
+    h.v0.resize(n);
+    for (int32_t p = 0; p < g->n_pops; ++p) {
+        const mgb_gen_pop& P = g->pops[p];
+        if (h.stream_base[P.stream] < 0) h.stream_base[P.stream] = first[p];
+    }
+    for (int32_t k = 0; k < nst; ++k)
+        if (h.stream_base[k] < 0) h.stream_base[k] = 0;
+    for (int32_t p = 0; p < g->n_pops; ++p) {
+        const mgb_gen_pop& P = g->pops[p];
+        for (int32_t j = 0; j < P.num; ++j) {
+            const int32_t i = first[p] + j;
+            h.type[i] = (uint8_t)P.type;
+            h.cset[i] = (uint8_t)P.cset;
+            h.stream[i] = (uint16_t)P.stream;
+            h.rate[i] = P.rate;
+            h.v0[i] = vinit_draw(h.stream_seed[P.stream], (uint32_t)(i - h.stream_base[P.stream]),
+                                 P.v_min, P.v_max);
+        }
+    }
+    // list sizes
+    std::vector<uint64_t> base(g->n_projs + 1, 0);
+    mgb_sim* s = new (std::nothrow) mgb_sim();
+    if (!s) {
+        set_error("host allocation failed");
+        return MGB_ERR_OOM;
+    }
+    mgb_status st = init_device(s, o);
+    Scratch X;
+    std::vector<std::vector<int64_t>> bern_off(g->n_projs);
+    std::vector<uint32_t*> bern_cnt(g->n_projs, nullptr);
+    for (int32_t j = 0; j < g->n_projs && st == MGB_OK; ++j) {
+        const mgb_gen_proj& J = g->projs[j];
+        if (J.src < 0 || J.src >= g->n_pops || J.tar < 0 || J.tar >= g->n_pops) {
+            set_error("projection %d references an unknown population", j);
+            st = MGB_ERR_INVALID;
+            break;
+        }
+        const int32_t sn = g->pops[J.src].num, tn = g->pops[J.tar].num;
+        uint64_t cntj = 0;
+        if (J.kind == MGB_CONN_FIXED_INDEGREE) {
+            if (J.k < 0 || (J.k > 0 && sn == 0)) {
+                set_error("projection %d: invalid in-degree", j);
+                st = MGB_ERR_INVALID;
+                break;
+            }
+            cntj = (uint64_t)tn * (uint64_t)J.k;
+        } else if (J.kind == MGB_CONN_BERNOULLI) {
+            GenProjArgs a{};
+            a.sn = sn; a.tn = tn; a.index = (uint32_t)J.index; a.seed = J.seed;
+            a.thr = (uint64_t)((double)J.p * 4294967296.0);
+            st = X.get(&bern_cnt[j], (size_t)std::max(sn, 1));
+            if (st != MGB_OK) break;
+            hipError_t e = gen_bern(a, bern_cnt[j], nullptr, false, s->stream);
+            if (e != hipSuccess) {
+                set_error("Bernoulli count kernel failed: %s", hipGetErrorString(e));
+                st = MGB_ERR_HIP;
+                break;
+            }
+            std::vector<uint32_t> rc((size_t)sn);
+            if (sn) {
+                e = hipMemcpyAsync(rc.data(), bern_cnt[j], 4 * (size_t)sn, hipMemcpyDeviceToHost, s->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+                if (e != hipSuccess) {
+                    set_error("copy failed: %s", hipGetErrorString(e));
+                    st = MGB_ERR_HIP;
+                    break;
+                }
+            }
+            bern_off[j].assign((size_t)sn + 1, 0);
+            for (int32_t m = 0; m < sn; ++m) bern_off[j][m + 1] = bern_off[j][m] + rc[m];
+            cntj = (uint64_t)bern_off[j][sn];
+        } else {
+            set_error("projection %d: connector kind %d is not generated on the device", j, J.kind);
+            st = MGB_ERR_UNSUPPORTED;
+            break;
+        }
+        base[j + 1] = base[j] + cntj;
+    }
+    if (st == MGB_OK) {
+        DevList L;
+        L.S = base[g->n_projs];
+        st = X.get(&L.src, L.S);
+        if (st == MGB_OK) st = X.get(&L.tar, L.S);
+        if (st == MGB_OK) st = X.get(&L.weight, L.S);
+        if (st == MGB_OK) st = X.get(&L.delay, L.S);
+        if (st == MGB_OK) st = X.get(&L.plastic, L.S);
+        for (int32_t j = 0; j < g->n_projs && st == MGB_OK; ++j) {
+            const mgb_gen_proj& J = g->projs[j];
+            GenProjArgs a{};
+            a.sn = g->pops[J.src].num; a.tn = g->pops[J.tar].num;
+            a.s0 = first[J.src]; a.t0 = first[J.tar]; a.k = J.k;
+            a.index = (uint32_t)J.index; a.seed = J.seed;
+            a.thr = (uint64_t)((double)J.p * 4294967296.0);
+            a.wmin = J.wmin; a.wmax = J.wmax; a.dmin = J.dmin; a.dmax = J.dmax;
+            a.dw = g->dw; a.dt = g->dt; a.plastic_flag = J.plastic ? 1 : 0;
+            a.base = base[j];
+            a.src = L.src; a.tar = L.tar; a.weight = L.weight; a.delay = L.delay; a.plastic = L.plastic;
+            hipError_t e;
+            if (J.kind == MGB_CONN_FIXED_INDEGREE) {
+                e = gen_fixk(a, s->stream);
+            } else {
+                int64_t* off = nullptr;
+                st = X.get(&off, bern_off[j].size());
+                if (st != MGB_OK) break;
+                e = hipMemcpyAsync(off, bern_off[j].data(), 8 * bern_off[j].size(), hipMemcpyHostToDevice,
+                                   s->stream);
+                if (e == hipSuccess) e = gen_bern(a, bern_cnt[j], off, true, s->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            }
+            if (e != hipSuccess) {
+                set_error("generator kernel failed: %s", hipGetErrorString(e));
+                st = MGB_ERR_HIP;
+            }
+        }
+        if (st == MGB_OK) st = create_common(s, h, L, o);
+    }
+    if (st != MGB_OK) {
+        free_all(s);
+        delete s;
+        return st;
+    }
+    *out = s;
+    return MGB_OK;
+}
+
+mgb_status mgb_run(mgb_sim* s, int32_t nsteps) {
+    if (!s || nsteps < 0) {
+        set_error("invalid arguments to mgb_run");
+        return MGB_ERR_INVALID;
+    }
+    MGB_HIP_TRY(hipSetDevice(s->device));
+    MGB_HIP_TRY(hipEventRecord(s->ev0, s->stream));
+    int32_t done = 0;
+    if (s->use_graph) {
+        // graph block: a whole number of windows, about 500 steps
+        const int32_t blk = std::max(1, 500 / s->window) * s->window;
+        if (nsteps >= blk) {
+            MGB_TRY(ensure_graph(s, blk));
+            while (nsteps - done >= blk) {
+                MGB_HIP_TRY(hipGraphLaunch(s->gexec, s->stream));
+                s->launches += 1;
+                done += blk;
+            }
+        }
+    }
+    if (nsteps - done > 0) MGB_TRY(issue_windows(s, nsteps - done, true));
+    MGB_HIP_TRY(hipEventRecord(s->ev1, s->stream));
+    s->timed = true;
+    return MGB_OK;
+}
+
+mgb_status mgb_sync(mgb_sim* s) {
+    if (!s) {
+        set_error("null handle");
+        return MGB_ERR_INVALID;
+    }
+    MGB_HIP_TRY(hipSetDevice(s->device));
+    MGB_HIP_TRY(hipStreamSynchronize(s->stream));
+    return MGB_OK;
+}
+
+mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
+    if (!s || !out) {
+        set_error("null argument");
+        return MGB_ERR_INVALID;
+    }
+    MGB_TRY(mgb_sync(s));
+    Ctl c;
+    MGB_HIP_TRY(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->n_neurons = s->n;
+    out->ring_len = s->D;
+    out->min_delay = s->min_delay;
+    out->window = s->window;
+    out->n_synapses = s->S;
+    out->step = c.base_step;
+    out->syn_events = (int64_t)c.se;
+    out->spikes_recorded = (int64_t)c.rec_cnt;
+    out->weight_format = s->wf;
+    out->n_classes = s->n_tbl;
+    out->fix_scale_exc = s->scale_e;
+    out->fix_scale_inh = s->scale_i;
+    out->device_bytes = s->bytes;
+    out->launches = s->launches;
+    out->last_run_ms = 0.0;
+    if (s->timed) {
+        float ms = 0.f;
+        MGB_HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        out->last_run_ms = ms;
+    }
+    return MGB_OK;
+}
+
+mgb_status mgb_read_spikes(mgb_sim* s, int64_t cap, int32_t* steps, int32_t* ids, int64_t* n_out) {
+    if (!s || cap < 0 || (cap > 0 && (!steps || !ids)) || !n_out) {
+        set_error("invalid arguments to mgb_read_spikes");
+        return MGB_ERR_INVALID;
+    }
+    MGB_TRY(mgb_sync(s));
+    Ctl c;
+    MGB_HIP_TRY(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    const uint64_t have = std::min<uint64_t>(c.rec_cnt, (uint64_t)s->rec_cap);
+    std::vector<uint64_t> r(have);
+    if (have) MGB_HIP_TRY(hipMemcpy(r.data(), s->rec, 8 * have, hipMemcpyDeviceToHost));
+    std::sort(r.begin(), r.end());
+    const uint64_t w = std::min<uint64_t>(have, (uint64_t)cap);
+    for (uint64_t k = 0; k < w; ++k) {
+        steps[k] = (int32_t)(r[k] >> 32);
+        ids[k] = (int32_t)(r[k] & 0xFFFFFFFFu);
+    }
+    *n_out = (int64_t)w;
+    if (c.rec_cnt > (uint64_t)s->rec_cap || c.rec_overflow || w < have) {
+        set_error("spike record truncated: %llu spikes, capacity %lld, caller capacity %lld",
+                  (unsigned long long)c.rec_cnt, (long long)s->rec_cap, (long long)cap);
+        return MGB_ERR_CAPACITY;
+    }
+    return MGB_OK;
+}
+
+mgb_status mgb_clear_spikes(mgb_sim* s) {
+    if (!s) {
+        set_error("null handle");
+        return MGB_ERR_INVALID;
+    }
+    MGB_TRY(mgb_sync(s));
+    Ctl c;
+    MGB_HIP_TRY(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    c.rec_cnt = 0;
+    c.rec_overflow = 0;
+    MGB_HIP_TRY(hipMemcpy(s->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
+    return MGB_OK;
+}
+
+mgb_status mgb_read_state(mgb_sim* s, int32_t field, void* out) {
+    if (!s || !out) {
+        set_error("null argument");
+        return MGB_ERR_INVALID;
+    }
+    const void* src = nullptr;
+    switch (field) {
+        case MGB_F_V: src = s->V; break;
+        case MGB_F_IEXC: src = s->Ie; break;
+        case MGB_F_IINH: src = s->Ii; break;
+        case MGB_F_REFR: src = s->refr; break;
+        case MGB_F_LAST: src = s->last; break;
+        case MGB_F_CNT: src = s->cnt; break;
+        default:
+            set_error("unknown state field %d", field);
+            return MGB_ERR_INVALID;
+    }
+    MGB_TRY(mgb_sync(s));
+    MGB_HIP_TRY(hipMemcpy(out, src, 4 * (size_t)s->n, hipMemcpyDeviceToHost));
+    return MGB_OK;
+}
+
+mgb_status mgb_fire_counts(mgb_sim* s, int32_t* out) { return mgb_read_state(s, MGB_F_CNT, out); }
+
+mgb_status mgb_read_csr(mgb_sim* s, int64_t* row_ptr, int32_t* col, int32_t* delay, float* weight) {
+    if (!s) {
+        set_error("null handle");
+        return MGB_ERR_INVALID;
+    }
+    MGB_TRY(mgb_sync(s));
+    const size_t S = (size_t)s->S;
+    if (row_ptr) MGB_HIP_TRY(hipMemcpy(row_ptr, s->row_ptr, 8 * ((size_t)s->n + 1), hipMemcpyDeviceToHost));
+    if (!col && !delay && !weight) return MGB_OK;
+    if (s->mode == kRingClass) {
+        std::vector<uint32_t> syn(S);
+        if (S) MGB_HIP_TRY(hipMemcpy(syn.data(), s->syn, 4 * S, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < S; ++k) {
+            const uint32_t x = syn[k];
+            const int32_t t = (int32_t)(x & kIdMask);
+            if (col) col[k] = t;
+            if (delay) delay[k] = (int32_t)((x >> 26) & 31u) + 1;
+            if (weight) weight[k] = s->class_w[2 * (size_t)s->h_tbl[t] + (x >> 31)];
+        }
+        return MGB_OK;
+    }
+    if (col && S) MGB_HIP_TRY(hipMemcpy(col, s->col, 4 * S, hipMemcpyDeviceToHost));
+    std::vector<uint8_t> dl;
+    if (delay || (weight && s->wide_kind != kWideF32)) {
+        dl.resize(S);
+        if (S) MGB_HIP_TRY(hipMemcpy(dl.data(), s->dly, S, hipMemcpyDeviceToHost));
+    }
+    if (delay)
+        for (size_t k = 0; k < S; ++k) delay[k] = (int32_t)(dl[k] & 127u);
+    if (weight && S) {
+        MGB_HIP_TRY(hipMemcpy(weight, s->w32, 4 * S, hipMemcpyDeviceToHost));
+        if (s->wide_kind == kWideBF16) {
+            std::vector<uint16_t> w16(S);
+            MGB_HIP_TRY(hipMemcpy(w16.data(), s->w16, 2 * S, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < S; ++k)
+                if (!(dl[k] & 128u)) {
+                    const uint32_t b = (uint32_t)w16[k] << 16;
+                    std::memcpy(&weight[k], &b, 4);
+                }
+        } else if (s->wide_kind == kWideFP8) {
+            std::vector<uint8_t> w8(S);
+            std::vector<float> rs((size_t)s->n);
+            std::vector<int64_t> rp((size_t)s->n + 1);
+            MGB_HIP_TRY(hipMemcpy(w8.data(), s->w8, S, hipMemcpyDeviceToHost));
+            MGB_HIP_TRY(hipMemcpy(rs.data(), s->row_scale, 4 * (size_t)s->n, hipMemcpyDeviceToHost));
+            MGB_HIP_TRY(hipMemcpy(rp.data(), s->row_ptr, 8 * ((size_t)s->n + 1), hipMemcpyDeviceToHost));
+            for (int32_t i = 0; i < s->n; ++i)
+                for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+                    if (dl[k] & 128u) continue;
+                    const uint8_t b = w8[k];
+                    const uint32_t sg = (uint32_t)(b >> 7) << 31, e = (b >> 3) & 15u, m = b & 7u;
+                    float v;
+                    if (e == 0) {
+                        v = (float)m * 0x1p-9f;
+                        uint32_t u;
+                        std::memcpy(&u, &v, 4);
+                        u |= sg;
+                        std::memcpy(&v, &u, 4);
+                    } else {
+                        const uint32_t u = sg | ((e + 120u) << 23) | (m << 20);
+                        std::memcpy(&v, &u, 4);
+                    }
+                    weight[k] = v * rs[i];
+                }
+        }
+    }
+    return MGB_OK;
+}
+
+mgb_status mgb_read_weights(mgb_sim* s, float* out) {
+    if (!out) {
+        set_error("null argument");
+        return MGB_ERR_INVALID;
+    }
+    return mgb_read_csr(s, nullptr, nullptr, nullptr, out);
+}
+
+mgb_status mgb_destroy(mgb_sim* s) {
+    if (!s) return MGB_OK;
+    (void)hipSetDevice(s->device);
+    (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return MGB_OK;
+}
+
+}  // extern "C"

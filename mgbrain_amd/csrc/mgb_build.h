@@ -1,0 +1,85 @@
+// mgb_build.h -- device-side network construction entry points (mgb_build.hip).
+#pragma once
+#include "mgb_internal.h"
+
+namespace mgb {
+
+constexpr uint32_t kWSentinel = 0xFFFFFFFFu;   // "no synapse of this class yet"
+constexpr uint32_t kFlagBadId = 1u;
+constexpr uint32_t kFlagDelayLow = 2u;         // quantised delay < 1 step
+constexpr uint32_t kFlagDelayHigh = 4u;        // quantised delay > kWideMaxDelay
+constexpr uint32_t kFlagMixed = 8u;            // a target sees two weights in one class
+
+struct StatArgs {
+    uint64_t S;
+    int32_t n;
+    float step;
+    const int32_t* src;
+    const int32_t* tar;
+    const float* weight;
+    const float* delay;
+    const uint8_t* plastic;
+    uint8_t* dq;            // [S] quantised delay
+    uint32_t* flags;
+    int32_t* min_d;
+    int32_t* max_d;
+    uint32_t* wcls;         // [2][n] class weight bits per target
+    uint32_t* indeg;        // [2][n] in-degree per target and class
+    uint32_t* in_pl;        // [n] plastic in-degree
+    uint32_t* absmax;       // max |w| (float bits)
+    unsigned long long* n_plastic;
+};
+
+struct GatherArgs {
+    uint64_t S;
+    const uint32_t* idx;    // sorted list indices
+    const int32_t* src;
+    const int32_t* tar;
+    const float* weight;
+    const uint8_t* dq;
+    const uint8_t* plastic;
+    int32_t* col;
+    uint8_t* dly;
+    float* w32;
+    uint16_t* w16;
+    uint8_t* w8;
+    const float* row_scale;
+    uint32_t* pos_of;       // [S] CSR slot of each list index (NULL if unused)
+};
+
+struct GenProjArgs {
+    int32_t sn, tn, s0, t0, k;
+    uint32_t index;         // projection index in its network (Philox counter word)
+    uint64_t seed;
+    uint64_t thr;           // Bernoulli threshold p * 2^32
+    float wmin, wmax, dmin, dmax, dw, dt;
+    int32_t plastic_flag;
+    uint64_t base;          // list offset of the projection
+    int32_t* src;
+    int32_t* tar;
+    float* weight;
+    float* delay;
+    uint8_t* plastic;
+};
+
+int blocks_for(uint64_t work, int block);
+hipError_t run_stats(const StatArgs& a, hipStream_t st);
+hipError_t make_sort_input(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
+                           bool class_payload, uint32_t* keys, uint32_t* vals, uint64_t S,
+                           hipStream_t st);
+hipError_t fp8_row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n,
+                          uint32_t* rmax_tmp, float* scale, hipStream_t st);
+hipError_t make_in_keys(const int32_t* tar, const uint8_t* plastic, int32_t n, uint32_t* keys,
+                        uint32_t* vals, uint64_t S, hipStream_t st);
+hipError_t sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* kin, uint32_t* kout,
+                      const uint32_t* vin, uint32_t* vout, uint64_t S, int bits, hipStream_t st);
+hipError_t row_bounds(const uint32_t* sorted_keys, uint64_t S, int32_t n, int64_t* out,
+                      hipStream_t st);
+hipError_t gather_wide(const GatherArgs& a, hipStream_t st);
+hipError_t in_fill(const uint32_t* sorted_idx, const uint32_t* pos_of, const int32_t* src,
+                   uint64_t np, uint32_t* in_pos, int32_t* in_src, hipStream_t st);
+hipError_t gen_fixk(const GenProjArgs& g, hipStream_t st);
+hipError_t gen_bern(const GenProjArgs& g, uint32_t* row_cnt, const int64_t* row_off, bool fill,
+                    hipStream_t st);
+
+}  // namespace mgb

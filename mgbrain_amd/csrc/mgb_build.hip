@@ -1,0 +1,332 @@
+// mgb_build.hip -- device-side network construction (gfx950).
+//
+// Replaces the host staging of MGSimulator::build_multinet (mgsim.cpp:8-144) and
+// copy_subnet_gpu (mgnet.cpp:92-146) for npart = 1:
+//   k_stats       validates ids, quantises delays round(delay/STEP) (mgsim.cpp:49),
+//                 and gathers per-target class statistics;
+//   radix sort    stable LSD sort of (source, payload) -- the reference's CSR keeps
+//                 each source's synapses in synapse-list order (mgsim.cpp:56,
+//                 114-131), which a stable sort by source reproduces exactly;
+//   k_bounds      row_ptr by lower_bound on the sorted keys (axon_offs);
+//   k_class_keys  CLASS-format payload (target | delay-1 | sign) sorted with the key;
+//   k_gather_wide per-slot arrays of the wide formats;
+//   k_gen_*       the new Bernoulli / fixed in-degree connectors generated on the
+//                 device, bit-identical to the host generator (same Philox draws).
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "mgb_build.h"
+#include "mgb_rng.h"
+
+namespace mgb {
+namespace {
+
+__device__ __forceinline__ bool wave_all_equal(uint32_t v) {
+    const uint32_t first = __builtin_amdgcn_readfirstlane(v);
+    return __all(v == first);
+}
+
+__global__ void k_stats(StatArgs a) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t total = a.S;
+    // every lane of a wave runs the same number of iterations (wave-uniform loop)
+    const uint64_t iters = (total + stride - 1) / stride;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t it = 0; it < iters; ++it, i += stride) {
+        const bool in = i < total;
+        uint32_t key = 0xFFFFFFFFu;    // (target, class) of this lane; ~0 = none
+        if (in) {
+            const int32_t s = a.src[i], t = a.tar[i];
+            if ((uint32_t)s >= (uint32_t)a.n || (uint32_t)t >= (uint32_t)a.n) {
+                atomicOr(a.flags, kFlagBadId);
+            } else {
+                const float q = a.delay[i] / a.step;                 // mgsim.cpp:49
+                const int d = (int)roundf(q);
+                if (d < 1) atomicOr(a.flags, kFlagDelayLow);
+                if (d > kWideMaxDelay) atomicOr(a.flags, kFlagDelayHigh);
+                const int dc = d < 0 ? 0 : (d > 255 ? 255 : d);
+                a.dq[i] = (uint8_t)dc;
+                atomicMin(a.min_d, dc);
+                atomicMax(a.max_d, dc);
+                const float w = a.weight[i];
+                const uint32_t cls = (w > 0.f) ? 0u : 1u;            // mgcore.cu:292-295
+                const uint32_t wb = __float_as_uint(w);
+                uint32_t* wc = a.wcls + (size_t)cls * (size_t)a.n + (size_t)t;
+                const uint32_t cur = *wc;
+                if (cur != wb) {
+                    const uint32_t old = atomicCAS(wc, kWSentinel, wb);
+                    if (old != kWSentinel && old != wb) atomicOr(a.flags, kFlagMixed);
+                }
+                atomicMax(a.absmax, __float_as_uint(fabsf(w)));
+                if (a.plastic && a.plastic[i]) {
+                    atomicAdd(a.n_plastic, 1ull);
+                    atomicAdd(&a.in_pl[t], 1u);
+                }
+                key = ((uint32_t)t << 1) | cls;
+            }
+        }
+        // in-degree per (target, class); target-major lists hit one address per wave
+        if (wave_all_equal(key)) {
+            if (key != 0xFFFFFFFFu && (threadIdx.x & 63u) == 0u) {
+                const uint32_t c = (uint32_t)__popcll(__ballot(true));
+                atomicAdd(&a.indeg[(size_t)(key & 1u) * (size_t)a.n + (key >> 1)], c);
+            }
+        } else if (key != 0xFFFFFFFFu) {
+            atomicAdd(&a.indeg[(size_t)(key & 1u) * (size_t)a.n + (key >> 1)], 1u);
+        }
+    }
+}
+
+__global__ void k_iota_keys(const int32_t* src, uint32_t* keys, uint32_t* vals, uint64_t S) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        keys[i] = (uint32_t)src[i];
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_class_keys(const int32_t* src, const int32_t* tar, const float* w,
+                             const uint8_t* dq, uint32_t* keys, uint32_t* vals, uint64_t S) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        keys[i] = (uint32_t)src[i];
+        const uint32_t inh = (w[i] > 0.f) ? 0u : 1u;
+        vals[i] = (uint32_t)tar[i] | ((uint32_t)(dq[i] - 1u) << 26) | (inh << 31);
+    }
+}
+
+__global__ void k_bounds(const uint32_t* keys, uint64_t S, int32_t n, int64_t* out) {
+    const int32_t m = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (m > n) return;
+    uint64_t lo = 0, hi = S;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < (uint32_t)m) lo = mid + 1;
+        else hi = mid;
+    }
+    out[m] = (int64_t)lo;
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+    const uint32_t u = __float_as_uint(f);
+    const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+
+// FP32 -> OCP FP8 E4M3 (round to nearest even, saturate to +-448).
+__device__ __forceinline__ uint8_t f32_to_fp8e4m3(float f) {
+    const uint32_t sgn = (__float_as_uint(f) >> 24) & 0x80u;
+    float a = fabsf(f);
+    if (!(a == a)) return (uint8_t)(sgn | 0x7Fu);
+    if (a >= 448.f) return (uint8_t)(sgn | 0x7Eu);
+    if (a < 0x1p-6f) {                                  // subnormal grid 2^-9
+        const float q = rintf(a * 0x1p9f);
+        const uint32_t m = (uint32_t)q;                 // 0..8 (8 -> min normal)
+        return (uint8_t)(sgn | (m >= 8u ? 0x08u : m));
+    }
+    const uint32_t u = __float_as_uint(a);
+    const uint32_t r = u + 0x7FFFFu + ((u >> 20) & 1u);   // RNE to 3 mantissa bits
+    const uint32_t e = (r >> 23) - 120u, m = (r >> 20) & 7u;
+    if (e > 15u || (e == 15u && m == 7u)) return (uint8_t)(sgn | 0x7Eu);
+    return (uint8_t)(sgn | (e << 3) | m);
+}
+
+__global__ void k_gather_wide(GatherArgs a) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.S;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = a.idx[k];
+        a.col[k] = a.tar[i];
+        const bool pl = a.plastic && a.plastic[i];
+        a.dly[k] = (uint8_t)(a.dq[i] | (pl ? 0x80u : 0u));
+        const float w = a.weight[i];
+        a.w32[k] = w;
+        if (a.w16) a.w16[k] = f32_to_bf16_rne(w);
+        if (a.w8) a.w8[k] = f32_to_fp8e4m3(w / a.row_scale[a.src[i]]);
+        if (a.pos_of) a.pos_of[i] = (uint32_t)k;
+    }
+}
+
+__global__ void k_row_absmax(const int32_t* src, const float* w, uint64_t S, uint32_t* rmax) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        atomicMax(&rmax[src[i]], __float_as_uint(fabsf(w[i])));
+}
+
+// FP8 scale of each source row: its largest |w| maps onto E4M3's largest normal.
+__global__ void k_row_scale(const uint32_t* rmax, int32_t n, float* scale) {
+    const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const float m = __uint_as_float(rmax[i]);
+    scale[i] = m > 0.f ? m / 448.f : 1.f;
+}
+
+__global__ void k_in_keys(const int32_t* tar, const uint8_t* plastic, int32_t n, uint32_t* keys,
+                          uint32_t* vals, uint64_t S) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        keys[i] = plastic[i] ? (uint32_t)tar[i] : (uint32_t)n;
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_in_fill(const uint32_t* sorted_idx, const uint32_t* pos_of, const int32_t* src,
+                          uint64_t np, uint32_t* in_pos, int32_t* in_src) {
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = sorted_idx[p];
+        in_pos[p] = pos_of[i];
+        in_src[p] = src[i];
+    }
+}
+
+// Fixed in-degree: list index base + n*K + k, one Philox call per 4 synapses.
+__global__ void k_gen_fixk(GenProjArgs g) {
+    const uint64_t total = (uint64_t)g.tn * (uint64_t)g.k;
+    const uint64_t nq = (total + 3) >> 2;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+         q += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k0 = (uint32_t)g.seed, k1 = (uint32_t)(g.seed >> 32);
+        const U32x4 o = philox4x32_10((uint32_t)q, (uint32_t)(q >> 32), g.index, kTagFixK, k0, k1);
+        U32x4 ow = {{0u, 0u, 0u, 0u}}, od = {{0u, 0u, 0u, 0u}};
+        if (g.wmin != g.wmax) ow = philox4x32_10((uint32_t)q, (uint32_t)(q >> 32), g.index, kTagFixKW, k0, k1);
+        if (g.dmin != g.dmax) od = philox4x32_10((uint32_t)q, (uint32_t)(q >> 32), g.index, kTagFixKD, k0, k1);
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint64_t f = 4 * q + j;
+            if (f >= total) break;
+            const uint64_t n = f / (uint64_t)g.k;
+            const uint32_t m = (uint32_t)(((uint64_t)pick4(o, j) * (uint64_t)g.sn) >> 32);
+            const uint64_t at = g.base + f;
+            g.src[at] = g.s0 + (int32_t)m;
+            g.tar[at] = g.t0 + (int32_t)n;
+            g.weight[at] = grid_weight(pick4(ow, j), g.wmin, g.wmax, g.dw);
+            g.delay[at] = grid_delay(pick4(od, j), g.dmin, g.dmax, g.dt);
+            g.plastic[at] = (uint8_t)g.plastic_flag;
+        }
+    }
+}
+
+// Bernoulli: one wave per source row, lanes over targets (row-major list order).
+template <bool kFill>
+__global__ void k_gen_bern(GenProjArgs g, uint32_t* row_cnt, const int64_t* row_off) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t k0 = (uint32_t)g.seed, k1 = (uint32_t)(g.seed >> 32);
+    for (uint32_t m = wave; m < (uint32_t)g.sn; m += nwaves) {
+        uint64_t at = kFill ? (uint64_t)row_off[m] : 0ull;
+        uint32_t count = 0;
+        for (uint32_t nb = 0; nb < (uint32_t)g.tn; nb += 64) {
+            const uint32_t n = nb + lane;
+            const uint64_t f = (uint64_t)m * (uint64_t)g.tn + n;
+            bool hit = false;
+            if (n < (uint32_t)g.tn) hit = (uint64_t)philox_pick(g.seed, f, g.index, kTagBern) < g.thr;
+            const uint64_t mask = __ballot(hit);
+            if (kFill && hit) {
+                const uint64_t p = g.base + at + (uint64_t)__builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                g.src[p] = g.s0 + (int32_t)m;
+                g.tar[p] = g.t0 + (int32_t)n;
+                g.weight[p] = grid_weight(philox_pick(g.seed, f, g.index, kTagBernW), g.wmin, g.wmax, g.dw);
+                g.delay[p] = grid_delay(philox_pick(g.seed, f, g.index, kTagBernD), g.dmin, g.dmax, g.dt);
+                g.plastic[p] = (uint8_t)g.plastic_flag;
+            }
+            at += (uint64_t)__popcll(mask);
+            count += (uint32_t)__popcll(mask);
+        }
+        if (!kFill && lane == 0) row_cnt[m] = count;
+    }
+    (void)k0; (void)k1;
+}
+
+}  // namespace
+
+int blocks_for(uint64_t work, int block) {
+    uint64_t b = (work + (uint64_t)block - 1) / (uint64_t)block;
+    if (b < 1) b = 1;
+    if (b > 65536) b = 65536;
+    return (int)b;
+}
+
+hipError_t run_stats(const StatArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_stats, dim3(blocks_for(a.S, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t make_sort_input(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
+                           bool class_payload, uint32_t* keys, uint32_t* vals, uint64_t S,
+                           hipStream_t st) {
+    if (S == 0) return hipSuccess;
+    if (class_payload)
+        hipLaunchKernelGGL(k_class_keys, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, tar, w, dq,
+                           keys, vals, S);
+    else
+        hipLaunchKernelGGL(k_iota_keys, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, keys, vals, S);
+    return hipGetLastError();
+}
+
+hipError_t fp8_row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n,
+                          uint32_t* rmax_tmp, float* scale, hipStream_t st) {
+    hipError_t e = hipMemsetAsync(rmax_tmp, 0, 4 * (size_t)n, st);
+    if (e != hipSuccess) return e;
+    if (S) hipLaunchKernelGGL(k_row_absmax, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, w, S, rmax_tmp);
+    hipLaunchKernelGGL(k_row_scale, dim3((n + 255) / 256), dim3(256), 0, st, rmax_tmp, n, scale);
+    return hipGetLastError();
+}
+
+hipError_t make_in_keys(const int32_t* tar, const uint8_t* plastic, int32_t n, uint32_t* keys,
+                        uint32_t* vals, uint64_t S, hipStream_t st) {
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_in_keys, dim3(blocks_for(S, 256)), dim3(256), 0, st, tar, plastic, n, keys,
+                       vals, S);
+    return hipGetLastError();
+}
+
+hipError_t sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* kin, uint32_t* kout,
+                      const uint32_t* vin, uint32_t* vout, uint64_t S, int bits, hipStream_t st) {
+    size_t sz = temp ? *temp_bytes : 0;
+    hipError_t e = rocprim::radix_sort_pairs(temp, sz, kin, kout, vin, vout, (size_t)S, 0u,
+                                             (unsigned)bits, st);
+    if (!temp) *temp_bytes = sz;
+    return e;
+}
+
+hipError_t row_bounds(const uint32_t* sorted_keys, uint64_t S, int32_t n, int64_t* out,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_bounds, dim3((n + 1 + 255) / 256), dim3(256), 0, st, sorted_keys, S, n, out);
+    return hipGetLastError();
+}
+
+hipError_t gather_wide(const GatherArgs& a, hipStream_t st) {
+    if (a.S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_wide, dim3(blocks_for(a.S, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t in_fill(const uint32_t* sorted_idx, const uint32_t* pos_of, const int32_t* src,
+                   uint64_t np, uint32_t* in_pos, int32_t* in_src, hipStream_t st) {
+    if (np == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_in_fill, dim3(blocks_for(np, 256)), dim3(256), 0, st, sorted_idx, pos_of,
+                       src, np, in_pos, in_src);
+    return hipGetLastError();
+}
+
+hipError_t gen_fixk(const GenProjArgs& g, hipStream_t st) {
+    const uint64_t nq = ((uint64_t)g.tn * (uint64_t)g.k + 3) >> 2;
+    if (nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gen_fixk, dim3(blocks_for(nq, 256)), dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+hipError_t gen_bern(const GenProjArgs& g, uint32_t* row_cnt, const int64_t* row_off, bool fill,
+                    hipStream_t st) {
+    if (g.sn == 0) return hipSuccess;
+    const int grid = blocks_for((uint64_t)g.sn * 64ull, 256);
+    if (fill)
+        hipLaunchKernelGGL(k_gen_bern<true>, dim3(grid), dim3(256), 0, st, g, row_cnt, row_off);
+    else
+        hipLaunchKernelGGL(k_gen_bern<false>, dim3(grid), dim3(256), 0, st, g, row_cnt, row_off);
+    return hipGetLastError();
+}
+
+}  // namespace mgb

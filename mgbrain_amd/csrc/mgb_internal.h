@@ -1,0 +1,139 @@
+// mgb_internal.h -- device data layout and kernel launchers of the HIP step.
+//
+// HBM layout of one simulator (all device arrays, SoA, neuron ids 0..n-1 in the
+// reference's id order; npad = n rounded up to 256):
+//   neurons   type u8, cset u8, tbl u16, stream u16, rate f32, V f32, Ie f32,
+//             Ii f32, refr i32, last i32, cnt i32                    [npad]
+//   CSR       row_ptr i64 [n+1]; CLASS format: syn u32 [S] = target (26 bits)
+//             | delay-1 (5 bits) | inhibitory bit; wide formats: col i32 [S],
+//             dly u8 [S] (bit 7 = plastic), w f32|bf16|fp8 [S]
+//   STDP      in_ptr i64 [n+1], in_pos u32 [Sp], in_src i32 [Sp] (plastic only)
+//   ring      CLASS: u32 [D][npad] (excitatory count | inhibitory count << 16);
+//             wide: i32 [D][npad] twice (fixed-point excitatory / inhibitory)
+//   window    spike lists u32 [2][npad*window] (id | step-in-window << 26)
+//   record    u64 [cap] (step << 32 | id)
+// The ring is slot-major so the neuron kernel reads one coalesced row per step.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mgbrain_hip.h"
+
+namespace mgb {
+
+void set_error(const char* fmt, ...);
+
+#define MGB_HIP_TRY(call)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::mgb::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                                    \
+            return e_ == hipErrorOutOfMemory ? MGB_ERR_OOM : MGB_ERR_HIP;                 \
+        }                                                                                  \
+    } while (0)
+
+constexpr int kNeuBlock = 256;
+constexpr int kMaxWindow = 63;        // step-in-window field is 6 bits
+constexpr uint32_t kIdMask = 0x3FFFFFFu;
+constexpr int kMaxNeurons = 1 << 26;
+constexpr int kClassMaxDelay = 32;    // 5-bit delay-1 field
+constexpr int kWideMaxDelay = 127;    // 7-bit delay field
+
+// Device control block (one per simulator).
+struct Ctl {
+    int64_t base_step;      // first step of the current launch block
+    int64_t base_window;    // windows issued before the block (spike-list parity)
+    uint32_t win_cnt[2];    // spikes appended to each window list
+    int32_t rec_overflow;
+    int32_t pad;
+    uint64_t rec_cnt;       // spikes recorded (may exceed capacity)
+    uint64_t se;            // synaptic events
+};
+
+// Constant set: LIFECONSTS[30] (mgcore.cu:32-63) + refractory steps in c[30].
+struct CSet {
+    float c[32];
+};
+
+struct NeuArgs {
+    int32_t n, npad, D;
+    const uint8_t* type;
+    const uint8_t* cset;
+    const uint16_t* tbl;
+    const uint16_t* stream;
+    const float* rate;
+    float* V;
+    float* Ie;
+    float* Ii;
+    int32_t* refr;
+    int32_t* last;
+    int32_t* cnt;
+    const CSet* csets;
+    const int32_t* stream_base;
+    const uint64_t* stream_seed;
+    // CLASS ring + class tables
+    uint32_t* ring;
+    const uint32_t* tbl_off;   // [n_tbl][2]: offsets of exc / inh seq-sum tables
+    const float* seq;          // seq[off + k] = k-fold sequential FP32 sum of the class weight
+    // fixed-point ring
+    int32_t* ring_e;
+    int32_t* ring_i;
+    float inv_e, inv_i;
+    Ctl* ctl;
+    uint32_t* win_buf;
+    uint32_t win_stride;
+};
+
+struct PropArgs {
+    int32_t n, npad, D;
+    const int64_t* row_ptr;
+    const uint32_t* syn;       // CLASS format
+    const int32_t* col;        // wide formats
+    const uint8_t* dly;
+    const float* w32;
+    const uint16_t* w16;
+    const uint8_t* w8;
+    const float* row_scale;    // FP8: per-source scale
+    uint32_t* ring;
+    int32_t* ring_e;
+    int32_t* ring_i;
+    float scale_e, scale_i;
+    Ctl* ctl;
+    const uint32_t* win_buf;
+    uint32_t win_stride;
+    uint64_t* rec;
+    uint64_t rec_cap;
+};
+
+struct StdpArgs {
+    int32_t n;
+    const int64_t* row_ptr;
+    const int32_t* col;
+    const uint8_t* dly;
+    float* w32;
+    const int64_t* in_ptr;
+    const uint32_t* in_pos;
+    const int32_t* in_src;
+    const int32_t* last;
+    const float* ltp;          // ltp[k] = A_LTP * expf(-k / TAU_LTP), k >= 1
+    const float* ltd;          // ltd[k] = A_LTD * expf(-k / TAU_LTD)
+    int32_t lut_n;
+    float w_max, w_min;
+    Ctl* ctl;
+    const uint32_t* win_buf;
+    uint32_t win_stride;
+};
+
+enum RingMode { kRingClass = 0, kRingFixed = 1 };
+enum WideKind { kWideF32 = 0, kWideBF16 = 1, kWideFP8 = 2 };
+
+// sim kernels (mgb_kernels.hip)
+hipError_t launch_neurons(int mode, const NeuArgs& a, int32_t off, int32_t wk, int32_t nsteps,
+                          hipStream_t st);
+hipError_t launch_prop(int mode, int wide_kind, const PropArgs& a, int32_t off, int32_t wk,
+                       int grid, int block, hipStream_t st);
+hipError_t launch_stdp(const StdpArgs& a, int32_t off, int32_t wk, int grid, hipStream_t st);
+hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
+
+}  // namespace mgb

@@ -369,6 +369,18 @@ float orc_poisson_u(uint64_t seed, uint32_t id, uint32_t step) {
     return (float)((x >> 8) + 1u) * 0x1p-24f;              /* (0, 1] like curand_uniform */
 }
 
+float orc_vinit(uint64_t seed, uint32_t local_id, float lo, float hi) {
+    if (hi == lo) return lo;
+    uint32_t ctr[4] = {local_id, 0u, 0u, 0x56494E49u /* "VINI" */};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t out[4];
+    orc_philox4x32_10(ctr, key, out);
+    float u = (float)(out[0] >> 8) * 0x1p-24f;
+    float span = hi - lo;
+    float t = u * span;
+    return lo + t;
+}
+
 /* mgcore.cu:346-364 with glibc expf in place of CUDA's expf */
 float orc_stdp_dw(const float* sc, int32_t dt) {
     if (dt < 0) return sc[0] * expf((float)dt / sc[2]);

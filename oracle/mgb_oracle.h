@@ -142,6 +142,10 @@ void orc_destroy(orc_sim* s);
 
 /* Poisson uniform in (0,1] of neuron id at step s (bit-identical on the GPU). */
 float orc_poisson_u(uint64_t seed, uint32_t id, uint32_t step);
+/* Initial membrane potential of neuron local_id of a Poisson/network stream,
+ * uniform on [lo, hi) with 24 random bits (new feature; the reference starts
+ * every neuron at LIFECONSTS[1], mgsim.cpp:97). */
+float orc_vinit(uint64_t seed, uint32_t local_id, float lo, float hi);
 /* STDP weight change for integer step difference dt (mgcore.cu:346-364). */
 float orc_stdp_dw(const float* stdpconst, int32_t dt);
 

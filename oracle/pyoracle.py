@@ -102,6 +102,8 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_poisson_u.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
         L.orc_poisson_u.restype = C.c_float
+        L.orc_vinit.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.c_float]
+        L.orc_vinit.restype = C.c_float
         L.orc_stdp_dw.argtypes = [_f32p, C.c_int32]
         L.orc_stdp_dw.restype = C.c_float
         _lib = L
@@ -137,6 +139,11 @@ def glibc_rand(seed, count):
 
 def poisson_u(seed, nid, step):
     return float(lib().orc_poisson_u(seed, nid, step))
+
+
+def vinit(seed, local_ids, lo, hi):
+    f = lib().orc_vinit
+    return np.array([f(seed, int(i), lo, hi) for i in local_ids], dtype=np.float32)
 
 
 def stdp_dw(stdpconst, dt):

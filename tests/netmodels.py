@@ -1,0 +1,179 @@
+"""Model descriptions used by the tests (pure Python, no GPU).
+
+brunel_ref / vogel_ref restate ModelMaker::make_brunel / make_vogel
+(model/maker.h:10-42) for the reference connectors; config1..config5 are the
+workloads of the benchmark, described with the new connectors (Bernoulli,
+fixed in-degree) that the device generator and the oracle share.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+RAW, LIF0, POISSON, LIF, LIFE, LIFB = range(6)
+BERN, FIXK = 1, 2
+f32 = np.float32
+
+
+def gen_life_const(v_init, v_rest, v_reset, cm, tau_m, tau_refrac, tau_syn_E, tau_syn_I, v_thresh, i_offset,
+                   dt=1e-4):
+    """constgen.h:9-107 in float32 (expf via numpy float32)."""
+    dt = f32(dt)
+    v_init, v_rest, v_reset, cm, tau_m, tau_refrac = map(f32, (v_init, v_rest, v_reset, cm, tau_m, tau_refrac))
+    tau_syn_E, tau_syn_I, v_thresh, i_offset = map(f32, (tau_syn_E, tau_syn_I, v_thresh, i_offset))
+    rm = f32(1.0)
+    if abs(float(cm)) > 0.00001:
+        rm = f32(tau_m / cm)
+    Cm = np.exp(f32(-dt / tau_m), dtype=f32) if tau_m > 0 else f32(0.0)
+    CE = np.exp(f32(-dt / tau_syn_E), dtype=f32) if tau_syn_E > 0 else f32(0.0)
+    CI = np.exp(f32(-dt / tau_syn_I), dtype=f32) if tau_syn_I > 0 else f32(0.0)
+    io = f32(0.0)  # the reference reads an uninitialised _i_offset; all callers pass 0
+    v_tmp = f32(io * rm + v_rest)
+    v_tmp = f32(v_tmp * f32(f32(1) - Cm))
+    C_E = f32(f32(rm * tau_syn_E) / f32(tau_syn_E - tau_m))
+    C_I = f32(f32(rm * tau_syn_I) / f32(tau_syn_I - tau_m))
+    C_E = f32(C_E * f32(CE - Cm))
+    C_I = f32(C_I * f32(CI - Cm))
+    c = np.zeros(30, dtype=f32)
+    c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8] = dt, v_init, v_rest, v_reset, v_thresh, cm, tau_m, i_offset, tau_refrac
+    c[10], c[11], c[12], c[13], c[14], c[15], c[16], c[17] = tau_syn_E, tau_syn_I, C_E, C_I, v_tmp, CE, CI, Cm
+    c[21], c[22], c[23], c[24], c[25] = f32(0.951229453), f32(0.967216074), f32(0.0799999982), f32(0.0599999987), f32(0.90483743)
+    return c
+
+
+def life_const1(dt=1e-4):
+    return gen_life_const(-0.06, -0.06, -0.06, 0, 0, 0.005, 1, 1, -0.05, 0, dt=dt)
+
+
+def life_const2(dt=1e-4):
+    return gen_life_const(0.0, 0.0, 0.0, 0.0, 0.0, 0.002, 1.0, 1.0, 0.02, 0.0, dt=dt)
+
+
+def default_consts(dt=1e-4):
+    """copy_consts_gpu(max_delay, dt) defaults (mgcore.cu:685-733): V_THRESH = 15."""
+    c = np.zeros(30, dtype=f32)
+    c[0], c[4], c[5], c[6], c[8], c[10], c[11] = dt, 15.0, 0.25, 10.0, 0.0004, 1.0, 1.0
+    c[21], c[22], c[23], c[24], c[25] = 0.951229453, 0.967216074, 0.0799999982, 0.0599999987, 0.90483743
+    return c
+
+
+STDP_DEFAULT = np.array([0.1, -0.01, 17, 34, 40, 0], dtype=f32)
+
+
+class Net:
+    """Populations + projections + constants of one network."""
+
+    def __init__(self, pops, projs, lifconst, name=""):
+        self.pops = pops      # list of (num, type, source)
+        self.projs = projs    # list of dicts (oracle Proj fields + kind)
+        self.lifconst = lifconst
+        self.name = name
+        self.rates = [0.0] * len(pops)
+        self.vrange = [None] * len(pops)
+
+    @property
+    def n(self):
+        return sum(p[0] for p in self.pops)
+
+    def first(self):
+        out, acc = [], 0
+        for p in self.pops:
+            out.append(acc)
+            acc += p[0]
+        return out
+
+    def neuron_arrays(self):
+        types = np.concatenate([np.full(p[0], p[1], dtype=np.int32) for p in self.pops])
+        rate = np.concatenate([np.full(p[0], r, dtype=f32) for p, r in zip(self.pops, self.rates)])
+        return types, rate
+
+    def gen_pops(self, stream=0):
+        out = []
+        v1 = float(self.lifconst[1])
+        for p, r, vr in zip(self.pops, self.rates, self.vrange):
+            lo, hi = (v1, v1) if vr is None else vr
+            out.append(dict(num=p[0], type=p[1], rate=r, v_min=lo, v_max=hi, cset=0, stream=stream))
+        return out
+
+
+def brunel_ref(nsyn, rate=0.002):
+    """ModelMaker::make_brunel (maker.h:10-27): quota connector, p = 0.1."""
+    cp = f32(0.1)
+    pop = int(math.sqrt(float(f32(nsyn)) / (float(cp) * 0.5)))
+    pops = [(pop * 5 // 10, POISSON, 1), (pop * 4 // 10, LIFB, 0), (pop * 1 // 10, LIFB, 0)]
+    w1 = float(f32(0.0001 * 20000 / pop))
+    w2 = float(f32(-0.0005 * 20000 / pop))
+    d1 = float(f32(0.0016))
+    pr = [dict(src=s, tar=t, w=w, d=d1, ctype=float(cp), kind=0)
+          for s, t, w in ((0, 1, w1), (0, 2, w1), (1, 1, w1), (1, 2, w1), (2, 2, w2), (2, 1, w2))]
+    net = Net(pops, pr, life_const2(), name=f"brunel_ref_{nsyn}")
+    net.rates = [rate, 0.0, 0.0]
+    return net
+
+
+def vogel_ref(nsyn):
+    """ModelMaker::make_vogel (maker.h:28-42)."""
+    cp = f32(0.02)
+    pop = int(math.sqrt(float(f32(nsyn)) / float(cp)))
+    pops = [(pop * 8 // 10, LIFE, 0), (pop * 2 // 10, LIFE, 0)]
+    w1 = float(f32(0.4 * 16000000 / pop / pop))
+    w2 = float(f32(-5.1 * 16000000 / pop / pop))
+    d1 = float(f32(0.0008))
+    pr = [dict(src=s, tar=t, w=w, d=d1, ctype=float(cp), kind=0)
+          for s, t, w in ((0, 0, w1), (0, 1, w1), (1, 0, w2), (1, 1, w2))]
+    return Net(pops, pr, life_const1(), name=f"vogel_ref_{nsyn}")
+
+
+def config1(scale=1.0, p=0.1, poisson_rate=0.002):
+    """Brunel 8000 E + 2000 I LIFB + 1000 Poisson, Bernoulli p, J = 0.1 mV, g = 5, 1.4 ms."""
+    nP, nE, nI = int(1000 * scale), int(8000 * scale), int(2000 * scale)
+    J, g, d = float(f32(1e-4)), 5.0, float(f32(0.0014))
+    wi = float(f32(-g * 1e-4))
+    pops = [(nP, POISSON, 1), (nE, LIFB, 0), (nI, LIFB, 0)]
+    pr = [dict(src=s, tar=t, w=w, d=d, kind=BERN, p=p)
+          for s, t, w in ((0, 1, J), (0, 2, J), (1, 1, J), (1, 2, J), (2, 1, wi), (2, 2, wi))]
+    lc = gen_life_const(0.0, 0.0, 0.010, 0.0, 0.0, 0.002, 1.0, 1.0, 0.020, 0.0)
+    net = Net(pops, pr, lc, name=f"config1_x{scale}")
+    net.rates = [poisson_rate, 0.0, 0.0]
+    return net
+
+
+def config2(scale=1.0, K=10000, poisson_rate=0.0015, stdp=False):
+    """Brunel 80k E + 20k I LIFB + 10k Poisson, fixed in-degree K split by source size,
+    delays on the 1.0-2.0 ms grid (11 values)."""
+    nP, nE, nI = int(10000 * scale), int(80000 * scale), int(20000 * scale)
+    ntot = nP + nE + nI
+    Ks = [int(round(K * n / ntot)) for n in (nP, nE, nI)]
+    J = float(f32(1e-4))
+    wi = float(f32(-5e-4))
+    dmin, dmax = float(f32(0.001)), float(f32(0.002))
+    pops = [(nP, POISSON, 1), (nE, LIFB, 0), (nI, LIFB, 0)]
+    pr = []
+    for tgt in (1, 2):
+        for src, k, w in ((0, Ks[0], J), (1, Ks[1], J), (2, Ks[2], wi)):
+            pr.append(dict(src=src, tar=tgt, wmin=w, wmax=w, dmin=dmin, dmax=dmax, kind=FIXK, k=k,
+                           plastic=1 if (stdp and src == 1 and tgt == 1) else 0))
+    lc = gen_life_const(0.0, 0.0, 0.010, 0.0, 0.0, 0.002, 1.0, 1.0, 0.020, 0.0)
+    net = Net(pops, pr, lc, name=f"config2_x{scale}")
+    net.rates = [poisson_rate, 0.0, 0.0]
+    return net
+
+
+def config3(scale=1.0, p=0.02):
+    """Vogels-Abbott COBA: 64k E + 16k I LIFE, Bernoulli p = 0.02, g_exc 0.4, g_inh 5.1 (reference
+    dimensionless units, maker.h:34-35), delay 0.8 ms, V0 ~ U[-60, -50] mV."""
+    nE, nI = int(64000 * scale), int(16000 * scale)
+    d = float(f32(0.0008))
+    pops = [(nE, LIFE, 0), (nI, LIFE, 0)]
+    we, wi = float(f32(0.4)), float(f32(-5.1))
+    pr = [dict(src=s, tar=t, w=w, d=d, kind=BERN, p=p) for s, t, w in ((0, 0, we), (0, 1, we), (1, 0, wi), (1, 1, wi))]
+    net = Net(pops, pr, life_const1(), name=f"config3_x{scale}")
+    net.vrange = [(float(f32(-0.060)), float(f32(-0.050)))] * 2
+    return net
+
+
+def stdp_consts_config4(J=1e-4):
+    """A+ = 0.01, A- = 0.0105 in units of W_max = 2J; tau = 20 ms = 200 steps; W in [0, 2J]."""
+    wmax = 2 * J
+    return np.array([0.01 * wmax, -0.0105 * wmax, 200.0, 200.0, wmax, 0.0], dtype=f32)
