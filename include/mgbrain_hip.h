@@ -59,7 +59,8 @@ typedef struct mgb_opts {
     int64_t record_capacity;   /* spike records kept on the device; 0 = none */
     int32_t prop_block;        /* propagation workgroup size (64..1024); 0 = tuned default */
     int32_t prop_grid;         /* propagation workgroups; 0 = tuned default */
-    int32_t reserved[8];
+    int32_t profile;           /* time every kernel with hipEvents (direct launches only) */
+    int32_t reserved[7];
 } mgb_opts;
 
 /* A network as the reference's Network holds it (model/network.h:12-68).
@@ -148,7 +149,11 @@ typedef struct mgb_info {
     float fix_scale_inh;
     int64_t device_bytes;      /* device memory held by the handle */
     double last_run_ms;        /* device time of the last mgb_run (hipEvents) */
-    int64_t launches;          /* kernel launches issued so far (graph nodes count once) */
+    int64_t launches;          /* kernel launches issued so far (a graph replay counts once) */
+    int64_t spikes;            /* spikes emitted so far */
+    /* per-kernel device time of the last mgb_run when opts.profile is set */
+    double neuron_ms, prop_ms, stdp_ms;
+    int64_t neuron_launches, prop_launches, stdp_launches;
 } mgb_info;
 
 typedef struct mgb_sim mgb_sim;

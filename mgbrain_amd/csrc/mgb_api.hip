@@ -92,6 +92,15 @@ struct mgb_sim {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int32_t graph_windows = 0;
+    // per-kernel profiling (opts.profile)
+    int profile = 0;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    struct Span {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Span> spans;
     // bookkeeping
     std::vector<void*> allocs;
     int64_t bytes = 0;
@@ -166,6 +175,8 @@ int bits_for(uint64_t v) {   // bits to represent 0..v
 }
 
 void free_all(mgb_sim* s) {
+    for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+    s->ev_pool.clear();
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
     for (void* p : s->allocs) (void)hipFree(p);
@@ -592,6 +603,7 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
         MGB_HIP_TRY(hipMemcpyAsync(s->ltd, ltd.data(), 4 * ltd.size(), hipMemcpyHostToDevice, st));
     }
     s->use_graph = o ? o->use_graph : 0;
+    s->profile = o ? o->profile : 0;
     s->prop_block = (o && o->prop_block >= 64 && o->prop_block <= 1024) ? (o->prop_block / 64) * 64 : 256;
     s->prop_grid = (o && o->prop_grid > 0) ? o->prop_grid : 0;
     MGB_HIP_TRY(hipStreamSynchronize(st));
@@ -643,20 +655,64 @@ int prop_grid(const mgb_sim* s, int32_t nst) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
 }
 
-// Issues the kernels of `windows` windows starting `off` steps after base_step.
-mgb_status issue_windows(mgb_sim* s, int32_t nsteps, bool finalize) {
+enum { kSpanNeuron = 0, kSpanProp = 1, kSpanStdp = 2 };
+
+mgb_status next_event(mgb_sim* s, hipEvent_t* e) {
+    if (s->ev_used == s->ev_pool.size()) {
+        hipEvent_t ne = nullptr;
+        MGB_HIP_TRY(hipEventCreate(&ne));
+        s->ev_pool.push_back(ne);
+    }
+    *e = s->ev_pool[s->ev_used++];
+    return MGB_OK;
+}
+
+// Brackets one kernel launch with events on the handle's stream when profiling.
+struct SpanGuard {
+    mgb_sim* s;
+    int kind;
+    bool on;
+    hipEvent_t a = nullptr;
+    mgb_status begin() {
+        if (!on) return MGB_OK;
+        MGB_TRY(next_event(s, &a));
+        MGB_HIP_TRY(hipEventRecord(a, s->stream));
+        return MGB_OK;
+    }
+    mgb_status end() {
+        if (!on) return MGB_OK;
+        hipEvent_t b = nullptr;
+        MGB_TRY(next_event(s, &b));
+        MGB_HIP_TRY(hipEventRecord(b, s->stream));
+        s->spans.push_back({kind, a, b});
+        return MGB_OK;
+    }
+};
+
+// Issues the kernels of the windows covering `nsteps` steps after base_step.
+mgb_status issue_windows(mgb_sim* s, int32_t nsteps, bool finalize, bool capturing = false) {
     const NeuArgs na = neu_args(s);
     const PropArgs pa = prop_args(s);
     const StdpArgs sa = stdp_args(s);
+    const bool prof = s->profile && !capturing;
     int32_t off = 0, wk = 0;
     while (off < nsteps) {
         const int32_t nst = std::min(s->window, nsteps - off);
+        SpanGuard g0{s, kSpanNeuron, prof};
+        MGB_TRY(g0.begin());
         MGB_HIP_TRY(launch_neurons(s->mode, na, off, wk, nst, s->stream));
+        MGB_TRY(g0.end());
+        SpanGuard g1{s, kSpanProp, prof};
+        MGB_TRY(g1.begin());
         MGB_HIP_TRY(launch_prop(s->mode, s->wide_kind, pa, off, wk, prop_grid(s, nst),
                                 s->prop_block, s->stream));
+        MGB_TRY(g1.end());
         s->launches += 2;
         if (s->stdp) {
+            SpanGuard g2{s, kSpanStdp, prof};
+            MGB_TRY(g2.begin());
             MGB_HIP_TRY(launch_stdp(sa, off, wk, prop_grid(s, nst), s->stream));
+            MGB_TRY(g2.end());
             s->launches += 1;
         }
         off += nst;
@@ -673,7 +729,7 @@ mgb_status ensure_graph(mgb_sim* s, int32_t steps) {
     if (s->gexec) return MGB_OK;
     MGB_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     const int64_t before = s->launches;
-    mgb_status st = issue_windows(s, steps, true);
+    mgb_status st = issue_windows(s, steps, true, true);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     s->launches = before;
@@ -972,6 +1028,12 @@ mgb_status mgb_run(mgb_sim* s, int32_t nsteps) {
         return MGB_ERR_INVALID;
     }
     MGB_HIP_TRY(hipSetDevice(s->device));
+    if (s->profile) {
+        // the previous run's events may still be pending; reuse the pool after it ends
+        MGB_HIP_TRY(hipStreamSynchronize(s->stream));
+        s->ev_used = 0;
+        s->spans.clear();
+    }
     MGB_HIP_TRY(hipEventRecord(s->ev0, s->stream));
     int32_t done = 0;
     if (s->use_graph) {
@@ -1025,11 +1087,26 @@ mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
     out->fix_scale_inh = s->scale_i;
     out->device_bytes = s->bytes;
     out->launches = s->launches;
+    out->spikes = (int64_t)c.rec_cnt;
     out->last_run_ms = 0.0;
     if (s->timed) {
         float ms = 0.f;
         MGB_HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
         out->last_run_ms = ms;
+    }
+    for (const auto& sp : s->spans) {
+        float ms = 0.f;
+        MGB_HIP_TRY(hipEventElapsedTime(&ms, sp.a, sp.b));
+        if (sp.kind == kSpanNeuron) {
+            out->neuron_ms += ms;
+            out->neuron_launches += 1;
+        } else if (sp.kind == kSpanProp) {
+            out->prop_ms += ms;
+            out->prop_launches += 1;
+        } else {
+            out->stdp_ms += ms;
+            out->stdp_launches += 1;
+        }
     }
     return MGB_OK;
 }

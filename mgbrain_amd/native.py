@@ -31,7 +31,7 @@ _p = C.c_void_p
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int32), ("weight_format", C.c_int32), ("window", C.c_int32),
                 ("use_graph", C.c_int32), ("record_capacity", C.c_int64), ("prop_block", C.c_int32),
-                ("prop_grid", C.c_int32), ("reserved", C.c_int32 * 8)]
+                ("prop_grid", C.c_int32), ("profile", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
 class NetDesc(C.Structure):
@@ -66,7 +66,10 @@ class Info(C.Structure):
                 ("window", C.c_int32), ("n_synapses", C.c_int64), ("step", C.c_int64),
                 ("syn_events", C.c_int64), ("spikes_recorded", C.c_int64), ("weight_format", C.c_int32),
                 ("n_classes", C.c_int32), ("fix_scale_exc", C.c_float), ("fix_scale_inh", C.c_float),
-                ("device_bytes", C.c_int64), ("last_run_ms", C.c_double), ("launches", C.c_int64)]
+                ("device_bytes", C.c_int64), ("last_run_ms", C.c_double), ("launches", C.c_int64),
+                ("spikes", C.c_int64), ("neuron_ms", C.c_double), ("prop_ms", C.c_double),
+                ("stdp_ms", C.c_double), ("neuron_launches", C.c_int64), ("prop_launches", C.c_int64),
+                ("stdp_launches", C.c_int64)]
 
 
 _lib = None
@@ -115,7 +118,7 @@ def _addr(a):
 
 
 def make_opts(weight_format=WF_AUTO, window=0, use_graph=0, record_capacity=0, device=0, prop_block=0,
-              prop_grid=0):
+              prop_grid=0, profile=0):
     o = Opts()
     o.device = device
     o.weight_format = weight_format
@@ -124,6 +127,7 @@ def make_opts(weight_format=WF_AUTO, window=0, use_graph=0, record_capacity=0, d
     o.record_capacity = record_capacity
     o.prop_block = prop_block
     o.prop_grid = prop_grid
+    o.profile = profile
     return o
 
 
