@@ -60,8 +60,20 @@ typedef struct mgb_opts {
     int32_t prop_block;        /* propagation workgroup size (64..1024); 0 = tuned default */
     int32_t prop_grid;         /* propagation workgroups; 0 = tuned default */
     int32_t profile;           /* time every kernel with hipEvents (direct launches only) */
-    int32_t reserved[7];
+    int32_t propagation;       /* MGB_PROP_* */
+    int32_t tile_size;         /* TILED: target neurons per workgroup; 0 = tuned default */
+    int32_t chunk;             /* TILED: synapses per propagation work item (4, 8, 16) */
+    int32_t reserved[4];
 } mgb_opts;
+
+/* Spike propagation strategy. */
+enum {
+    MGB_PROP_AUTO = 0,    /* TILED for the CLASS format without STDP, else DIRECT */
+    MGB_PROP_DIRECT = 1,  /* one wave per spike over its CSR row, integer atomics to the ring in HBM */
+    MGB_PROP_TILED = 2    /* one fused kernel per window: each workgroup owns a tile of targets,
+                             keeps their ring slice in LDS, adds the arriving synapses from a
+                             tile-sorted CSR copy and advances the tile's neurons */
+};
 
 /* A network as the reference's Network holds it (model/network.h:12-68).
  * Reference: the host->device seam copy_subnet_gpu (mgnet.h:31, mgnet.cpp:92-146)
@@ -154,6 +166,10 @@ typedef struct mgb_info {
     /* per-kernel device time of the last mgb_run when opts.profile is set */
     double neuron_ms, prop_ms, stdp_ms;
     int64_t neuron_launches, prop_launches, stdp_launches;
+    int32_t propagation;       /* MGB_PROP_* in use (TILED: the fused kernel's time is prop_ms) */
+    int32_t tile_size;
+    int32_t tiles;
+    int32_t block;             /* workgroup size of the propagation / fused kernel */
 } mgb_info;
 
 typedef struct mgb_sim mgb_sim;

@@ -84,6 +84,13 @@ struct mgb_sim {
     uint32_t* win_buf = nullptr;
     uint32_t win_stride = 0;
     uint64_t* rec = nullptr;
+    // fused tiled path
+    int tiled = 0;
+    int32_t B = 0, G = 0, chunk = 8, tblock = 512;
+    uint32_t* syn_t = nullptr;
+    uint16_t* offs16 = nullptr;
+    uint32_t* offs32 = nullptr;
+    uint32_t* ring_t = nullptr;
     // host mirrors for read-back
     std::vector<float> class_w;        // [n_tbl][2]
     std::vector<uint16_t> h_tbl;       // [n]
@@ -293,8 +300,32 @@ uint64_t host_u64(const void* p) {
     return v;
 }
 
+// Tile size of the fused path: ~256 tiles to fill the chip, bounded by LDS.
+mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D) {
+    int lds_max = 65536;
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
+    lds_max = std::min(lds_max, 160 * 1024);
+    int32_t bmax = (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0)) / (4 * (int64_t)D));
+    bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, 1 << 14));
+    int32_t B = (o && o->tile_size > 0) ? o->tile_size : std::max<int32_t>(256, (n + 255) / 256);
+    B = std::max<int32_t>(1, std::min(B, bmax));
+    B = std::min(B, n);
+    s->B = B;
+    s->G = (n + B - 1) / B;
+    s->chunk = (o && (o->chunk == 4 || o->chunk == 16)) ? o->chunk : 8;
+    s->tblock = (o && o->prop_block >= 64 && o->prop_block <= 1024) ? (o->prop_block / 64) * 64 : 512;
+    if ((uint64_t)n * (uint64_t)(s->G + 1) >= 0xFFFFFFFFull) {
+        set_error("tiled propagation: %d neurons x %d tiles exceed 32-bit keys", n, s->G);
+        return MGB_ERR_UNSUPPORTED;
+    }
+    const size_t lds = tiled_lds_bytes(D, B);
+    if (lds > 65536) MGB_HIP_TRY(set_tiled_lds_limit(lds));
+    return MGB_OK;
+}
+
 // Builds device CSR + tables from a device synapse list; frees nothing of `L`.
-mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int wf_req) {
+mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int wf_req,
+                         const mgb_opts* o) {
     const int32_t n = h.n;
     const uint64_t S = L.S;
     hipStream_t st = s->stream;
@@ -462,6 +493,44 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         MGB_HIP_TRY(hipMemcpyAsync(s->tbl, s->h_tbl.data(), 2 * (size_t)n, hipMemcpyHostToDevice, st));
         MGB_HIP_TRY(hipMemcpyAsync(s->tbl_off, offs.data(), 4 * offs.size(), hipMemcpyHostToDevice, st));
         MGB_HIP_TRY(hipMemcpyAsync(s->seq, seq.data(), 4 * seq.size(), hipMemcpyHostToDevice, st));
+        const int prop = o ? o->propagation : MGB_PROP_AUTO;
+        s->tiled = (prop == MGB_PROP_AUTO || prop == MGB_PROP_TILED) ? 1 : 0;
+        if (s->tiled) {
+            MGB_TRY(choose_tiles(s, o, n, s->D));
+            // tile-sorted copy: stable sort by (source, target tile) keeps list order
+            // inside every (source, tile) sub-row
+            MGB_HIP_TRY(make_tile_keys(L.src, L.tar, L.weight, dq, s->B, s->G, kin, vin, S, st));
+            const int tbits = bits_for((uint64_t)n * (uint64_t)s->G);
+            if (S > 0) {
+                size_t tb2 = 0;
+                MGB_HIP_TRY(sort_pairs(nullptr, &tb2, kin, kout, vin, vout, S, tbits, st));
+                if (tb2 > tb) {
+                    X.release(temp);
+                    MGB_TRY(X.get(reinterpret_cast<uint8_t**>(&temp), tb2));
+                    tb = tb2;
+                }
+                MGB_HIP_TRY(sort_pairs(temp, &tb2, kin, kout, vin, vout, S, tbits, st));
+            }
+            MGB_TRY(dalloc_n(s, &s->syn_t, S));
+            if (S) MGB_HIP_TRY(hipMemcpyAsync(s->syn_t, vout, S * 4, hipMemcpyDeviceToDevice, st));
+            uint32_t max_row = 0;
+            {
+                std::vector<int64_t> rp((size_t)n + 1);
+                MGB_HIP_TRY(hipMemcpyAsync(rp.data(), s->row_ptr, 8 * ((size_t)n + 1),
+                                           hipMemcpyDeviceToHost, st));
+                MGB_HIP_TRY(hipStreamSynchronize(st));
+                for (int32_t m = 0; m < n; ++m)
+                    max_row = std::max<uint32_t>(max_row, (uint32_t)(rp[m + 1] - rp[m]));
+            }
+            if (max_row >= (1u << 26)) {
+                set_error("tiled propagation: a CSR row of %u synapses exceeds 2^26", max_row);
+                return MGB_ERR_UNSUPPORTED;
+            }
+            const size_t no = (size_t)n * (size_t)(s->G + 1);
+            if (max_row <= 0xFFFFu) MGB_TRY(dalloc_n(s, &s->offs16, no));
+            else MGB_TRY(dalloc_n(s, &s->offs32, no));
+            MGB_HIP_TRY(tile_offsets(kout, S, n, s->G, s->row_ptr, s->offs16, s->offs32, st));
+        }
         MGB_HIP_TRY(hipStreamSynchronize(st));
     } else {
         // fixed-point scale: |ring value| <= in-degree * max |w| < 2^31
@@ -572,7 +641,11 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     MGB_HIP_TRY(hipMemcpyAsync(s->stream_seed, h.stream_seed.data(), 8 * h.stream_seed.size(),
                                hipMemcpyHostToDevice, st));
     // ring
-    if (s->mode == kRingClass) {
+    if (s->tiled) {
+        const size_t rt = (size_t)s->G * (size_t)s->D * (size_t)s->B;
+        MGB_TRY(dalloc_n(s, &s->ring_t, rt));
+        MGB_HIP_TRY(hipMemsetAsync(s->ring_t, 0, 4 * rt, st));
+    } else if (s->mode == kRingClass) {
         MGB_TRY(dalloc_n(s, &s->ring, (size_t)s->D * np));
         MGB_HIP_TRY(hipMemsetAsync(s->ring, 0, 4 * (size_t)s->D * np, st));
     } else {
@@ -588,7 +661,7 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     if (h.stdp) win = 1;
     s->window = std::max(win, 1);
     s->win_stride = (uint32_t)(np * (size_t)s->window);
-    MGB_TRY(dalloc_n(s, &s->win_buf, 2 * (size_t)s->win_stride));
+    MGB_TRY(dalloc_n(s, &s->win_buf, (s->tiled ? 3 : 2) * (size_t)s->win_stride));
     // record
     s->rec_cap = o ? std::max<int64_t>(o->record_capacity, 0) : 0;
     if (s->rec_cap > 0) MGB_TRY(dalloc_n(s, &s->rec, (size_t)s->rec_cap));
@@ -646,6 +719,20 @@ StdpArgs stdp_args(const mgb_sim* s) {
     return a;
 }
 
+TiledArgs tiled_args(const mgb_sim* s) {
+    TiledArgs a{};
+    a.n = s->n; a.D = s->D; a.B = s->B; a.G = s->G; a.chunk = s->chunk;
+    a.type = s->type; a.cset = s->cset; a.tbl = s->tbl; a.stream = s->stream_idx;
+    a.rate = s->rate; a.V = s->V; a.Ie = s->Ie; a.Ii = s->Ii;
+    a.refr = s->refr; a.last = s->last; a.cnt = s->cnt;
+    a.csets = s->csets; a.stream_base = s->stream_base; a.stream_seed = s->stream_seed;
+    a.tbl_off = s->tbl_off; a.seq = s->seq;
+    a.row_ptr = s->row_ptr; a.syn_t = s->syn_t; a.offs16 = s->offs16; a.offs32 = s->offs32;
+    a.ring_t = s->ring_t; a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
+    a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
+    return a;
+}
+
 int prop_grid(const mgb_sim* s, int32_t nst) {
     if (s->prop_grid > 0) return s->prop_grid;
     // waves >= expected spikes; capped so an idle window costs little
@@ -696,6 +783,24 @@ mgb_status issue_windows(mgb_sim* s, int32_t nsteps, bool finalize, bool capturi
     const StdpArgs sa = stdp_args(s);
     const bool prof = s->profile && !capturing;
     int32_t off = 0, wk = 0;
+    if (s->tiled) {
+        const TiledArgs ta = tiled_args(s);
+        while (off < nsteps) {
+            const int32_t nst = std::min(s->window, nsteps - off);
+            SpanGuard g1{s, kSpanProp, prof};
+            MGB_TRY(g1.begin());
+            MGB_HIP_TRY(launch_tiled(ta, off, wk, nst, s->tblock, s->stream));
+            MGB_TRY(g1.end());
+            s->launches += 1;
+            off += nst;
+            ++wk;
+        }
+        if (finalize) {
+            MGB_HIP_TRY(launch_finalize_tiled(s->ctl, nsteps, wk, s->stream));
+            s->launches += 1;
+        }
+        return MGB_OK;
+    }
     while (off < nsteps) {
         const int32_t nst = std::min(s->window, nsteps - off);
         SpanGuard g0{s, kSpanNeuron, prof};
@@ -757,7 +862,7 @@ mgb_status copy_from_host(mgb_sim* s, Scratch& X, void** dst, const void* src, s
 
 mgb_status create_common(mgb_sim* s, const HostNeurons& h, const DevList& L, const mgb_opts* o) {
     MGB_TRY(validate_neurons(h));
-    MGB_TRY(build_network(s, h, L, o ? o->weight_format : MGB_WF_AUTO));
+    MGB_TRY(build_network(s, h, L, o ? o->weight_format : MGB_WF_AUTO, o));
     MGB_TRY(setup_state(s, h, o));
     return MGB_OK;
 }
@@ -1088,6 +1193,10 @@ mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
     out->device_bytes = s->bytes;
     out->launches = s->launches;
     out->spikes = (int64_t)c.rec_cnt;
+    out->propagation = s->tiled ? MGB_PROP_TILED : MGB_PROP_DIRECT;
+    out->tile_size = s->B;
+    out->tiles = s->G;
+    out->block = s->tiled ? s->tblock : s->prop_block;
     out->last_run_ms = 0.0;
     if (s->timed) {
         float ms = 0.f;

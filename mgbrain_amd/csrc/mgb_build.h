@@ -67,6 +67,10 @@ hipError_t run_stats(const StatArgs& a, hipStream_t st);
 hipError_t make_sort_input(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
                            bool class_payload, uint32_t* keys, uint32_t* vals, uint64_t S,
                            hipStream_t st);
+hipError_t make_tile_keys(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
+                          int32_t B, int32_t G, uint32_t* keys, uint32_t* vals, uint64_t S, hipStream_t st);
+hipError_t tile_offsets(const uint32_t* sorted_keys, uint64_t S, int32_t n, int32_t G, const int64_t* row_ptr,
+                        uint16_t* offs16, uint32_t* offs32, hipStream_t st);
 hipError_t fp8_row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n,
                           uint32_t* rmax_tmp, float* scale, hipStream_t st);
 hipError_t make_in_keys(const int32_t* tar, const uint8_t* plastic, int32_t n, uint32_t* keys,

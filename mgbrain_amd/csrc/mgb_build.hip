@@ -96,6 +96,37 @@ __global__ void k_class_keys(const int32_t* src, const int32_t* tar, const float
     }
 }
 
+// Tile-sorted copy of the CLASS CSR: key = source * G + target tile.
+__global__ void k_tile_keys(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
+                            int32_t B, int32_t G, uint32_t* keys, uint32_t* vals, uint64_t S) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        keys[i] = (uint32_t)src[i] * (uint32_t)G + (uint32_t)(tar[i] / B);
+        const uint32_t inh = (w[i] > 0.f) ? 0u : 1u;
+        vals[i] = (uint32_t)tar[i] | ((uint32_t)(dq[i] - 1u) << 26) | (inh << 31);
+    }
+}
+
+// offs[m][t] = first slot of (source m, tile t) relative to row m, t in [0, G].
+template <class T>
+__global__ void k_tile_offsets(const uint32_t* keys, uint64_t S, int32_t n, int32_t G,
+                               const int64_t* row_ptr, T* offs) {
+    const uint64_t total = (uint64_t)n * (uint64_t)(G + 1);
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t m = (uint32_t)(q / (uint64_t)(G + 1));
+        const uint32_t t = (uint32_t)(q % (uint64_t)(G + 1));
+        const uint64_t key = (uint64_t)m * (uint64_t)G + t;
+        uint64_t lo = 0, hi = S;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if ((uint64_t)keys[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        offs[q] = (T)(lo - (uint64_t)row_ptr[m]);
+    }
+}
+
 __global__ void k_bounds(const uint32_t* keys, uint64_t S, int32_t n, int64_t* out) {
     const int32_t m = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (m > n) return;
@@ -262,6 +293,26 @@ hipError_t make_sort_input(const int32_t* src, const int32_t* tar, const float* 
                            keys, vals, S);
     else
         hipLaunchKernelGGL(k_iota_keys, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, keys, vals, S);
+    return hipGetLastError();
+}
+
+hipError_t make_tile_keys(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
+                          int32_t B, int32_t G, uint32_t* keys, uint32_t* vals, uint64_t S, hipStream_t st) {
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_keys, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, tar, w, dq, B, G, keys,
+                       vals, S);
+    return hipGetLastError();
+}
+
+hipError_t tile_offsets(const uint32_t* sorted_keys, uint64_t S, int32_t n, int32_t G, const int64_t* row_ptr,
+                        uint16_t* offs16, uint32_t* offs32, hipStream_t st) {
+    const uint64_t total = (uint64_t)n * (uint64_t)(G + 1);
+    if (offs16)
+        hipLaunchKernelGGL(k_tile_offsets<uint16_t>, dim3(blocks_for(total, 256)), dim3(256), 0, st,
+                           sorted_keys, S, n, G, row_ptr, offs16);
+    else
+        hipLaunchKernelGGL(k_tile_offsets<uint32_t>, dim3(blocks_for(total, 256)), dim3(256), 0, st,
+                           sorted_keys, S, n, G, row_ptr, offs32);
     return hipGetLastError();
 }
 

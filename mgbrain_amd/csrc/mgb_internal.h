@@ -49,7 +49,12 @@ struct Ctl {
     int32_t pad;
     uint64_t rec_cnt;       // spikes recorded (may exceed capacity)
     uint64_t se;            // synaptic events
+    // fused tiled path: three spike lists (write k%3, read (k-1)%3, clear (k+1)%3)
+    uint32_t tcnt[3];
+    uint32_t pad2;
+    int64_t tbase[3];       // first step of the window that wrote each list
 };
+
 
 // Constant set: LIFECONSTS[30] (mgcore.cu:32-63) + refractory steps in c[30].
 struct CSet {
@@ -125,6 +130,41 @@ struct StdpArgs {
     uint32_t win_stride;
 };
 
+// Fused tiled window kernel: workgroup g owns targets [g*B, g*B+B) and keeps
+// their ring slice ring_t[g][D][B] in LDS while it (1) adds the previous
+// window's spikes arriving in the tile (tile-sorted CSR copy syn_t, per-row
+// tile offsets offs[src][G+1]) and (2) advances the tile's neurons.
+struct TiledArgs {
+    int32_t n, D, B, G;
+    int32_t chunk;          // synapses per work item in the propagation phase
+    const uint8_t* type;
+    const uint8_t* cset;
+    const uint16_t* tbl;
+    const uint16_t* stream;
+    const float* rate;
+    float* V;
+    float* Ie;
+    float* Ii;
+    int32_t* refr;
+    int32_t* last;
+    int32_t* cnt;
+    const CSet* csets;
+    const int32_t* stream_base;
+    const uint64_t* stream_seed;
+    const uint32_t* tbl_off;
+    const float* seq;
+    const int64_t* row_ptr;
+    const uint32_t* syn_t;
+    const uint16_t* offs16;
+    const uint32_t* offs32;
+    uint32_t* ring_t;
+    Ctl* ctl;
+    uint32_t* win_buf;
+    uint32_t win_stride;
+    uint64_t* rec;
+    uint64_t rec_cap;
+};
+
 enum RingMode { kRingClass = 0, kRingFixed = 1 };
 enum WideKind { kWideF32 = 0, kWideBF16 = 1, kWideFP8 = 2 };
 
@@ -135,5 +175,11 @@ hipError_t launch_prop(int mode, int wide_kind, const PropArgs& a, int32_t off, 
                        int grid, int block, hipStream_t st);
 hipError_t launch_stdp(const StdpArgs& a, int32_t off, int32_t wk, int grid, hipStream_t st);
 hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
+constexpr int kTiledBatch = 1024;   // spikes staged in LDS per propagation batch
+size_t tiled_lds_bytes(int D, int B);
+hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
+                        hipStream_t st);
+hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
+hipError_t set_tiled_lds_limit(size_t bytes);
 
 }  // namespace mgb

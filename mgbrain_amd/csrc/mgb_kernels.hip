@@ -45,6 +45,139 @@ __device__ __forceinline__ float fp8e4m3_to_f32(uint8_t b) {
 
 enum : int { T_RAW = 0, T_LIF0 = 1, T_POISSON = 2, T_LIF = 3, T_LIFE = 4, T_LIFB = 5 };
 
+// Registers of one neuron across the steps of a window.
+struct NState {
+    float V, Ie, Ii, rate;
+    int32_t refr, last, cnt, refr_steps;
+    float c3, c4, c12, c13, c14, c15, c16, c17;
+    uint32_t off_e, off_i;       // CLASS seq-sum table offsets
+    uint32_t key0, key1, lid;    // Poisson stream
+    U32x4 rnd;
+};
+
+__device__ __forceinline__ bool integrates(int type) {
+    return type == T_LIF || type == T_LIFB || type == T_LIFE;
+}
+
+// Loads the neuron's registers (state, constants, stream).
+template <class A>
+__device__ __forceinline__ void neuron_load(const A& a, int i, int type, bool class_tables, NState& st) {
+    st.V = st.Ie = st.Ii = st.rate = 0.f;
+    st.refr = st.refr_steps = 0;
+    st.c3 = st.c4 = st.c12 = st.c13 = st.c14 = st.c15 = st.c16 = st.c17 = 0.f;
+    st.off_e = st.off_i = st.key0 = st.key1 = st.lid = 0u;
+    st.rnd = U32x4{{0u, 0u, 0u, 0u}};
+    st.last = a.last[i];
+    st.cnt = a.cnt[i];
+    if (type == T_POISSON) {
+        st.rate = a.rate[i];
+        const uint32_t sidx = a.stream ? a.stream[i] : 0u;
+        const uint64_t seed = a.stream_seed[sidx];
+        st.key0 = (uint32_t)seed;
+        st.key1 = (uint32_t)(seed >> 32);
+        st.lid = (uint32_t)(i - a.stream_base[sidx]);
+    } else if (integrates(type)) {
+        const CSet& C = a.csets[a.cset[i]];
+        st.c3 = C.c[3];
+        st.c4 = C.c[4];
+        st.refr_steps = __float_as_int(C.c[30]);
+        if (type == T_LIF) {
+            st.c12 = C.c[12]; st.c13 = C.c[13]; st.c14 = C.c[14];
+            st.c15 = C.c[15]; st.c16 = C.c[16]; st.c17 = C.c[17];
+        }
+        st.V = a.V[i];
+        st.refr = a.refr[i];
+        if (type != T_LIFB) {
+            st.Ie = a.Ie[i];
+            st.Ii = a.Ii[i];
+        }
+        if (class_tables) {
+            const uint32_t t = a.tbl[i];
+            st.off_e = a.tbl_off[2 * t];
+            st.off_i = a.tbl_off[2 * t + 1];
+        }
+    }
+}
+
+template <class A>
+__device__ __forceinline__ void neuron_store(const A& a, int i, int type, const NState& st) {
+    a.last[i] = st.last;
+    a.cnt[i] = st.cnt;
+    if (integrates(type)) {
+        a.V[i] = st.V;
+        a.refr[i] = st.refr;
+        if (type != T_LIFB) {
+            a.Ie[i] = st.Ie;
+            a.Ii[i] = st.Ii;
+        }
+    }
+}
+
+// Poisson draw of step s (w = step index inside the window), mgcore.cu:121-131.
+__device__ __forceinline__ bool poisson_step(NState& st, int32_t s, int32_t w) {
+    if (w == 0 || (s & 3) == 0)
+        st.rnd = philox4x32_10(st.lid, (uint32_t)s >> 2, 0u, kTagPoisson, st.key0, st.key1);
+    const float u = u01_open0(pick4(st.rnd, (uint32_t)s & 3u));
+    const bool fired = st.rate > u;
+    if (fired) {
+        ++st.cnt;
+        st.last = s;
+    }
+    return fired;
+}
+
+// One step of an integrating neuron given this step's ring input (be, bi).
+__device__ __forceinline__ bool integrate_step(int type, NState& st, int32_t s, float be, float bi) {
+    bool fired = false;
+    if (type == T_LIFB) {                                        // mgcore.cu:164-196
+        if (st.refr > 0) {
+            --st.refr;
+        } else if (st.V >= st.c4) {
+            fired = true; ++st.cnt; st.V = st.c3; st.last = s; st.refr = st.refr_steps;
+        } else {
+            st.V = st.V + fmaf(-0.005f, st.V, be + bi);
+        }
+    } else if (type == T_LIFE) {                                 // :197-233
+        st.Ie = fmaf(st.Ie, 0.98f, be);
+        st.Ii = fmaf(st.Ii, 0.99f, -bi);
+        if (st.refr > 0) {
+            --st.refr;
+        } else if (st.V >= st.c4) {
+            fired = true; ++st.cnt; st.V = st.c3; st.last = s; st.refr = st.refr_steps;
+        } else {
+            const float V = st.V;
+            const float t0 = -0.06f - V;
+            const float t1 = fmaf(st.Ie, 0.0f - V, t0);
+            const float t2 = fmaf(st.Ii, -0.08f - V, t1);
+            const float t3 = t2 + 0.02f;
+            st.V = fmaf(t3, 0.005f, V);
+        }
+    } else {                                                     // LIF :132-163
+        if (st.refr > 0) {
+            --st.refr;
+        } else {
+            st.V = fmaf(st.Ii, st.c13, fmaf(st.Ie, st.c12, fmaf(st.c17, st.V, st.c14)));
+            st.Ie *= st.c15;
+            st.Ii *= st.c16;
+            if (st.V >= st.c4) {
+                fired = true; ++st.cnt; st.V = st.c3; st.last = s; st.refr = st.refr_steps;
+            } else {
+                st.Ie += be;
+                st.Ii += bi;
+            }
+        }
+    }
+    return fired;
+}
+
+// CLASS ring word -> (excitatory, inhibitory) input: k-fold FP32 sums of the weight.
+__device__ __forceinline__ void class_input(uint32_t x, const float* seq, const NState& st, float& be,
+                                            float& bi) {
+    const uint32_t ce = x & 0xFFFFu, ci = x >> 16;
+    be = ce ? seq[st.off_e + ce] : 0.f;
+    bi = ci ? seq[st.off_i + ci] : 0.f;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kNeuBlock) void k_neurons(NeuArgs a, int32_t off, int32_t wk,
                                                       int32_t nsteps) {
@@ -60,72 +193,22 @@ __global__ __launch_bounds__(kNeuBlock) void k_neurons(NeuArgs a, int32_t off, i
     const bool active = i < a.n;
     const int type = active ? (int)a.type[i] : T_RAW;
     uint32_t* wbuf = a.win_buf + (size_t)par * a.win_stride;
-
-    float V = 0.f, Ie = 0.f, Ii = 0.f, rate = 0.f;
-    int32_t refr = 0, last = 0, cnt = 0;
-    float c3 = 0.f, c4 = 0.f, c12 = 0.f, c13 = 0.f, c14 = 0.f, c15 = 0.f, c16 = 0.f, c17 = 0.f;
-    int32_t refr_steps = 0;
-    uint32_t off_e = 0, off_i = 0;
-    uint32_t key0 = 0, key1 = 0, lid = 0;
-    if (active) {
-        last = a.last[i];
-        cnt = a.cnt[i];
-        if (type == T_POISSON) {
-            rate = a.rate[i];
-            const uint32_t st = a.stream ? a.stream[i] : 0u;
-            const uint64_t seed = a.stream_seed[st];
-            key0 = (uint32_t)seed;
-            key1 = (uint32_t)(seed >> 32);
-            lid = (uint32_t)(i - a.stream_base[st]);
-        } else if (type == T_LIF || type == T_LIFB || type == T_LIFE) {
-            const CSet& C = a.csets[a.cset[i]];
-            c3 = C.c[3];
-            c4 = C.c[4];
-            refr_steps = __float_as_int(C.c[30]);
-            if (type == T_LIF) {
-                c12 = C.c[12]; c13 = C.c[13]; c14 = C.c[14];
-                c15 = C.c[15]; c16 = C.c[16]; c17 = C.c[17];
-            }
-            V = a.V[i];
-            refr = a.refr[i];
-            if (type != T_LIFB) {
-                Ie = a.Ie[i];
-                Ii = a.Ii[i];
-            }
-            if (MODE == kRingClass) {
-                const uint32_t t = a.tbl[i];
-                off_e = a.tbl_off[2 * t];
-                off_i = a.tbl_off[2 * t + 1];
-            }
-        }
-    }
-    const bool integrates = type == T_LIF || type == T_LIFB || type == T_LIFE;
+    NState st;
+    if (active) neuron_load(a, i, type, MODE == kRingClass, st);
+    const bool integ = integrates(type);
     int32_t slot = (int32_t)(s0 % a.D);
-    U32x4 rnd = {{0u, 0u, 0u, 0u}};
-
     for (int32_t w = 0; w < nsteps; ++w) {
         const int32_t s = (int32_t)(s0 + w);
         bool fired = false;
-        if (type == T_POISSON) {                                     // mgcore.cu:121-131
-            if (w == 0 || (s & 3) == 0)
-                rnd = philox4x32_10(lid, (uint32_t)s >> 2, 0u, kTagPoisson, key0, key1);
-            const float u = u01_open0(pick4(rnd, (uint32_t)s & 3u));
-            fired = rate > u;
-            if (fired) { ++cnt; last = s; }
-        } else if (integrates) {
+        if (type == T_POISSON) {
+            fired = poisson_step(st, s, w);
+        } else if (integ) {
             const size_t loc = (size_t)slot * (size_t)a.npad + (size_t)i;
             float be, bi;
             if (MODE == kRingClass) {
                 const uint32_t x = a.ring[loc];
-                if (x != 0u) {
-                    a.ring[loc] = 0u;                                // clearInput :99-103
-                    const uint32_t ce = x & 0xFFFFu, ci = x >> 16;
-                    be = ce ? a.seq[off_e + ce] : 0.f;
-                    bi = ci ? a.seq[off_i + ci] : 0.f;
-                } else {
-                    be = 0.f;
-                    bi = 0.f;
-                }
+                if (x != 0u) a.ring[loc] = 0u;                       // clearInput :99-103
+                class_input(x, a.seq, st, be, bi);
             } else {
                 const int32_t qe = a.ring_e[loc], qi = a.ring_i[loc];
                 if (qe != 0) a.ring_e[loc] = 0;
@@ -133,43 +216,7 @@ __global__ __launch_bounds__(kNeuBlock) void k_neurons(NeuArgs a, int32_t off, i
                 be = (float)qe * a.inv_e;
                 bi = (float)qi * a.inv_i;
             }
-            if (type == T_LIFB) {                                    // :164-196
-                if (refr > 0) {
-                    --refr;
-                } else if (V >= c4) {
-                    fired = true; ++cnt; V = c3; last = s; refr = refr_steps;
-                } else {
-                    V = V + fmaf(-0.005f, V, be + bi);
-                }
-            } else if (type == T_LIFE) {                             // :197-233
-                Ie = fmaf(Ie, 0.98f, be);
-                Ii = fmaf(Ii, 0.99f, -bi);
-                if (refr > 0) {
-                    --refr;
-                } else if (V >= c4) {
-                    fired = true; ++cnt; V = c3; last = s; refr = refr_steps;
-                } else {
-                    const float t0 = -0.06f - V;
-                    const float t1 = fmaf(Ie, 0.0f - V, t0);
-                    const float t2 = fmaf(Ii, -0.08f - V, t1);
-                    const float t3 = t2 + 0.02f;
-                    V = fmaf(t3, 0.005f, V);
-                }
-            } else {                                                 // LIF :132-163
-                if (refr > 0) {
-                    --refr;
-                } else {
-                    V = fmaf(Ii, c13, fmaf(Ie, c12, fmaf(c17, V, c14)));
-                    Ie *= c15;
-                    Ii *= c16;
-                    if (V >= c4) {
-                        fired = true; ++cnt; V = c3; last = s; refr = refr_steps;
-                    } else {
-                        Ie += be;
-                        Ii += bi;
-                    }
-                }
-            }
+            fired = integrate_step(type, st, s, be, bi);
         }
         // wave-ballot compaction of this step's spikes
         const uint64_t m = __ballot(fired);
@@ -182,18 +229,7 @@ __global__ __launch_bounds__(kNeuBlock) void k_neurons(NeuArgs a, int32_t off, i
         }
         slot = (slot + 1 == a.D) ? 0 : slot + 1;
     }
-    if (active) {
-        a.last[i] = last;
-        a.cnt[i] = cnt;
-        if (integrates) {
-            a.V[i] = V;
-            a.refr[i] = refr;
-            if (type != T_LIFB) {
-                a.Ie[i] = Ie;
-                a.Ii[i] = Ii;
-            }
-        }
-    }
+    if (active) neuron_store(a, i, type, st);
 }
 
 // Propagation: one wave per spike, lanes stride the spiker's CSR row.
@@ -317,6 +353,192 @@ __global__ void k_finalize(Ctl* ctl, int32_t steps, int32_t windows) {
     ctl->base_window += windows;
 }
 
+
+// Inclusive block-wide prefix sum (blockDim.x a multiple of 64, <= 1024).
+__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* wsum) {
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    if (lane == 63) wsum[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+        uint32_t x = lane < nw ? wsum[lane] : 0u;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(x, o);
+            if (lane >= (uint32_t)o) x += t;
+        }
+        if (lane < nw) wsum[lane] = x;
+    }
+    __syncthreads();
+    if (wid > 0) v += wsum[wid - 1];
+    return v;
+}
+
+// One window of the fused tiled step (CLASS format).  Workgroup g:
+//   1. loads its ring slice ring_t[g] (D x B words) into LDS;
+//   2. adds the previous window's spikes: for each spiker, the sub-row of its
+//      tile-sorted CSR row that targets this tile, split into work items of
+//      `chunk` synapses balanced over the block, one LDS atomic per synapse;
+//   3. advances the tile's neurons through this window's steps reading and
+//      clearing their LDS ring rows, appending spikes (+ record, synaptic
+//      events) with one wave ballot per step;
+//   4. writes the ring slice back.
+// Spike lists rotate over three buffers: window k writes k%3, reads (k-1)%3
+// and clears the counter of (k+1)%3, which no workgroup touches in window k.
+template <int CH>
+__global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_t wk, int32_t nsteps) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t wsum[16];
+    __shared__ unsigned long long se_blk;
+    const int32_t D = a.D, B = a.B, g = (int32_t)blockIdx.x, bd = (int32_t)blockDim.x;
+    const int32_t tid = (int32_t)threadIdx.x;
+    uint32_t* ring = lds;                                   // [D][B]
+    uint32_t* sp_beg = ring + (size_t)D * B;                // [kTiledBatch]
+    uint32_t* sp_len = sp_beg + kTiledBatch;                // len | slot-of-spike << 26
+    uint32_t* sp_pre = sp_len + kTiledBatch;                // inclusive prefix of work items
+    Ctl* ctl = a.ctl;
+    const int64_t s0 = ctl->base_step + off;
+    const int64_t kwin = ctl->base_window + wk;
+    const int bcur = (int)(kwin % 3), bprev = (int)((kwin + 2) % 3), bnext = (int)((kwin + 1) % 3);
+    const int32_t t0 = g * B;
+    const int32_t tn = min(B, a.n - t0);
+    if (g == 0 && tid == 0) {
+        ctl->tcnt[bnext] = 0u;
+        ctl->tbase[bcur] = s0;
+    }
+    if (tid == 0) se_blk = 0ull;
+    // 1. ring slice -> LDS
+    uint32_t* gring = a.ring_t + (size_t)g * (size_t)D * (size_t)B;
+    for (int32_t k = tid; k < D * B; k += bd) ring[k] = gring[k];
+    __syncthreads();
+
+    // 2. previous window's spikes into the tile
+    const uint32_t E = ctl->tcnt[bprev];
+    const int64_t sprev = ctl->tbase[bprev];
+    const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride;
+    const int32_t G1 = a.G + 1;
+    for (uint32_t b0 = 0; b0 < E; b0 += (uint32_t)bd) {
+        const uint32_t nb = min((uint32_t)bd, E - b0);
+        uint32_t items = 0u;
+        if ((uint32_t)tid < nb) {
+            const uint32_t ent = L[b0 + tid];
+            const uint32_t src = ent & kIdMask;
+            const size_t ob = (size_t)src * (size_t)G1 + (size_t)g;
+            uint32_t o0, o1;
+            if (a.offs16) { o0 = a.offs16[ob]; o1 = a.offs16[ob + 1]; }
+            else { o0 = a.offs32[ob]; o1 = a.offs32[ob + 1]; }
+            const uint32_t len = o1 - o0;
+            const int32_t sD = (int32_t)((sprev + (int64_t)(ent >> 26)) % D);
+            sp_beg[tid] = (uint32_t)a.row_ptr[src] + o0;
+            sp_len[tid] = len | ((uint32_t)sD << 26);
+            items = (len + CH - 1) / CH;
+        }
+        const uint32_t incl = block_incl_scan(items, wsum);
+        if ((uint32_t)tid < nb) sp_pre[tid] = incl;
+        __syncthreads();
+        const uint32_t total = sp_pre[nb - 1];
+        for (uint32_t it = (uint32_t)tid; it < total; it += (uint32_t)bd) {
+            uint32_t lo = 0, hi = nb - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sp_pre[mid] > it) hi = mid;
+                else lo = mid + 1;
+            }
+            const uint32_t c = it - (lo ? sp_pre[lo - 1] : 0u);
+            const uint32_t meta = sp_len[lo];
+            const uint32_t len = meta & kIdMask;
+            const int32_t sD = (int32_t)(meta >> 26);
+            const uint32_t kb = sp_beg[lo] + c * CH;
+            const uint32_t cnt = min((uint32_t)CH, len - c * CH);
+            uint32_t x[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u)
+                if ((uint32_t)u < cnt) x[u] = __builtin_nontemporal_load(&a.syn_t[kb + u]);
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                if ((uint32_t)u < cnt) {
+                    int32_t slot = sD + (int32_t)((x[u] >> 26) & 31u) + 1;
+                    slot -= (slot >= D) ? D : 0;
+                    const int32_t loc = (int32_t)(x[u] & kIdMask) - t0;
+                    atomicAdd(&ring[slot * B + loc], (x[u] >> 31) ? 0x10000u : 1u);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // 3. neurons of the tile through this window's steps
+    uint32_t* wbuf = a.win_buf + (size_t)bcur * a.win_stride;
+    const int32_t slot0 = (int32_t)(s0 % D);
+    unsigned long long se_local = 0ull;
+    for (int32_t j0 = 0; j0 < B; j0 += bd) {
+        const int32_t j = j0 + tid;
+        const bool active = j < tn;
+        const int32_t i = t0 + j;
+        const int type = active ? (int)a.type[i] : T_RAW;
+        NState st;
+        if (active) neuron_load(a, i, type, true, st);
+        const bool integ = integrates(type);
+        int32_t outdeg = 0;
+        if (active && type != T_RAW && type != T_LIF0)
+            outdeg = (int32_t)(a.row_ptr[i + 1] - a.row_ptr[i]);
+        int32_t slot = slot0;
+        for (int32_t w = 0; w < nsteps; ++w) {
+            const int32_t s = (int32_t)(s0 + w);
+            bool fired = false;
+            if (type == T_POISSON) {
+                fired = poisson_step(st, s, w);
+            } else if (integ) {
+                uint32_t* r = &ring[slot * B + j];
+                const uint32_t x = *r;
+                *r = 0u;                                              // clearInput
+                float be, bi;
+                class_input(x, a.seq, st, be, bi);
+                fired = integrate_step(type, st, s, be, bi);
+            }
+            const uint64_t m = __ballot(fired);
+            if (m != 0ull) {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                const uint32_t pc = (uint32_t)__popcll(m);
+                uint32_t base = 0u;
+                unsigned long long rb = 0ull;
+                if (lane_id() == leader) {
+                    base = atomicAdd(&ctl->tcnt[bcur], pc);
+                    rb = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)pc);
+                }
+                base = __shfl(base, (int)leader);
+                rb = __shfl(rb, (int)leader);
+                if (fired) {
+                    const uint32_t r = rank_below(m);
+                    wbuf[base + r] = (uint32_t)i | ((uint32_t)w << 26);
+                    se_local += (unsigned long long)outdeg;
+                    if (a.rec) {
+                        const uint64_t at = rb + r;
+                        if (at < a.rec_cap) a.rec[at] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)i;
+                        else ctl->rec_overflow = 1;
+                    }
+                }
+            }
+            slot = (slot + 1 == D) ? 0 : slot + 1;
+        }
+        if (active) neuron_store(a, i, type, st);
+    }
+    if (se_local) atomicAdd(&se_blk, se_local);
+    __syncthreads();
+    if (tid == 0 && se_blk) atomicAdd((unsigned long long*)&ctl->se, se_blk);
+    // 4. LDS -> ring slice
+    for (int32_t k = tid; k < D * B; k += bd) gring[k] = ring[k];
+}
+
+__global__ void k_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows) {
+    ctl->base_step += steps;
+    ctl->base_window += windows;
+}
+
 }  // namespace
 
 hipError_t launch_neurons(int mode, const NeuArgs& a, int32_t off, int32_t wk, int32_t nsteps,
@@ -355,6 +577,39 @@ hipError_t launch_stdp(const StdpArgs& a, int32_t off, int32_t wk, int grid, hip
 
 hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1), 0, st, ctl, steps, windows);
+    return hipGetLastError();
+}
+
+size_t tiled_lds_bytes(int D, int B) {
+    return sizeof(uint32_t) * ((size_t)D * (size_t)B + 3 * (size_t)kTiledBatch);
+}
+
+hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
+                        hipStream_t st) {
+    const size_t lds = tiled_lds_bytes(a.D, a.B);
+    switch (a.chunk) {
+        case 4:
+            hipLaunchKernelGGL(k_tiled<4>, dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_tiled<16>, dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
+            break;
+        default:
+            hipLaunchKernelGGL(k_tiled<8>, dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
+            break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t set_tiled_lds_limit(size_t bytes) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_tiled<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)k_tiled<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)k_tiled<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return e;
+}
+
+hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st) {
+    hipLaunchKernelGGL(k_finalize_tiled, dim3(1), dim3(1), 0, st, ctl, steps, windows);
     return hipGetLastError();
 }
 

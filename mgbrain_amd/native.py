@@ -18,6 +18,7 @@ HIP_LIB = os.path.join(LIB_DIR, "libmgbrain_hip.so")
 OK, ERR_INVALID, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_CAPACITY = range(6)
 RAW, LIF0, POISSON, LIF, LIFE, LIFB = range(6)
 WF_AUTO, WF_CLASS, WF_F32, WF_BF16, WF_FP8 = range(5)
+PROP_AUTO, PROP_DIRECT, PROP_TILED = range(3)
 CONN_BERNOULLI, CONN_FIXED_INDEGREE = 1, 2
 F_V, F_IEXC, F_IINH, F_REFR, F_LAST, F_CNT = range(6)
 
@@ -31,7 +32,8 @@ _p = C.c_void_p
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int32), ("weight_format", C.c_int32), ("window", C.c_int32),
                 ("use_graph", C.c_int32), ("record_capacity", C.c_int64), ("prop_block", C.c_int32),
-                ("prop_grid", C.c_int32), ("profile", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("prop_grid", C.c_int32), ("profile", C.c_int32), ("propagation", C.c_int32),
+                ("tile_size", C.c_int32), ("chunk", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class NetDesc(C.Structure):
@@ -69,7 +71,8 @@ class Info(C.Structure):
                 ("device_bytes", C.c_int64), ("last_run_ms", C.c_double), ("launches", C.c_int64),
                 ("spikes", C.c_int64), ("neuron_ms", C.c_double), ("prop_ms", C.c_double),
                 ("stdp_ms", C.c_double), ("neuron_launches", C.c_int64), ("prop_launches", C.c_int64),
-                ("stdp_launches", C.c_int64)]
+                ("stdp_launches", C.c_int64), ("propagation", C.c_int32), ("tile_size", C.c_int32),
+                ("tiles", C.c_int32), ("block", C.c_int32)]
 
 
 _lib = None
@@ -118,7 +121,7 @@ def _addr(a):
 
 
 def make_opts(weight_format=WF_AUTO, window=0, use_graph=0, record_capacity=0, device=0, prop_block=0,
-              prop_grid=0, profile=0):
+              prop_grid=0, profile=0, propagation=0, tile_size=0, chunk=0):
     o = Opts()
     o.device = device
     o.weight_format = weight_format
@@ -128,6 +131,9 @@ def make_opts(weight_format=WF_AUTO, window=0, use_graph=0, record_capacity=0, d
     o.prop_block = prop_block
     o.prop_grid = prop_grid
     o.profile = profile
+    o.propagation = propagation
+    o.tile_size = tile_size
+    o.chunk = chunk
     return o
 
 

@@ -55,12 +55,37 @@ def _compare_run(net, syn, nsteps, seed=0, gpu_kw=None, generated=False, acc=po.
     return n_spk
 
 
-def test_brunel_reference_generator_class_format():
-    _say("case: make_brunel(2e5) quota generator, Poisson p=0.01, 1000 steps, CLASS format")
+@pytest.mark.parametrize("prop", [1, 2])
+def test_brunel_reference_generator_class_format(prop):
+    _say(f"case: make_brunel(2e5) quota generator, Poisson p=0.01, 1000 steps, CLASS format, prop={prop}")
     net = M.brunel_ref(200_000, rate=0.01)
     syn = H.host_synapses(net, mt_seed=4242)
-    n = _compare_run(net, syn, 1000, seed=7, gpu_kw=dict(weight_format=1))
+    n = _compare_run(net, syn, 1000, seed=7, gpu_kw=dict(weight_format=1, propagation=prop))
     assert n > 1000
+
+
+@pytest.mark.parametrize("tile,chunk,block", [(64, 4, 256), (100, 8, 512), (333, 16, 1024), (2000, 8, 64)])
+def test_tiled_shapes(tile, chunk, block):
+    _say(f"case: config1 x0.2 tiled tile={tile} chunk={chunk} block={block}")
+    net = M.config1(scale=0.2, poisson_rate=0.2)
+    syn = H.host_synapses(net, seed=12)
+    _compare_run(net, syn, 700, seed=12, gpu_kw=dict(propagation=2, tile_size=tile, chunk=chunk,
+                                                      prop_block=block))
+
+
+def test_tiled_split_runs():
+    _say("case: tiled path advanced in uneven mgb_run calls")
+    net = M.config1(scale=0.2, poisson_rate=0.2)
+    syn = H.host_synapses(net, seed=13)
+    g = H.gpu_from_list(net, syn, seed=13, record_capacity=2_000_000, propagation=2)
+    for k in (1, 5, 13, 100, 7, 374):
+        g.run(k)
+    steps, ids = g.spikes()
+    got = H.split_steps(steps, ids, 500)
+    _, o = H.oracle(net, syn, seed=13)
+    want = o.run(500)
+    assert H.first_mismatch(got, want) < 0
+    assert np.array_equal(g.state(0).view(np.uint32), o.state(po.F_V).astype(np.float32).view(np.uint32))
 
 
 def test_brunel_reference_generator_fixed_format():
@@ -77,6 +102,7 @@ def test_window_sizes_identical(window):
     net = M.config1(scale=0.2, poisson_rate=0.2)
     syn = H.host_synapses(net, seed=11)
     _compare_run(net, syn, 600, seed=5, gpu_kw=dict(window=window))
+    _compare_run(net, syn, 600, seed=5, gpu_kw=dict(window=window, propagation=1))
 
 
 def test_generated_bernoulli_matches_host_list():
@@ -105,3 +131,4 @@ def test_graph_replay_identical():
     net = M.config1(scale=0.2, poisson_rate=0.2)
     syn = H.host_synapses(net, seed=31)
     _compare_run(net, syn, 1200, seed=31, gpu_kw=dict(use_graph=1))
+    _compare_run(net, syn, 1200, seed=31, gpu_kw=dict(use_graph=1, propagation=1))
