@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from mgbrain_amd import native as N  # noqa: E402
+from mgbrain_amd import tune as T  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 DT = 1e-4
@@ -122,6 +123,11 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--propagation", type=int, default=0)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
@@ -135,9 +141,27 @@ def main(argv=None):
         pops, projs, desc, ntot = brunel_bernoulli(args.scale, seed=args.seed)
         workload = "brunel10k_bernoulli_p0.1"
     lc = lifb_consts()
+
+    def make(tile, block=0, chunk=0, profile=0):
+        return N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed, use_graph=args.graph,
+                                     profile=profile, propagation=args.propagation, tile_size=tile,
+                                     prop_block=block, chunk=chunk)
+
+    tuned = None
+    tile, block, chunk = args.tile, args.block, args.chunk
+    if not args.no_tune and args.propagation in (0, 2) and not (tile and block and chunk):
+        probe = make(0)
+        pinfo = probe.info()
+        probe.close()
+        if pinfo["propagation"] == N.PROP_TILED:
+            cands = [tile] if tile else T.tile_candidates(pinfo["n_neurons"], pinfo["ring_len"])
+            best, res = T.autotune(lambda b: make(b), cands,
+                                   log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+            if best:
+                tile, block, chunk = best["tile_size"], best["block"], best["chunk"]
+                tuned = dict(best=best, candidates=len(res))
     t0 = time.perf_counter()
-    sim = N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed,
-                                use_graph=args.graph, profile=0 if args.no_profile else 1)
+    sim = make(tile, block, chunk, profile=0 if args.no_profile else 1)
     sim.sync()
     setup_s = time.perf_counter() - t0
     sim._bench_types = np.concatenate([np.full(p["num"], p["type"], np.int32) for p in pops])
@@ -168,20 +192,33 @@ def main(argv=None):
 
     roof = None
     if b["prop_launches"] > 0:
-        # algorithmic HBM bytes of the propagation kernel (CLASS format): per spike 8 B
-        # of row pointers + 4 B spike-list entry, per synaptic event one 4 B synapse word
-        alg = 12.0 * spikes + 4.0 * se
-        per_launch = alg / b["prop_launches"]
-        avg_s = b["prop_ms"] / 1e3 / b["prop_launches"]
+        n = info0["n_neurons"]
+        D = info0["ring_len"]
+        n_poisson = int(sum(p["num"] for p in pops if p["type"] == N.POISSON))
+        n_lif = n - n_poisson
+        launches = b["prop_launches"]
+        if b["propagation"] == N.PROP_TILED:
+            # fused window kernel, bytes each launch must move through HBM (each datum once):
+            # synapse words 4 B/event; spike entry 4 B written + row pointers 8 B per spike;
+            # (source, tile) offset pairs 4 B per spike and tile; ring slice in + out 8 B per
+            # (slot, target); LIFB state V, refr, last, cnt read + write 32 B, Poisson rate +
+            # last + cnt 20 B per neuron
+            G = b["tiles"]
+            alg = (4.0 * se + 12.0 * spikes + 4.0 * spikes * G
+                   + launches * (8.0 * D * G * b["tile_size"] + 32.0 * n_lif + 20.0 * n_poisson))
+            kname = "k_tiled (fused neuron update + CLASS propagation)"
+        else:
+            alg = 12.0 * spikes + 4.0 * se
+            kname = "k_prop (CLASS, DIRECT)"
+        per_launch = alg / launches
+        avg_s = b["prop_ms"] / 1e3 / launches
         ach = per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_prop (CLASS)",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
                 "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(per_launch),
-                "launches": b["prop_launches"],
-                "neuron_kernel_avg_us": round(b["neuron_ms"] * 1e3 / max(1, b["neuron_launches"]), 3)}
-        print(f"# k_prop: {b['prop_launches']} launches, avg {avg_s * 1e6:.2f} us, {per_launch / 1e6:.3f} MB "
-              f"algorithmic -> {ach:.1f} GB/s; k_neurons avg "
-              f"{b['neuron_ms'] * 1e3 / max(1, b['neuron_launches']):.2f} us", file=sys.stderr, flush=True)
+                "launches": launches, "tile_size": b["tile_size"], "tiles": b["tiles"], "block": b["block"]}
+        print(f"# {kname}: {launches} launches, avg {avg_s * 1e6:.2f} us, {per_launch / 1e6:.3f} MB "
+              f"algorithmic -> {ach:.1f} GB/s", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -210,6 +247,7 @@ def main(argv=None):
         "rtf": wall / sim_s,
         "syn_events": se,
         "roofline": roof,
+        "autotune": tuned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

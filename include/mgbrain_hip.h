@@ -198,6 +198,12 @@ mgb_status mgb_run(mgb_sim* sim, int32_t nsteps);
 /* Blocks until the handle's stream is idle. */
 mgb_status mgb_sync(mgb_sim* sim);
 
+/* Changes the launch shape of the propagation (DIRECT) or fused (TILED) kernel
+ * without touching the data layout: workgroup size (64..1024, multiple of 64)
+ * and, for TILED, synapses per work item (4, 8, 16).  0 keeps the current
+ * value.  Used by the autotuner (mgbrain_amd/tune.py); no reference counterpart. */
+mgb_status mgb_set_launch(mgb_sim* sim, int32_t block, int32_t chunk);
+
 mgb_status mgb_get_info(mgb_sim* sim, mgb_info* out);
 
 /* Recorded spikes sorted by (step, neuron id): at most cap entries are

@@ -79,6 +79,7 @@ struct mgb_sim {
     int32_t *ring_e = nullptr, *ring_i = nullptr;
     uint32_t* tbl_off = nullptr;
     float* seq = nullptr;
+    float* tbl_w = nullptr;
     float *ltp = nullptr, *ltd = nullptr;
     int32_t lut_n = 1;
     uint32_t* win_buf = nullptr;
@@ -490,6 +491,8 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         MGB_TRY(dalloc_n(s, &s->tbl, (size_t)n));
         MGB_TRY(dalloc_n(s, &s->tbl_off, offs.size()));
         MGB_TRY(dalloc_n(s, &s->seq, seq.size()));
+        MGB_TRY(dalloc_n(s, &s->tbl_w, s->class_w.size()));
+        MGB_HIP_TRY(hipMemcpyAsync(s->tbl_w, s->class_w.data(), 4 * s->class_w.size(), hipMemcpyHostToDevice, st));
         MGB_HIP_TRY(hipMemcpyAsync(s->tbl, s->h_tbl.data(), 2 * (size_t)n, hipMemcpyHostToDevice, st));
         MGB_HIP_TRY(hipMemcpyAsync(s->tbl_off, offs.data(), 4 * offs.size(), hipMemcpyHostToDevice, st));
         MGB_HIP_TRY(hipMemcpyAsync(s->seq, seq.data(), 4 * seq.size(), hipMemcpyHostToDevice, st));
@@ -690,7 +693,7 @@ NeuArgs neu_args(const mgb_sim* s) {
     a.rate = s->rate; a.V = s->V; a.Ie = s->Ie; a.Ii = s->Ii;
     a.refr = s->refr; a.last = s->last; a.cnt = s->cnt;
     a.csets = s->csets; a.stream_base = s->stream_base; a.stream_seed = s->stream_seed;
-    a.ring = s->ring; a.tbl_off = s->tbl_off; a.seq = s->seq;
+    a.ring = s->ring; a.tbl_off = s->tbl_off; a.seq = s->seq; a.tbl_w = s->tbl_w;
     a.ring_e = s->ring_e; a.ring_i = s->ring_i;
     a.inv_e = 1.0f / s->scale_e; a.inv_i = 1.0f / s->scale_i;
     a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
@@ -726,7 +729,7 @@ TiledArgs tiled_args(const mgb_sim* s) {
     a.rate = s->rate; a.V = s->V; a.Ie = s->Ie; a.Ii = s->Ii;
     a.refr = s->refr; a.last = s->last; a.cnt = s->cnt;
     a.csets = s->csets; a.stream_base = s->stream_base; a.stream_seed = s->stream_seed;
-    a.tbl_off = s->tbl_off; a.seq = s->seq;
+    a.tbl_off = s->tbl_off; a.seq = s->seq; a.tbl_w = s->tbl_w;
     a.row_ptr = s->row_ptr; a.syn_t = s->syn_t; a.offs16 = s->offs16; a.offs32 = s->offs32;
     a.ring_t = s->ring_t; a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
     a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
@@ -1166,6 +1169,29 @@ mgb_status mgb_sync(mgb_sim* s) {
     }
     MGB_HIP_TRY(hipSetDevice(s->device));
     MGB_HIP_TRY(hipStreamSynchronize(s->stream));
+    return MGB_OK;
+}
+
+mgb_status mgb_set_launch(mgb_sim* s, int32_t block, int32_t chunk) {
+    if (!s || block < 0 || (block > 0 && (block < 64 || block > 1024 || block % 64 != 0)) ||
+        !(chunk == 0 || chunk == 4 || chunk == 8 || chunk == 16)) {
+        set_error("invalid launch shape (block %d, chunk %d)", block, chunk);
+        return MGB_ERR_INVALID;
+    }
+    MGB_TRY(mgb_sync(s));
+    if (block > 0) {
+        if (s->tiled) s->tblock = block;
+        else s->prop_block = block;
+    }
+    if (chunk > 0) s->chunk = chunk;
+    if (s->gexec) {
+        (void)hipGraphExecDestroy(s->gexec);
+        s->gexec = nullptr;
+    }
+    if (s->graph) {
+        (void)hipGraphDestroy(s->graph);
+        s->graph = nullptr;
+    }
     return MGB_OK;
 }
 

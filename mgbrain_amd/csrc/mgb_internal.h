@@ -81,6 +81,7 @@ struct NeuArgs {
     uint32_t* ring;
     const uint32_t* tbl_off;   // [n_tbl][2]: offsets of exc / inh seq-sum tables
     const float* seq;          // seq[off + k] = k-fold sequential FP32 sum of the class weight
+    const float* tbl_w;        // [n_tbl][2]: the exc / inh class weights
     // fixed-point ring
     int32_t* ring_e;
     int32_t* ring_i;
@@ -153,6 +154,7 @@ struct TiledArgs {
     const uint64_t* stream_seed;
     const uint32_t* tbl_off;
     const float* seq;
+    const float* tbl_w;
     const int64_t* row_ptr;
     const uint32_t* syn_t;
     const uint16_t* offs16;

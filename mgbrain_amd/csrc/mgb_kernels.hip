@@ -51,6 +51,7 @@ struct NState {
     int32_t refr, last, cnt, refr_steps;
     float c3, c4, c12, c13, c14, c15, c16, c17;
     uint32_t off_e, off_i;       // CLASS seq-sum table offsets
+    float we, wi;                // CLASS weights
     uint32_t key0, key1, lid;    // Poisson stream
     U32x4 rnd;
 };
@@ -66,6 +67,7 @@ __device__ __forceinline__ void neuron_load(const A& a, int i, int type, bool cl
     st.refr = st.refr_steps = 0;
     st.c3 = st.c4 = st.c12 = st.c13 = st.c14 = st.c15 = st.c16 = st.c17 = 0.f;
     st.off_e = st.off_i = st.key0 = st.key1 = st.lid = 0u;
+    st.we = st.wi = 0.f;
     st.rnd = U32x4{{0u, 0u, 0u, 0u}};
     st.last = a.last[i];
     st.cnt = a.cnt[i];
@@ -95,6 +97,8 @@ __device__ __forceinline__ void neuron_load(const A& a, int i, int type, bool cl
             const uint32_t t = a.tbl[i];
             st.off_e = a.tbl_off[2 * t];
             st.off_i = a.tbl_off[2 * t + 1];
+            st.we = a.tbl_w[2 * t];
+            st.wi = a.tbl_w[2 * t + 1];
         }
     }
 }
@@ -170,12 +174,21 @@ __device__ __forceinline__ bool integrate_step(int type, NState& st, int32_t s, 
     return fired;
 }
 
-// CLASS ring word -> (excitatory, inhibitory) input: k-fold FP32 sums of the weight.
+// k-fold sequential FP32 sum of w: what k float atomicAdds of w into a zeroed
+// buffer leave (mgcore.cu:112-115).  Short sums are added in registers, long
+// ones read from the host-built table.
+__device__ __forceinline__ float class_sum(uint32_t k, float w, const float* seq, uint32_t off) {
+    if (k > 24u) return seq[off + k];
+    float acc = 0.f;
+    for (uint32_t j = 0; j < k; ++j) acc = acc + w;
+    return acc;
+}
+
+// CLASS ring word -> (excitatory, inhibitory) input.
 __device__ __forceinline__ void class_input(uint32_t x, const float* seq, const NState& st, float& be,
                                             float& bi) {
-    const uint32_t ce = x & 0xFFFFu, ci = x >> 16;
-    be = ce ? seq[st.off_e + ce] : 0.f;
-    bi = ci ? seq[st.off_i + ci] : 0.f;
+    be = class_sum(x & 0xFFFFu, st.we, seq, st.off_e);
+    bi = class_sum(x >> 16, st.wi, seq, st.off_i);
 }
 
 template <int MODE>

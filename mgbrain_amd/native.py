@@ -23,8 +23,8 @@ CONN_BERNOULLI, CONN_FIXED_INDEGREE = 1, 2
 F_V, F_IEXC, F_IINH, F_REFR, F_LAST, F_CNT = range(6)
 
 EXPORTS = ["mgb_abi_version", "mgb_last_error", "mgb_create", "mgb_create_generated", "mgb_run",
-           "mgb_sync", "mgb_get_info", "mgb_read_spikes", "mgb_clear_spikes", "mgb_read_state",
-           "mgb_fire_counts", "mgb_read_csr", "mgb_read_weights", "mgb_destroy"]
+           "mgb_sync", "mgb_set_launch", "mgb_get_info", "mgb_read_spikes", "mgb_clear_spikes",
+           "mgb_read_state", "mgb_fire_counts", "mgb_read_csr", "mgb_read_weights", "mgb_destroy"]
 
 _p = C.c_void_p
 
@@ -99,6 +99,7 @@ def lib():
         L.mgb_create_generated.argtypes = [C.POINTER(GenDesc), C.POINTER(Opts), C.POINTER(_p)]
         L.mgb_run.argtypes = [_p, C.c_int32]
         L.mgb_sync.argtypes = [_p]
+        L.mgb_set_launch.argtypes = [_p, C.c_int32, C.c_int32]
         L.mgb_get_info.argtypes = [_p, C.POINTER(Info)]
         L.mgb_read_spikes.argtypes = [_p, C.c_int64, _p, _p, C.POINTER(C.c_int64)]
         L.mgb_clear_spikes.argtypes = [_p]
@@ -247,6 +248,9 @@ class Simulator:
 
     def sync(self):
         _check(lib().mgb_sync(self.h))
+
+    def set_launch(self, block=0, chunk=0):
+        _check(lib().mgb_set_launch(self.h, int(block), int(chunk)))
 
     def info(self):
         i = Info()
