@@ -863,6 +863,17 @@ mgb_status copy_from_host(mgb_sim* s, Scratch& X, void** dst, const void* src, s
     return MGB_OK;
 }
 
+mgb_status validate_opts(const mgb_opts* o) {
+    if (!o) return MGB_OK;
+    if (o->weight_format < MGB_WF_AUTO || o->weight_format > MGB_WF_FP8 || o->window < 0 ||
+        o->record_capacity < 0 || o->propagation < MGB_PROP_AUTO || o->propagation > MGB_PROP_TILED ||
+        o->tile_size < 0 || o->device < 0) {
+        set_error("invalid mgb_opts");
+        return MGB_ERR_INVALID;
+    }
+    return MGB_OK;
+}
+
 mgb_status create_common(mgb_sim* s, const HostNeurons& h, const DevList& L, const mgb_opts* o) {
     MGB_TRY(validate_neurons(h));
     MGB_TRY(build_network(s, h, L, o ? o->weight_format : MGB_WF_AUTO, o));
@@ -884,6 +895,7 @@ mgb_status mgb_create(const mgb_net_desc* d, const mgb_opts* o, mgb_sim** out) {
         return MGB_ERR_INVALID;
     }
     *out = nullptr;
+    MGB_TRY(validate_opts(o));
     if (d->n_neurons <= 0 || !d->type || !d->lifconst || d->n_csets < 1 ||
         (d->n_synapses > 0 && (!d->syn_src || !d->syn_tar || !d->syn_weight || !d->syn_delay)) ||
         d->n_synapses < 0) {
@@ -927,6 +939,26 @@ mgb_status mgb_create(const mgb_net_desc* d, const mgb_opts* o, mgb_sim** out) {
         h.v0[i] = d->v_init ? d->v_init[i] : h.lifconst[30 * (size_t)h.cset[i] + 1];
     }
     if (d->fp8_scale) h.fp8_scale.assign(d->fp8_scale, d->fp8_scale + n);
+    // host-side checks first, so invalid networks fail before any device work
+    MGB_TRY(validate_neurons(h));
+    for (int64_t k = 0; k < d->n_synapses; ++k) {
+        const int32_t a = d->syn_src[k], b = d->syn_tar[k];
+        if (a < 0 || a >= n || b < 0 || b >= n) {
+            set_error("synapse %lld references neuron (%d -> %d) outside [0, %d)", (long long)k, a, b, n);
+            return MGB_ERR_INVALID;
+        }
+        const int q = (int)roundf(d->syn_delay[k] / d->step);
+        if (q < 1) {
+            set_error("synapse %lld: delay %g s rounds to %d steps; at least one step is required (the "
+                      "reference divides by min_delay = 0, mgsim.cpp:802)",
+                      (long long)k, (double)d->syn_delay[k], q);
+            return MGB_ERR_INVALID;
+        }
+        if (q > kWideMaxDelay) {
+            set_error("synapse %lld: delay of %d steps exceeds %d", (long long)k, q, kWideMaxDelay);
+            return MGB_ERR_UNSUPPORTED;
+        }
+    }
 
     mgb_sim* s = new (std::nothrow) mgb_sim();
     if (!s) {
@@ -972,6 +1004,7 @@ mgb_status mgb_create_generated(const mgb_gen_desc* g, const mgb_opts* o, mgb_si
         return MGB_ERR_INVALID;
     }
     *out = nullptr;
+    MGB_TRY(validate_opts(o));
     HostNeurons h;
     h.step = g->step;
     h.stdp = g->stdp ? 1 : 0;

@@ -242,6 +242,26 @@ int orc_gen_net(const orc_pop* pops, int32_t n_pops, const orc_proj* pros, int32
                 }
                 free(genlist);
             }
+        } else if (pr->kind == ORC_CONN_REF_SHUFFLE && pr->ctype < 1.0f && pr->ctype > 0.0f) {
+            /* Config::SHUFFLE_GEN quota: netgen.cpp:127-163, std::random_shuffle of
+             * libstdc++ (swap element i with element rand() % (i + 1)) */
+            uint64_t len = (uint64_t)sn * (uint64_t)tn;
+            uint64_t gen_num = (uint64_t)roundf((float)len * pr->ctype);
+            uint8_t* gl = (uint8_t*)malloc((size_t)(len ? len : 1));
+            if (!gl) { rc = -1; break; }
+            for (uint64_t k = 0; k < len; ++k) gl[k] = k < gen_num;
+            for (uint64_t i = 1; i < len; ++i) {
+                uint64_t jj = (uint64_t)((long)orc_glibc_rand_next(&gr) % (long)(i + 1));
+                if (jj != i) { uint8_t t = gl[i]; gl[i] = gl[jj]; gl[jj] = t; }
+            }
+            for (uint64_t m = 0; m < (uint64_t)sn && rc == 0; ++m)
+                for (uint64_t n = 0; n < (uint64_t)tn && rc == 0; ++n)
+                    if (gl[m * (uint64_t)tn + n]) {
+                        float w = ref_rand_weight(&gr, pr->wmin, pr->wmax, dw);
+                        float d = ref_rand_delay(&gr, pr->dmin, pr->dmax, dt);
+                        rc = synbuf_push(&b, s0 + (int32_t)m, t0 + (int32_t)n, w, d, pl);
+                    }
+            free(gl);
         } else if (pr->kind == ORC_CONN_BERNOULLI) {
             /* new: independent Bernoulli(p) per ordered pair, source-major */
             uint64_t thr = (uint64_t)((double)pr->p * 4294967296.0);

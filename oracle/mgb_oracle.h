@@ -44,7 +44,7 @@ void orc_glibc_srand(orc_glibc_rand* g, uint32_t seed);
 int32_t orc_glibc_rand_next(orc_glibc_rand* g);
 
 /* ---- network generation (model/netgen.cpp restated) --------------------- */
-enum { ORC_CONN_REF = 0, ORC_CONN_BERNOULLI = 1, ORC_CONN_FIXED_INDEGREE = 2 };
+enum { ORC_CONN_REF = 0, ORC_CONN_BERNOULLI = 1, ORC_CONN_FIXED_INDEGREE = 2, ORC_CONN_REF_SHUFFLE = 3 };
 
 typedef struct {
     int32_t num;

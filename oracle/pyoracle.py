@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
 REF_NETGEN = os.path.join(HERE, "_ref", "ref_netgen")
 
 ACC_F32, ACC_FIXED, ACC_F64 = 0, 1, 2
-CONN_REF, CONN_BERNOULLI, CONN_FIXED_INDEGREE = 0, 1, 2
+CONN_REF, CONN_BERNOULLI, CONN_FIXED_INDEGREE, CONN_REF_SHUFFLE = 0, 1, 2, 3
 RAW, LIF0, POISSON, LIF, LIFE, LIFB = 0, 1, 2, 3, 4, 5
 F_V, F_IEXC, F_IINH, F_REFR, F_LAST, F_CNT = range(6)
 
