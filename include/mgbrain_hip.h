@@ -143,7 +143,8 @@ typedef struct mgb_gen_desc {
     float dt;                     /* Config::DT (delay grid) */
     float dw;                     /* Config::DW (weight grid) */
     int32_t stdp;
-    uint64_t seed;
+    uint64_t seed;                /* Poisson key when stream_seed is NULL */
+    uint64_t net_seed;            /* key of the initial-V draws (NetGen's seed) */
 } mgb_gen_desc;
 
 typedef struct mgb_info {

@@ -174,7 +174,8 @@ MGSimulator::MGSimulator(Model& model, uint64_t seed) {
     desc.dt = Config::DT;
     desc.dw = Config::DW;
     desc.stdp = Config::STDP ? 1 : 0;
-    desc.seed = seed;
+    desc.seed = Config::SEED;     // Poisson streams, as the host-network constructor
+    desc.net_seed = seed;         // initial-V draws, as NetGen::gen_net(net, model, seed)
     mgb_opts o = opts_from_config();
     check(mgb_create_generated(&desc, &o, &sim));
     std::cout << "run single-net on single-GPU" << std::endl;

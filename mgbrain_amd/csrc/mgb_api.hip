@@ -540,7 +540,7 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         float wabs = 0.f;
         std::memcpy(&wabs, &habs, 4);
         if (h.stdp) wabs = std::max({wabs, std::fabs(h.stdpconst[4]), std::fabs(h.stdpconst[5])});
-        if (s->wide_kind == kWideFP8) wabs *= 1.0625f;   // FP8 rounding headroom
+        if (s->wide_kind != kWideF32) wabs *= 1.0625f;   // BF16 / FP8 rounding headroom
         const double bound = (double)std::max<uint32_t>(max_deg_tot, 1) * (double)wabs;
         int e = 30;
         if (bound > 0) e = (int)std::floor(std::log2(2147483520.0 / bound));
@@ -1053,7 +1053,7 @@ mgb_status mgb_create_generated(const mgb_gen_desc* g, const mgb_opts* o, mgb_si
             h.cset[i] = (uint8_t)P.cset;
             h.stream[i] = (uint16_t)P.stream;
             h.rate[i] = P.rate;
-            h.v0[i] = vinit_draw(h.stream_seed[P.stream], (uint32_t)(i - h.stream_base[P.stream]),
+            h.v0[i] = vinit_draw(g->net_seed, (uint32_t)(i - h.stream_base[P.stream]),
                                  P.v_min, P.v_max);
         }
     }

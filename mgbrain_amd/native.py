@@ -60,7 +60,7 @@ class GenDesc(C.Structure):
     _fields_ = [("n_pops", C.c_int32), ("pops", _p), ("n_projs", C.c_int32), ("projs", _p),
                 ("n_csets", C.c_int32), ("lifconst", _p), ("stdpconst", _p), ("n_streams", C.c_int32),
                 ("stream_seed", _p), ("step", C.c_float), ("dt", C.c_float), ("dw", C.c_float),
-                ("stdp", C.c_int32), ("seed", C.c_uint64)]
+                ("stdp", C.c_int32), ("seed", C.c_uint64), ("net_seed", C.c_uint64)]
 
 
 class Info(C.Structure):
@@ -205,7 +205,7 @@ class Simulator:
 
     @classmethod
     def generated(cls, pops, projs, lifconst, step=1e-4, dt=1e-4, dw=0.01, stdpconst=None, stdp=False,
-                  seed=0, stream_seed=None, **opts):
+                  seed=0, stream_seed=None, net_seed=None, **opts):
         """pops: list of dicts (num, type, rate, v_min, v_max, cset, stream);
         projs: list of dicts (src, tar, kind, p, k, wmin, wmax, dmin, dmax, plastic, index, seed)."""
         P = (GenPop * len(pops))()
@@ -237,6 +237,7 @@ class Simulator:
         g.dw = dw
         g.stdp = int(bool(stdp))
         g.seed = seed
+        g.net_seed = seed if net_seed is None else net_seed
         o = make_opts(**opts)
         h = _p()
         _check(lib().mgb_create_generated(C.byref(g), C.byref(o), C.byref(h)))

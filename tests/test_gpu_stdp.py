@@ -1,0 +1,71 @@
+"""Pair-based STDP on the GPU against the oracle (mgcore.cu:346-412).
+
+Bit-exact spikes and weights over the run, for (a) the reference's global
+STDP (every synapse plastic, default STDPCONSTS, inhibitory weights clipped to
+W_MIN = 0 at their first update) and (b) plasticity on E->E only with the
+benchmark's constants (A+ = 0.01 W_max, A- = -0.0105 W_max, tau = 200 steps,
+W in [0, 2J]).  The oracle processes spikers sequentially in id order; the GPU
+updates all spikers' rows in parallel: each synapse receives at most one
+non-zero update per step, so both give the same weights.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from mgbrain_amd import native as N
+from oracle import pyoracle as po
+from tests import harness as H
+from tests import netmodels as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(net, syn, sc, nsteps, seed, wf):
+    g = H.gpu_from_list(net, syn, seed=seed, stdp=True, stdpconst=sc, weight_format=wf, record_capacity=4_000_000)
+    info = g.info()
+    assert info["window"] == 1
+    _, o = H.oracle(net, syn, seed=seed, acc=po.ACC_FIXED, stdp=True, stdpconst=sc,
+                    scale=(info["fix_scale_exc"], info["fix_scale_inh"]))
+    w0 = g.weights()
+    g.run(nsteps)
+    st, ids = g.spikes()
+    got = H.split_steps(st, ids, nsteps)
+    want = o.run(nsteps)
+    k = H.first_mismatch(got, want)
+    assert k < 0, f"spikes differ at step {k}"
+    wg = g.weights()
+    wo = o.weights().astype(np.float32)
+    assert np.array_equal(wg.view(np.uint32), wo.view(np.uint32))
+    changed = int(np.sum(wg != w0))
+    g.close()
+    o.close()
+    return changed, sum(len(x) for x in want)
+
+
+def test_reference_global_stdp():
+    print("case: make_brunel(2e5) global STDP, default STDPCONSTS", flush=True)
+    sys.stdout.flush()
+    net = M.brunel_ref(200_000, rate=0.01)
+    syn = H.host_synapses(net, mt_seed=31)
+    syn.plastic[:] = 1
+    changed, n = _run(net, syn, M.STDP_DEFAULT, 600, 3, N.WF_F32)
+    assert changed > 1000 and n > 1000
+
+
+@pytest.mark.parametrize("wf", [N.WF_F32, N.WF_BF16])
+def test_ee_stdp_benchmark_constants(wf):
+    print(f"case: fixed in-degree net, E->E STDP, weight_format={wf}", flush=True)
+    sys.stdout.flush()
+    net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
+    syn = H.host_synapses(net, seed=6)
+    assert 0 < syn.plastic.sum() < syn.n
+    sc = M.stdp_consts_config4(J=1e-4)
+    if wf == N.WF_BF16:
+        # static weights are stored as BF16 on the device; give the oracle the same values
+        stat = syn.plastic == 0
+        u = syn.weight.view(np.uint32).astype(np.uint64)
+        r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+        syn.weight[stat] = r[stat]
+    changed, n = _run(net, syn, sc, 800, 6, wf)
+    assert changed > 100 and n > 1000
