@@ -123,7 +123,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner instead of the tuned-shapes table")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
@@ -149,7 +149,13 @@ def main(argv=None):
 
     tuned = None
     tile, block, chunk = args.tile, args.block, args.chunk
-    if not args.no_tune and args.propagation in (0, 2) and not (tile and block and chunk):
+    table = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
+    if not args.tune and not (tile or block or chunk) and args.scale == 1.0 and os.path.exists(table):
+        shapes = json.load(open(table)).get(workload)
+        if shapes:
+            tile, block, chunk = shapes["tile_size"], shapes["block"], shapes["chunk"]
+            tuned = dict(best=shapes, source="mgbrain_amd/tuned_shapes.json")
+    if args.tune and args.propagation in (0, 2) and not (tile and block and chunk):
         probe = make(0)
         pinfo = probe.info()
         probe.close()
