@@ -168,6 +168,7 @@ def main(argv=None):
                 tuned = dict(best=best, candidates=len(res))
     t0 = time.perf_counter()
     sim = make(tile, block, chunk, profile=0 if args.no_profile else 1)
+    tuned_chunk = chunk if chunk else 8
     sim.sync()
     setup_s = time.perf_counter() - t0
     sim._bench_types = np.concatenate([np.full(p["num"], p["type"], np.int32) for p in pops])
@@ -219,8 +220,16 @@ def main(argv=None):
         per_launch = alg / launches
         avg_s = b["prop_ms"] / 1e3 / launches
         ach = per_launch / avg_s / 1e9
+        # measured HBM bytes per launch: FETCH_SIZE / WRITE_SIZE counter passes of this
+        # workload scaled by a copy kernel of known size (profiles/traffic_*.json)
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", "traffic_" + workload.replace("_fixed_indegree", "") + ".json")
+        if os.path.exists(tfile):
+            t = json.load(open(tfile))
+            if (t.get("tile_size"), t.get("block"), t.get("chunk")) == (b["tile_size"], b["block"], tuned_chunk):
+                traffic = round(t["traffic_bytes_per_launch"])
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
                 "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(per_launch),
                 "launches": launches, "tile_size": b["tile_size"], "tiles": b["tiles"], "block": b["block"]}
         print(f"# {kname}: {launches} launches, avg {avg_s * 1e6:.2f} us, {per_launch / 1e6:.3f} MB "
