@@ -30,52 +30,13 @@ sys.path.insert(0, ROOT)
 
 from mgbrain_amd import native as N  # noqa: E402
 from mgbrain_amd import tune as T  # noqa: E402
+from mgbrain_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-DT = 1e-4
-f32 = np.float32
-
-
-def lifb_consts():
-    """gen_life_const(v_init 0, v_rest 0, v_reset 10 mV, cm 0, tau_m 0, t_ref 2 ms, 1, 1,
-    V_th 20 mV, 0) at dt = 0.1 ms (constgen.h:9-107); LIFB reads DT, V_INIT, V_RESET,
-    V_THRESH and TAU_REFRAC only."""
-    c = np.zeros(30, dtype=f32)
-    c[0], c[3], c[4], c[8] = DT, 0.010, 0.020, 0.002
-    c[10], c[11] = 1.0, 1.0
-    e = np.exp(f32(-DT), dtype=f32)
-    c[12], c[13], c[15], c[16] = e, e, e, e
-    c[21], c[22], c[23], c[24], c[25] = 0.951229453, 0.967216074, 0.0799999982, 0.0599999987, 0.90483743
-    return c
-
-
-def brunel_fixed_k(scale=1.0, K=10000, rate=0.0015, seed=0):
-    nP, nE, nI = int(10000 * scale), int(80000 * scale), int(20000 * scale)
-    ntot = nP + nE + nI
-    Ks = [int(round(K * n / ntot)) for n in (nP, nE, nI)]
-    J, wi = float(f32(1e-4)), float(f32(-5e-4))
-    dmin, dmax = float(f32(0.001)), float(f32(0.002))
-    pops = [dict(num=nP, type=N.POISSON, rate=rate), dict(num=nE, type=N.LIFB), dict(num=nI, type=N.LIFB)]
-    projs = []
-    for tgt in (1, 2):
-        for src, k, w in ((0, Ks[0], J), (1, Ks[1], J), (2, Ks[2], wi)):
-            projs.append(dict(src=src, tar=tgt, kind=N.CONN_FIXED_INDEGREE, k=k, wmin=w, wmax=w, dmin=dmin,
-                              dmax=dmax, index=len(projs), seed=seed))
-    desc = (f"Brunel fixed in-degree: {nE} E + {nI} I LIFB + {nP} Poisson @ {rate / DT:.0f} Hz, "
-            f"K={K}, J=0.1 mV, g=5, delay U{{1.0..2.0}} ms, dt=0.1 ms")
-    return pops, projs, desc, ntot
-
-
-def brunel_bernoulli(scale=1.0, p=0.1, rate=0.002, seed=0):
-    nP, nE, nI = int(1000 * scale), int(8000 * scale), int(2000 * scale)
-    J, wi, d = float(f32(1e-4)), float(f32(-5e-4)), float(f32(0.0014))
-    pops = [dict(num=nP, type=N.POISSON, rate=rate), dict(num=nE, type=N.LIFB), dict(num=nI, type=N.LIFB)]
-    projs = []
-    for s, t, w in ((0, 1, J), (0, 2, J), (1, 1, J), (1, 2, J), (2, 1, wi), (2, 2, wi)):
-        projs.append(dict(src=s, tar=t, kind=N.CONN_BERNOULLI, p=p, w=w, d=d, index=len(projs), seed=seed))
-    desc = (f"Brunel Bernoulli p={p}: {nE} E + {nI} I LIFB + {nP} Poisson @ {rate / DT:.0f} Hz, "
-            f"J=0.1 mV, g=5, delay 1.4 ms, dt=0.1 ms")
-    return pops, projs, desc, nP + nE + nI
+DT = W.DT
+lifb_consts = W.lifb_consts
+brunel_fixed_k = W.brunel_fixed_k
+brunel_bernoulli = W.brunel_bernoulli
 
 
 def cpu_baseline(sim, budget_s=12.0, max_steps=4000, seed=0):
@@ -111,12 +72,115 @@ def cpu_baseline(sim, budget_s=12.0, max_steps=4000, seed=0):
                 steps=steps, seconds=el, syn_events=se, rtf=el / (steps * DT))
 
 
+def batched_leg(args, shapes=None, log=None):
+    """The batched sweep: 64 independent Brunel networks (sizes log-uniform on
+    [2k, 50k] LIF neurons, g x Poisson rate grid, seeds 0-63) advanced together by one
+    simulator (mgb_create_batch_generated; every fused-kernel launch covers all
+    networks), against the same 64 networks run one after another, each alone."""
+    log = log or (lambda m: None)
+    params, nets = W.sweep_nets(args.nets)
+    sizes = [W.net_size(x) for x in nets]
+    shp = dict(tile_size=0, prop_block=0, chunk=0)
+    if shapes:
+        shp = dict(tile_size=shapes["tile_size"], prop_block=shapes["block"], chunk=shapes["chunk"])
+    t0 = time.perf_counter()
+    sim = N.Simulator.batch_generated(nets, use_graph=args.graph, profile=0 if args.no_profile else 1, **shp)
+    sim.sync()
+    setup_s = time.perf_counter() - t0
+    info0 = sim.info()
+    log(f"# batched: {len(nets)} networks, {info0['n_neurons']} neurons, {info0['n_synapses']} synapses, "
+        f"{info0['segments']} segments, {info0['tiles']} tiles of {info0['tile_size']}, setup {setup_s:.2f} s, "
+        f"{info0['device_bytes'] / 2**30:.2f} GiB")
+    sim.run(args.warmup)
+    sim.sync()
+    a = sim.info()
+    t1 = time.perf_counter()
+    sim.run(args.steps)
+    sim.sync()
+    wall = time.perf_counter() - t1
+    b = sim.info()
+    counts = sim.fire_counts()
+    sim.close()
+    se = b["syn_events"] - a["syn_events"]
+    spikes = b["spikes"] - a["spikes"]
+    roof = None
+    if b["propagation"] == N.PROP_TILED and b["prop_launches"] > 0:
+        n_poisson = sum(p["num"] for x in nets for p in x["pops"] if p["type"] == N.POISSON)
+        n_lif = info0["n_neurons"] - n_poisson
+        alg = (4.0 * se + 12.0 * spikes + 4.0 * spikes * (b["tiles"] / max(1, b["segments"]))
+               + b["prop_launches"] * (8.0 * info0["ring_len"] * b["tiles"] * b["tile_size"]
+                                       + 32.0 * n_lif + 20.0 * n_poisson))
+        avg_s = b["prop_ms"] / 1e3 / b["prop_launches"]
+        ach = alg / b["prop_launches"] / avg_s / 1e9 if avg_s > 0 else 0.0
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_s * 1e6, 3),
+                "bytes_per_launch": round(alg / b["prop_launches"])}
+    log(f"# batched: {args.steps} steps in {wall * 1e3:.1f} ms, {se:.4g} events, {se / wall:.4g} SE/s, "
+        f"RTF {wall / (args.steps * DT):.4f}")
+    # the same networks one after another, each in its own simulator
+    seq_wall, seq_se, identical, max_rate_err = 0.0, 0, True, 0.0
+    base = 0
+    for k, net in enumerate(nets):
+        s1 = N.Simulator.generated(**net, use_graph=args.graph, profile=0)
+        s1.run(args.warmup)
+        s1.sync()
+        a1 = s1.info()
+        t = time.perf_counter()
+        s1.run(args.steps)
+        s1.sync()
+        seq_wall += time.perf_counter() - t
+        seq_se += s1.info()["syn_events"] - a1["syn_events"]
+        c1 = s1.fire_counts()
+        s1.close()
+        cb = counts[base:base + sizes[k]]
+        identical = identical and bool(np.array_equal(c1, cb))
+        r1, rb = float(c1.sum()), float(cb.sum())
+        if r1 > 0:
+            max_rate_err = max(max_rate_err, abs(rb - r1) / r1)
+        base += sizes[k]
+    log(f"# sequential: {seq_se:.4g} events in {seq_wall * 1e3:.1f} ms, {seq_se / seq_wall:.4g} SE/s; "
+        f"per-network fire counts identical: {identical}")
+    return {
+        "workload": f"brunel_sweep{len(nets)}",
+        "description": (f"{len(nets)} independent Brunel networks: N log-uniform on [2000, 50000] LIFB "
+                        f"(80/20 E/I) + 0.1 N Poisson, g in {list(W.SWEEP_G)}, Poisson rate in "
+                        f"{list(W.SWEEP_NU_HZ)} Hz, Bernoulli p = min(0.1, 5000/sources), J = 0.1 mV, "
+                        f"delays U{{1.0..2.0}} ms, seeds 0-{len(nets) - 1}"),
+        "value": se / wall, "unit": "syn_events/s", "rtf": wall / (args.steps * DT),
+        "neurons": info0["n_neurons"], "synapses": info0["n_synapses"], "segments": info0["segments"],
+        "weight_format": "f32 per-class (4 B synapse word; exact)", "graph": args.graph,
+        "setup_s": round(setup_s, 2), "syn_events": se,
+        "sequential_value": seq_se / seq_wall, "sequential_rtf": seq_wall / (args.steps * DT),
+        "speedup_vs_sequential": (se / wall) / (seq_se / seq_wall),
+        "fire_counts_identical_to_sequential": identical, "max_rate_err_vs_sequential": max_rate_err,
+        "roofline": roof, "shapes": shapes,
+    }
+
+
+def tune_batched(args, log):
+    """Autotunes the batched sweep's launch shapes (each tile size builds the batch anew)."""
+    _, nets = W.sweep_nets(args.nets)
+    probe = N.Simulator.batch_generated(nets)
+    pinfo = probe.info()
+    probe.close()
+    if pinfo["propagation"] != N.PROP_TILED:
+        return None
+    cands = T.tile_candidates(max(W.net_size(x) for x in nets), pinfo["ring_len"])
+    best, res = T.autotune(lambda b: N.Simulator.batch_generated(nets, tile_size=b), cands,
+                           blocks=(256, 512, 1024), chunks=(4, 8), log=lambda m: log("# " + m))
+    return dict(best, candidates=len(res)) if best else None
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--config", type=int, default=2, choices=(1, 2))
+    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 5),
+                    help="1: Brunel 10k Bernoulli; 2: Brunel 100k K=10000 (default, plus the batched leg); "
+                         "5: batched sweep only")
+    ap.add_argument("--nets", type=int, default=64, help="networks in the batched sweep")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg of the default run")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--graph", type=int, default=0)
@@ -134,6 +198,26 @@ def main(argv=None):
                                    "not an implemented path", "n_gpus": args.gpus}))
         return 2
 
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    table = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
+    tuned_table = json.load(open(table)) if os.path.exists(table) else {}
+    batch_shapes = None if args.tune else tuned_table.get(f"brunel_sweep{args.nets}")
+    if args.config == 5:
+        if args.tune:
+            batch_shapes = tune_batched(args, log)
+        r = batched_leg(args, batch_shapes, log)
+        out = {
+            "metric": "syn. events/s & realtime factor, 1 MI355X; plain vs batched-BF16 @<=2% rate err",
+            "value": r["value"], "unit": "syn_events/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": r["rtf"] * DT * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (networks generated on the device from seeds 0-%d)" % (args.nets - 1),
+            "config": {"workload": r["workload"], "description": r["description"], "neurons": r["neurons"],
+                       "synapses": r["synapses"], "simulated_s": args.steps * DT},
+            "rtf": r["rtf"], "batched": r, "roofline": r["roofline"], "cpu_baseline": None,
+        }
+        print(json.dumps(out))
+        return 0
     if args.config == 2:
         pops, projs, desc, ntot = brunel_fixed_k(args.scale, seed=args.seed)
         workload = "brunel100k_fixed_indegree_K10000"
@@ -149,9 +233,8 @@ def main(argv=None):
 
     tuned = None
     tile, block, chunk = args.tile, args.block, args.chunk
-    table = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
-    if not args.tune and not (tile or block or chunk) and args.scale == 1.0 and os.path.exists(table):
-        shapes = json.load(open(table)).get(workload)
+    if not args.tune and not (tile or block or chunk) and args.scale == 1.0:
+        shapes = tuned_table.get(workload)
         if shapes:
             tile, block, chunk = shapes["tile_size"], shapes["block"], shapes["chunk"]
             tuned = dict(best=shapes, source="mgbrain_amd/tuned_shapes.json")
@@ -243,6 +326,10 @@ def main(argv=None):
               file=sys.stderr, flush=True)
     sim.close()
 
+    batched = None
+    if args.config == 2 and not args.no_batch:
+        batched = batched_leg(args, batch_shapes, log)
+
     out = {
         "metric": "syn. events/s & realtime factor, 1 MI355X; plain vs batched-BF16 @<=2% rate err",
         "value": value,
@@ -264,6 +351,7 @@ def main(argv=None):
         "roofline": roof,
         "autotune": tuned,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out))
     return 0

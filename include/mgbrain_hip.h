@@ -171,6 +171,7 @@ typedef struct mgb_info {
     int32_t tile_size;
     int32_t tiles;
     int32_t block;             /* workgroup size of the propagation / fused kernel */
+    int32_t segments;          /* independent segments of the fused path (networks of a batch) */
 } mgb_info;
 
 typedef struct mgb_sim mgb_sim;
@@ -191,6 +192,16 @@ mgb_status mgb_create(const mgb_net_desc* net, const mgb_opts* opts, mgb_sim** o
 /* As mgb_create, with the synapse list generated on the device from
  * population/projection descriptions (new connectors; see mgb_gen_proj). */
 mgb_status mgb_create_generated(const mgb_gen_desc* gen, const mgb_opts* opts, mgb_sim** out);
+
+/* Batched form (no counterpart in the reference, which simulates one network
+ * per MGSimulator): n_nets independent networks in one simulator, advanced by
+ * the same kernel launches.  Network b's neuron ids follow those of network
+ * b-1 (read-back is in that order); its constant sets, Poisson streams and
+ * seeds are its own and every draw uses ids local to it, so each network
+ * evolves exactly as mgb_create_generated(&nets[b]) alone would (CLASS
+ * format).  The networks must share the time step and the STDP settings. */
+mgb_status mgb_create_batch_generated(const mgb_gen_desc* nets, int32_t n_nets, const mgb_opts* opts,
+                                      mgb_sim** out);
 
 /* Advances nsteps time steps.  Replaces MGSimulator::simulate (mgsim.cpp:921-941)
  * -> simulate_sparse (:794-857) -> mgsim_step_sparse (mgcore.cu:535-554). */

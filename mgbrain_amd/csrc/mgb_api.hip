@@ -88,6 +88,15 @@ struct mgb_sim {
     // fused tiled path
     int tiled = 0;
     int32_t B = 0, G = 0, chunk = 8, tblock = 512;
+    int32_t nseg = 1;
+    std::vector<int32_t> h_seg_lo;     // [nseg+1] segment boundaries (independent networks)
+    int32_t* tile_lo = nullptr;        // [G+1]
+    uint16_t* tile_seg = nullptr;      // [G]
+    int32_t *seg_lo = nullptr, *seg_tile0 = nullptr;   // [nseg+1]
+    uint32_t* seg_kbase = nullptr;     // [nseg+1]
+    uint64_t* seg_obase = nullptr;     // [nseg+1]
+    uint64_t n_offs = 0, n_keys = 0;
+    uint32_t* tcnt = nullptr;          // [3][nseg]
     uint32_t* syn_t = nullptr;
     uint16_t* offs16 = nullptr;
     uint32_t* offs32 = nullptr;
@@ -302,26 +311,93 @@ uint64_t host_u64(const void* p) {
 }
 
 // Tile size of the fused path: ~256 tiles to fill the chip, bounded by LDS.
+// Tile shape of the fused path.  Tiles never straddle a segment boundary:
+// segment q gets ceil(Nq / B) tiles of B targets (the last one shorter).
 mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D) {
     int lds_max = 65536;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
     lds_max = std::min(lds_max, 160 * 1024);
-    int32_t bmax = (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0)) / (4 * (int64_t)D));
+    // the kernel's static LDS (block scan sums, event counter) shares the limit
+    int32_t bmax = (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0) - kTiledStaticLds) / (4 * (int64_t)D));
     bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, 1 << 14));
     int32_t B = (o && o->tile_size > 0) ? o->tile_size : std::max<int32_t>(256, (n + 255) / 256);
     B = std::max<int32_t>(1, std::min(B, bmax));
-    B = std::min(B, n);
+    const std::vector<int32_t>& lo = s->h_seg_lo;
+    const int32_t nseg = (int32_t)lo.size() - 1;
+    int32_t seg_max = 0;
+    for (int32_t q = 0; q < nseg; ++q) seg_max = std::max(seg_max, lo[q + 1] - lo[q]);
+    B = std::min(B, std::max(seg_max, 1));
     s->B = B;
-    s->G = (n + B - 1) / B;
     s->chunk = (o && (o->chunk == 4 || o->chunk == 16)) ? o->chunk : 8;
     s->tblock = (o && o->prop_block >= 64 && o->prop_block <= 1024) ? (o->prop_block / 64) * 64 : 512;
-    if ((uint64_t)n * (uint64_t)(s->G + 1) >= 0xFFFFFFFFull) {
-        set_error("tiled propagation: %d neurons x %d tiles exceed 32-bit keys", n, s->G);
-        return MGB_ERR_UNSUPPORTED;
+    std::vector<int32_t> tlo, tile0(nseg + 1, 0);
+    std::vector<uint16_t> tseg;
+    std::vector<uint32_t> kbase(nseg + 1, 0);
+    std::vector<uint64_t> obase(nseg + 1, 0);
+    uint64_t kb = 0, ob = 0;
+    for (int32_t q = 0; q < nseg; ++q) {
+        const int32_t nq = lo[q + 1] - lo[q];
+        const int32_t gq = (nq + B - 1) / B;
+        tile0[q] = (int32_t)tlo.size();
+        kbase[q] = (uint32_t)kb;
+        obase[q] = ob;
+        for (int32_t k = 0; k < gq; ++k) {
+            tlo.push_back(lo[q] + k * B);
+            tseg.push_back((uint16_t)q);
+        }
+        kb += (uint64_t)nq * (uint64_t)gq;
+        ob += (uint64_t)nq * (uint64_t)(gq + 1);
+        if (kb >= 0xFFFFFFFFull) {
+            set_error("tiled propagation: %d neurons x %d tiles exceed 32-bit keys", n,
+                      (int32_t)tlo.size());
+            return MGB_ERR_UNSUPPORTED;
+        }
     }
+    tlo.push_back(n);
+    tile0[nseg] = (int32_t)tseg.size();
+    kbase[nseg] = (uint32_t)kb;
+    obase[nseg] = ob;
+    s->G = (int32_t)tseg.size();
+    s->nseg = nseg;
+    s->n_offs = ob;
+    s->n_keys = kb;
+    hipStream_t st = s->stream;
+    MGB_TRY(dalloc_n(s, &s->tile_lo, tlo.size()));
+    MGB_TRY(dalloc_n(s, &s->tile_seg, tseg.size()));
+    MGB_TRY(dalloc_n(s, &s->seg_lo, lo.size()));
+    MGB_TRY(dalloc_n(s, &s->seg_tile0, tile0.size()));
+    MGB_TRY(dalloc_n(s, &s->seg_kbase, kbase.size()));
+    MGB_TRY(dalloc_n(s, &s->seg_obase, obase.size()));
+    MGB_HIP_TRY(hipMemcpyAsync(s->tile_lo, tlo.data(), 4 * tlo.size(), hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->tile_seg, tseg.data(), 2 * tseg.size(), hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->seg_lo, lo.data(), 4 * lo.size(), hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->seg_tile0, tile0.data(), 4 * tile0.size(), hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->seg_kbase, kbase.data(), 4 * kbase.size(), hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->seg_obase, obase.data(), 8 * obase.size(), hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipStreamSynchronize(st));   // host vectors go out of scope
     const size_t lds = tiled_lds_bytes(D, B);
     if (lds > 65536) MGB_HIP_TRY(set_tiled_lds_limit(lds));
     return MGB_OK;
+}
+
+// Candidate segments: maximal runs of equal Poisson-stream index in id order,
+// when every stream occupies one run.  They become segments of the fused path
+// if no synapse joins two of them (checked by k_stats).
+std::vector<int32_t> stream_runs(const HostNeurons& h, std::vector<uint16_t>& run_of) {
+    std::vector<int32_t> lo{0};
+    run_of.assign((size_t)h.n, 0);
+    std::vector<uint8_t> seen(h.stream_seed.size() + 1, 0);
+    if (h.n > 0) seen[h.stream[0]] = 1;
+    for (int32_t i = 1; i < h.n; ++i) {
+        if (h.stream[i] != h.stream[i - 1]) {
+            if (seen[h.stream[i]] || lo.size() >= 0xFFFFu) return {0, h.n};
+            seen[h.stream[i]] = 1;
+            lo.push_back(i);
+        }
+        run_of[i] = (uint16_t)(lo.size() - 1);
+    }
+    lo.push_back(h.n);
+    return lo;
 }
 
 // Builds device CSR + tables from a device synapse list; frees nothing of `L`.
@@ -350,6 +426,13 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
     MGB_TRY(X.get(&in_pl, (size_t)n));
     MGB_TRY(X.get(&absmax, 1));
     MGB_TRY(X.get(&npl, 1));
+    std::vector<uint16_t> run_of;
+    std::vector<int32_t> runs = stream_runs(h, run_of);
+    uint16_t* seg_dev = nullptr;
+    if (runs.size() > 2) {
+        MGB_TRY(X.get(&seg_dev, (size_t)n));
+        MGB_HIP_TRY(hipMemcpyAsync(seg_dev, run_of.data(), 2 * (size_t)n, hipMemcpyHostToDevice, st));
+    }
     const int32_t mm_init[2] = {0x7FFFFFFF, 0};
     MGB_HIP_TRY(hipMemsetAsync(flags, 0, 4, st));
     MGB_HIP_TRY(hipMemcpyAsync(mm, mm_init, 8, hipMemcpyHostToDevice, st));
@@ -363,6 +446,7 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         a.S = S; a.n = n; a.step = h.step;
         a.src = L.src; a.tar = L.tar; a.weight = L.weight; a.delay = L.delay;
         a.plastic = h.stdp ? L.plastic : nullptr;
+        a.seg = seg_dev;
         a.dq = dq; a.flags = flags; a.min_d = mm; a.max_d = mm + 1;
         a.wcls = wcls; a.indeg = indeg; a.in_pl = in_pl; a.absmax = absmax; a.n_plastic = npl;
         MGB_HIP_TRY(run_stats(a, st));
@@ -391,6 +475,7 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         set_error("a synapse delay exceeds %d steps", kWideMaxDelay);
         return MGB_ERR_UNSUPPORTED;
     }
+    s->h_seg_lo = (hflags & kFlagCrossSeg) ? std::vector<int32_t>{0, n} : runs;
     const int32_t max_d = S ? hmm[1] : 0;
     s->min_delay = S ? hmm[0] : 0;
     s->D = max_d + 1;                                   // mgsim.cpp:71
@@ -502,8 +587,13 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
             MGB_TRY(choose_tiles(s, o, n, s->D));
             // tile-sorted copy: stable sort by (source, target tile) keeps list order
             // inside every (source, tile) sub-row
-            MGB_HIP_TRY(make_tile_keys(L.src, L.tar, L.weight, dq, s->B, s->G, kin, vin, S, st));
-            const int tbits = bits_for((uint64_t)n * (uint64_t)s->G);
+            TileTabs tt{};
+            tt.nseg = s->nseg; tt.B = s->B;
+            tt.seg_of = s->nseg > 1 ? seg_dev : nullptr;
+            tt.seg_lo = s->seg_lo; tt.seg_tile0 = s->seg_tile0;
+            tt.seg_kbase = s->seg_kbase; tt.seg_obase = s->seg_obase;
+            MGB_HIP_TRY(make_tile_keys(L.src, L.tar, L.weight, dq, tt, kin, vin, S, st));
+            const int tbits = bits_for(s->n_keys);
             if (S > 0) {
                 size_t tb2 = 0;
                 MGB_HIP_TRY(sort_pairs(nullptr, &tb2, kin, kout, vin, vout, S, tbits, st));
@@ -529,10 +619,10 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
                 set_error("tiled propagation: a CSR row of %u synapses exceeds 2^26", max_row);
                 return MGB_ERR_UNSUPPORTED;
             }
-            const size_t no = (size_t)n * (size_t)(s->G + 1);
+            const size_t no = (size_t)s->n_offs;
             if (max_row <= 0xFFFFu) MGB_TRY(dalloc_n(s, &s->offs16, no));
             else MGB_TRY(dalloc_n(s, &s->offs32, no));
-            MGB_HIP_TRY(tile_offsets(kout, S, n, s->G, s->row_ptr, s->offs16, s->offs32, st));
+            MGB_HIP_TRY(tile_offsets(kout, s->row_ptr, tt, s->n_offs, s->offs16, s->offs32, st));
         }
         MGB_HIP_TRY(hipStreamSynchronize(st));
     } else {
@@ -665,6 +755,10 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     s->window = std::max(win, 1);
     s->win_stride = (uint32_t)(np * (size_t)s->window);
     MGB_TRY(dalloc_n(s, &s->win_buf, (s->tiled ? 3 : 2) * (size_t)s->win_stride));
+    if (s->tiled) {
+        MGB_TRY(dalloc_n(s, &s->tcnt, 3 * (size_t)s->nseg));
+        MGB_HIP_TRY(hipMemsetAsync(s->tcnt, 0, 12 * (size_t)s->nseg, st));
+    }
     // record
     s->rec_cap = o ? std::max<int64_t>(o->record_capacity, 0) : 0;
     if (s->rec_cap > 0) MGB_TRY(dalloc_n(s, &s->rec, (size_t)s->rec_cap));
@@ -725,6 +819,9 @@ StdpArgs stdp_args(const mgb_sim* s) {
 TiledArgs tiled_args(const mgb_sim* s) {
     TiledArgs a{};
     a.n = s->n; a.D = s->D; a.B = s->B; a.G = s->G; a.chunk = s->chunk;
+    a.nseg = s->nseg; a.window = s->window;
+    a.tile_lo = s->tile_lo; a.tile_seg = s->tile_seg; a.seg_lo = s->seg_lo;
+    a.seg_tile0 = s->seg_tile0; a.seg_obase = s->seg_obase; a.tcnt = s->tcnt;
     a.type = s->type; a.cset = s->cset; a.tbl = s->tbl; a.stream = s->stream_idx;
     a.rate = s->rate; a.V = s->V; a.Ie = s->Ie; a.Ii = s->Ii;
     a.refr = s->refr; a.last = s->last; a.cnt = s->cnt;
@@ -997,41 +1094,105 @@ mgb_status mgb_create(const mgb_net_desc* d, const mgb_opts* o, mgb_sim** out) {
     return MGB_OK;
 }
 
-mgb_status mgb_create_generated(const mgb_gen_desc* g, const mgb_opts* o, mgb_sim** out) {
-    if (!g || !out || g->n_pops <= 0 || !g->pops || (g->n_projs > 0 && !g->projs) || !g->lifconst ||
-        g->n_csets < 1) {
+namespace {
+
+// One population / projection of a (possibly batched) generator description,
+// with indices rebased onto the merged network.
+struct FlatPop {
+    mgb_gen_pop p;
+    uint64_t net_seed;
+};
+struct FlatProj {
+    mgb_gen_proj j;
+    float dw, dt;
+};
+
+// Networks b = 0..nn-1 are laid out one after another: neuron ids, constant
+// sets, Poisson streams and populations of network b follow those of b-1.
+// Every draw (Poisson, connectors, initial V) is keyed by the network's own
+// seeds and by ids local to its streams and populations, so each network
+// evolves exactly as it would alone.
+mgb_status create_generated(const mgb_gen_desc* gs, int32_t nn, const mgb_opts* o, mgb_sim** out) {
+    if (!gs || nn <= 0 || !out) {
         set_error("incomplete generator description");
         return MGB_ERR_INVALID;
     }
     *out = nullptr;
     MGB_TRY(validate_opts(o));
     HostNeurons h;
-    h.step = g->step;
-    h.stdp = g->stdp ? 1 : 0;
-    h.lifconst.assign(g->lifconst, g->lifconst + 30 * (size_t)g->n_csets);
-    if (g->stdpconst) std::memcpy(h.stdpconst, g->stdpconst, sizeof(h.stdpconst));
-    const int32_t nst = std::max(g->n_streams, 1);
-    h.stream_seed.assign(nst, g->seed);
-    if (g->stream_seed)
-        for (int32_t k = 0; k < g->n_streams; ++k) h.stream_seed[k] = g->stream_seed[k];
-    h.stream_base.assign(nst, -1);
-    std::vector<int32_t> first(g->n_pops + 1, 0);
-    int64_t total = 0;
-    for (int32_t p = 0; p < g->n_pops; ++p) {
-        const mgb_gen_pop& P = g->pops[p];
-        if (P.num < 0 || P.stream < 0 || P.stream >= nst || P.cset < 0 || P.cset >= g->n_csets ||
-            P.type < 0 || P.type > MGB_LIFB) {
-            set_error("population %d is invalid", p);
+    h.step = gs[0].step;
+    h.stdp = gs[0].stdp ? 1 : 0;
+    if (gs[0].stdpconst) std::memcpy(h.stdpconst, gs[0].stdpconst, sizeof(h.stdpconst));
+    std::vector<FlatPop> pops;
+    std::vector<FlatProj> projs;
+    for (int32_t b = 0; b < nn; ++b) {
+        const mgb_gen_desc* g = &gs[b];
+        if (g->n_pops <= 0 || !g->pops || (g->n_projs > 0 && !g->projs) || !g->lifconst || g->n_csets < 1) {
+            set_error("network %d: incomplete generator description", b);
             return MGB_ERR_INVALID;
         }
+        float sc[6];
+        std::memcpy(sc, h.stdpconst, sizeof(sc));
+        if (g->stdpconst) std::memcpy(sc, g->stdpconst, sizeof(sc));
+        if (g->step != h.step || (g->stdp ? 1 : 0) != h.stdp ||
+            (h.stdp && std::memcmp(sc, h.stdpconst, sizeof(sc)) != 0)) {
+            set_error("network %d: the networks of a batch must share the time step and the STDP "
+                      "switch and constants", b);
+            return MGB_ERR_INVALID;
+        }
+        const int32_t cs0 = (int32_t)(h.lifconst.size() / 30);
+        if (cs0 + g->n_csets > 255) {
+            set_error("more than 255 constant sets in the batch");
+            return MGB_ERR_UNSUPPORTED;
+        }
+        h.lifconst.insert(h.lifconst.end(), g->lifconst, g->lifconst + 30 * (size_t)g->n_csets);
+        const int32_t st0 = (int32_t)h.stream_seed.size();
+        const int32_t nst = std::max(g->n_streams, 1);
+        if (st0 + nst > 65535) {
+            set_error("more than 65535 Poisson streams in the batch");
+            return MGB_ERR_UNSUPPORTED;
+        }
+        for (int32_t k = 0; k < nst; ++k)
+            h.stream_seed.push_back((g->stream_seed && k < g->n_streams) ? g->stream_seed[k] : g->seed);
+        const int32_t p0 = (int32_t)pops.size();
+        for (int32_t p = 0; p < g->n_pops; ++p) {
+            const mgb_gen_pop& P = g->pops[p];
+            if (P.num < 0 || P.stream < 0 || P.stream >= nst || P.cset < 0 || P.cset >= g->n_csets ||
+                P.type < 0 || P.type > MGB_LIFB) {
+                set_error("network %d: population %d is invalid", b, p);
+                return MGB_ERR_INVALID;
+            }
+            FlatPop f{P, g->net_seed};
+            f.p.cset += cs0;
+            f.p.stream += st0;
+            pops.push_back(f);
+        }
+        for (int32_t j = 0; j < g->n_projs; ++j) {
+            const mgb_gen_proj& J = g->projs[j];
+            if (J.src < 0 || J.src >= g->n_pops || J.tar < 0 || J.tar >= g->n_pops) {
+                set_error("network %d: projection %d references an unknown population", b, j);
+                return MGB_ERR_INVALID;
+            }
+            FlatProj f{J, g->dw, g->dt};
+            f.j.src += p0;
+            f.j.tar += p0;
+            projs.push_back(f);
+        }
+    }
+    const int32_t npops = (int32_t)pops.size(), nprojs = (int32_t)projs.size();
+    const int32_t nst = (int32_t)h.stream_seed.size();
+    h.stream_base.assign(nst, -1);
+    std::vector<int32_t> first(npops + 1, 0);
+    int64_t total = 0;
+    for (int32_t p = 0; p < npops; ++p) {
         first[p] = (int32_t)total;
-        total += P.num;
+        total += pops[p].p.num;
         if (total > kMaxNeurons) {
             set_error("too many neurons");
             return MGB_ERR_UNSUPPORTED;
         }
     }
-    first[g->n_pops] = (int32_t)total;
+    first[npops] = (int32_t)total;
     const int32_t n = (int32_t)total;
     h.n = n;
     h.type.resize(n);
@@ -1039,26 +1200,26 @@ mgb_status mgb_create_generated(const mgb_gen_desc* g, const mgb_opts* o, mgb_si
     h.stream.resize(n);
     h.rate.resize(n);
     h.v0.resize(n);
-    for (int32_t p = 0; p < g->n_pops; ++p) {
-        const mgb_gen_pop& P = g->pops[p];
+    for (int32_t p = 0; p < npops; ++p) {
+        const mgb_gen_pop& P = pops[p].p;
         if (h.stream_base[P.stream] < 0) h.stream_base[P.stream] = first[p];
     }
     for (int32_t k = 0; k < nst; ++k)
         if (h.stream_base[k] < 0) h.stream_base[k] = 0;
-    for (int32_t p = 0; p < g->n_pops; ++p) {
-        const mgb_gen_pop& P = g->pops[p];
+    for (int32_t p = 0; p < npops; ++p) {
+        const mgb_gen_pop& P = pops[p].p;
         for (int32_t j = 0; j < P.num; ++j) {
             const int32_t i = first[p] + j;
             h.type[i] = (uint8_t)P.type;
             h.cset[i] = (uint8_t)P.cset;
             h.stream[i] = (uint16_t)P.stream;
             h.rate[i] = P.rate;
-            h.v0[i] = vinit_draw(g->net_seed, (uint32_t)(i - h.stream_base[P.stream]),
+            h.v0[i] = vinit_draw(pops[p].net_seed, (uint32_t)(i - h.stream_base[P.stream]),
                                  P.v_min, P.v_max);
         }
     }
     // list sizes
-    std::vector<uint64_t> base(g->n_projs + 1, 0);
+    std::vector<uint64_t> base(nprojs + 1, 0);
     mgb_sim* s = new (std::nothrow) mgb_sim();
     if (!s) {
         set_error("host allocation failed");
@@ -1066,16 +1227,11 @@ mgb_status mgb_create_generated(const mgb_gen_desc* g, const mgb_opts* o, mgb_si
     }
     mgb_status st = init_device(s, o);
     Scratch X;
-    std::vector<std::vector<int64_t>> bern_off(g->n_projs);
-    std::vector<uint32_t*> bern_cnt(g->n_projs, nullptr);
-    for (int32_t j = 0; j < g->n_projs && st == MGB_OK; ++j) {
-        const mgb_gen_proj& J = g->projs[j];
-        if (J.src < 0 || J.src >= g->n_pops || J.tar < 0 || J.tar >= g->n_pops) {
-            set_error("projection %d references an unknown population", j);
-            st = MGB_ERR_INVALID;
-            break;
-        }
-        const int32_t sn = g->pops[J.src].num, tn = g->pops[J.tar].num;
+    std::vector<std::vector<int64_t>> bern_off(nprojs);
+    std::vector<uint32_t*> bern_cnt(nprojs, nullptr);
+    for (int32_t j = 0; j < nprojs && st == MGB_OK; ++j) {
+        const mgb_gen_proj& J = projs[j].j;
+        const int32_t sn = pops[J.src].p.num, tn = pops[J.tar].p.num;
         uint64_t cntj = 0;
         if (J.kind == MGB_CONN_FIXED_INDEGREE) {
             if (J.k < 0 || (J.k > 0 && sn == 0)) {
@@ -1116,23 +1272,28 @@ mgb_status mgb_create_generated(const mgb_gen_desc* g, const mgb_opts* o, mgb_si
         }
         base[j + 1] = base[j] + cntj;
     }
+    if (st == MGB_OK && base[nprojs] >= 0xFFFFFFFFull) {
+        set_error("%llu synapses exceed the 2^32-1 limit of one simulator",
+                  (unsigned long long)base[nprojs]);
+        st = MGB_ERR_UNSUPPORTED;
+    }
     if (st == MGB_OK) {
         DevList L;
-        L.S = base[g->n_projs];
+        L.S = base[nprojs];
         st = X.get(&L.src, L.S);
         if (st == MGB_OK) st = X.get(&L.tar, L.S);
         if (st == MGB_OK) st = X.get(&L.weight, L.S);
         if (st == MGB_OK) st = X.get(&L.delay, L.S);
         if (st == MGB_OK) st = X.get(&L.plastic, L.S);
-        for (int32_t j = 0; j < g->n_projs && st == MGB_OK; ++j) {
-            const mgb_gen_proj& J = g->projs[j];
+        for (int32_t j = 0; j < nprojs && st == MGB_OK; ++j) {
+            const mgb_gen_proj& J = projs[j].j;
             GenProjArgs a{};
-            a.sn = g->pops[J.src].num; a.tn = g->pops[J.tar].num;
+            a.sn = pops[J.src].p.num; a.tn = pops[J.tar].p.num;
             a.s0 = first[J.src]; a.t0 = first[J.tar]; a.k = J.k;
             a.index = (uint32_t)J.index; a.seed = J.seed;
             a.thr = (uint64_t)((double)J.p * 4294967296.0);
             a.wmin = J.wmin; a.wmax = J.wmax; a.dmin = J.dmin; a.dmax = J.dmax;
-            a.dw = g->dw; a.dt = g->dt; a.plastic_flag = J.plastic ? 1 : 0;
+            a.dw = projs[j].dw; a.dt = projs[j].dt; a.plastic_flag = J.plastic ? 1 : 0;
             a.base = base[j];
             a.src = L.src; a.tar = L.tar; a.weight = L.weight; a.delay = L.delay; a.plastic = L.plastic;
             hipError_t e;
@@ -1161,6 +1322,17 @@ mgb_status mgb_create_generated(const mgb_gen_desc* g, const mgb_opts* o, mgb_si
     }
     *out = s;
     return MGB_OK;
+}
+
+}  // namespace
+
+mgb_status mgb_create_generated(const mgb_gen_desc* g, const mgb_opts* o, mgb_sim** out) {
+    return create_generated(g, 1, o, out);
+}
+
+mgb_status mgb_create_batch_generated(const mgb_gen_desc* nets, int32_t n_nets, const mgb_opts* o,
+                                      mgb_sim** out) {
+    return create_generated(nets, n_nets, o, out);
 }
 
 mgb_status mgb_run(mgb_sim* s, int32_t nsteps) {
@@ -1256,6 +1428,7 @@ mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
     out->tile_size = s->B;
     out->tiles = s->G;
     out->block = s->tiled ? s->tblock : s->prop_block;
+    out->segments = s->tiled ? s->nseg : 1;
     out->last_run_ms = 0.0;
     if (s->timed) {
         float ms = 0.f;

@@ -9,6 +9,7 @@ constexpr uint32_t kFlagBadId = 1u;
 constexpr uint32_t kFlagDelayLow = 2u;         // quantised delay < 1 step
 constexpr uint32_t kFlagDelayHigh = 4u;        // quantised delay > kWideMaxDelay
 constexpr uint32_t kFlagMixed = 8u;            // a target sees two weights in one class
+constexpr uint32_t kFlagCrossSeg = 16u;        // a synapse joins two candidate segments
 
 struct StatArgs {
     uint64_t S;
@@ -19,6 +20,7 @@ struct StatArgs {
     const float* weight;
     const float* delay;
     const uint8_t* plastic;
+    const uint16_t* seg;    // [n] candidate segment of each neuron (NULL: not checked)
     uint8_t* dq;            // [S] quantised delay
     uint32_t* flags;
     int32_t* min_d;
@@ -28,6 +30,16 @@ struct StatArgs {
     uint32_t* in_pl;        // [n] plastic in-degree
     uint32_t* absmax;       // max |w| (float bits)
     unsigned long long* n_plastic;
+};
+
+// Segment / tile tables of the tiled layout (device pointers; see TiledArgs).
+struct TileTabs {
+    int32_t nseg, B;
+    const uint16_t* seg_of;     // [n] segment of each neuron (NULL: one segment)
+    const int32_t* seg_lo;      // [nseg+1]
+    const int32_t* seg_tile0;   // [nseg+1]
+    const uint32_t* seg_kbase;  // [nseg+1] first sort key of each segment
+    const uint64_t* seg_obase;  // [nseg+1] first offset entry of each segment
 };
 
 struct GatherArgs {
@@ -68,9 +80,9 @@ hipError_t make_sort_input(const int32_t* src, const int32_t* tar, const float* 
                            bool class_payload, uint32_t* keys, uint32_t* vals, uint64_t S,
                            hipStream_t st);
 hipError_t make_tile_keys(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
-                          int32_t B, int32_t G, uint32_t* keys, uint32_t* vals, uint64_t S, hipStream_t st);
-hipError_t tile_offsets(const uint32_t* sorted_keys, uint64_t S, int32_t n, int32_t G, const int64_t* row_ptr,
-                        uint16_t* offs16, uint32_t* offs32, hipStream_t st);
+                          const TileTabs& tt, uint32_t* keys, uint32_t* vals, uint64_t S, hipStream_t st);
+hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, const TileTabs& tt,
+                        uint64_t total, uint16_t* offs16, uint32_t* offs32, hipStream_t st);
 hipError_t fp8_row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n,
                           uint32_t* rmax_tmp, float* scale, hipStream_t st);
 hipError_t make_in_keys(const int32_t* tar, const uint8_t* plastic, int32_t n, uint32_t* keys,

@@ -41,6 +41,7 @@ __global__ void k_stats(StatArgs a) {
             if ((uint32_t)s >= (uint32_t)a.n || (uint32_t)t >= (uint32_t)a.n) {
                 atomicOr(a.flags, kFlagBadId);
             } else {
+                if (a.seg && a.seg[s] != a.seg[t]) atomicOr(a.flags, kFlagCrossSeg);
                 const float q = a.delay[i] / a.step;                 // mgsim.cpp:49
                 const int d = (int)roundf(q);
                 if (d < 1) atomicOr(a.flags, kFlagDelayLow);
@@ -96,34 +97,52 @@ __global__ void k_class_keys(const int32_t* src, const int32_t* tar, const float
     }
 }
 
-// Tile-sorted copy of the CLASS CSR: key = source * G + target tile.
+// Tile-sorted copy of the CLASS CSR.  Source s of segment q (which has Gq
+// tiles of B targets from seg_lo[q]) gets keys seg_kbase[q] + (s - seg_lo[q]) *
+// Gq + local tile of the target: sorted, they order each row by tile and keep
+// the rows in source order.
 __global__ void k_tile_keys(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
-                            int32_t B, int32_t G, uint32_t* keys, uint32_t* vals, uint64_t S) {
+                            TileTabs tt, uint32_t* keys, uint32_t* vals, uint64_t S) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        keys[i] = (uint32_t)src[i] * (uint32_t)G + (uint32_t)(tar[i] / B);
+        const int32_t s = src[i], t = tar[i];
+        const int32_t q = tt.seg_of ? (int32_t)tt.seg_of[s] : 0;
+        const int32_t lo = tt.seg_lo[q];
+        const uint32_t gq = (uint32_t)(tt.seg_tile0[q + 1] - tt.seg_tile0[q]);
+        keys[i] = tt.seg_kbase[q] + (uint32_t)(s - lo) * gq + (uint32_t)((t - lo) / tt.B);
         const uint32_t inh = (w[i] > 0.f) ? 0u : 1u;
-        vals[i] = (uint32_t)tar[i] | ((uint32_t)(dq[i] - 1u) << 26) | (inh << 31);
+        vals[i] = (uint32_t)t | ((uint32_t)(dq[i] - 1u) << 26) | (inh << 31);
     }
 }
 
-// offs[m][t] = first slot of (source m, tile t) relative to row m, t in [0, G].
+// Offset entry e = seg_obase[q] + (m - seg_lo[q]) * (Gq + 1) + t, t in [0, Gq]:
+// first slot of (source m, local tile t) relative to row m (t = Gq: the row length).
 template <class T>
-__global__ void k_tile_offsets(const uint32_t* keys, uint64_t S, int32_t n, int32_t G,
-                               const int64_t* row_ptr, T* offs) {
-    const uint64_t total = (uint64_t)n * (uint64_t)(G + 1);
-    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-         q += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t m = (uint32_t)(q / (uint64_t)(G + 1));
-        const uint32_t t = (uint32_t)(q % (uint64_t)(G + 1));
-        const uint64_t key = (uint64_t)m * (uint64_t)G + t;
-        uint64_t lo = 0, hi = S;
-        while (lo < hi) {
-            const uint64_t mid = (lo + hi) >> 1;
-            if ((uint64_t)keys[mid] < key) lo = mid + 1;
+__global__ void k_tile_offsets(const uint32_t* keys, const int64_t* row_ptr, TileTabs tt, uint64_t total,
+                               T* offs) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        int32_t lo = 0, hi = tt.nseg;          // seg_obase[lo] <= e < seg_obase[hi]
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (tt.seg_obase[mid] <= e) lo = mid;
             else hi = mid;
         }
-        offs[q] = (T)(lo - (uint64_t)row_ptr[m]);
+        const int32_t q = lo;
+        const uint64_t r = e - tt.seg_obase[q];
+        const uint64_t gq = (uint64_t)(tt.seg_tile0[q + 1] - tt.seg_tile0[q]);
+        const uint64_t rel = r / (gq + 1);
+        const uint32_t m = (uint32_t)tt.seg_lo[q] + (uint32_t)rel;
+        const uint64_t key = (uint64_t)tt.seg_kbase[q] + rel * gq + (r - rel * (gq + 1));
+        // row m's keys lie in [row_ptr[m], row_ptr[m+1])
+        uint64_t a = (uint64_t)row_ptr[m], b = (uint64_t)row_ptr[m + 1];
+        const uint64_t a0 = a;
+        while (a < b) {
+            const uint64_t mid = (a + b) >> 1;
+            if ((uint64_t)keys[mid] < key) a = mid + 1;
+            else b = mid;
+        }
+        offs[e] = (T)(a - a0);
     }
 }
 
@@ -297,22 +316,22 @@ hipError_t make_sort_input(const int32_t* src, const int32_t* tar, const float* 
 }
 
 hipError_t make_tile_keys(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
-                          int32_t B, int32_t G, uint32_t* keys, uint32_t* vals, uint64_t S, hipStream_t st) {
+                          const TileTabs& tt, uint32_t* keys, uint32_t* vals, uint64_t S, hipStream_t st) {
     if (S == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_keys, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, tar, w, dq, B, G, keys,
+    hipLaunchKernelGGL(k_tile_keys, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, tar, w, dq, tt, keys,
                        vals, S);
     return hipGetLastError();
 }
 
-hipError_t tile_offsets(const uint32_t* sorted_keys, uint64_t S, int32_t n, int32_t G, const int64_t* row_ptr,
-                        uint16_t* offs16, uint32_t* offs32, hipStream_t st) {
-    const uint64_t total = (uint64_t)n * (uint64_t)(G + 1);
+hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, const TileTabs& tt,
+                        uint64_t total, uint16_t* offs16, uint32_t* offs32, hipStream_t st) {
+    if (total == 0) return hipSuccess;
     if (offs16)
         hipLaunchKernelGGL(k_tile_offsets<uint16_t>, dim3(blocks_for(total, 256)), dim3(256), 0, st,
-                           sorted_keys, S, n, G, row_ptr, offs16);
+                           sorted_keys, row_ptr, tt, total, offs16);
     else
         hipLaunchKernelGGL(k_tile_offsets<uint32_t>, dim3(blocks_for(total, 256)), dim3(256), 0, st,
-                           sorted_keys, S, n, G, row_ptr, offs32);
+                           sorted_keys, row_ptr, tt, total, offs32);
     return hipGetLastError();
 }
 

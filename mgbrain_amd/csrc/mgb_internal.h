@@ -49,9 +49,8 @@ struct Ctl {
     int32_t pad;
     uint64_t rec_cnt;       // spikes recorded (may exceed capacity)
     uint64_t se;            // synaptic events
-    // fused tiled path: three spike lists (write k%3, read (k-1)%3, clear (k+1)%3)
-    uint32_t tcnt[3];
-    uint32_t pad2;
+    // fused tiled path: three spike lists (write k%3, read (k-1)%3, clear (k+1)%3);
+    // their per-segment counters live in TiledArgs::tcnt
     int64_t tbase[3];       // first step of the window that wrote each list
 };
 
@@ -131,13 +130,27 @@ struct StdpArgs {
     uint32_t win_stride;
 };
 
-// Fused tiled window kernel: workgroup g owns targets [g*B, g*B+B) and keeps
-// their ring slice ring_t[g][D][B] in LDS while it (1) adds the previous
-// window's spikes arriving in the tile (tile-sorted CSR copy syn_t, per-row
-// tile offsets offs[src][G+1]) and (2) advances the tile's neurons.
+// Fused tiled window kernel: workgroup g owns targets [tile_lo[g], tile_lo[g+1])
+// (at most B) and keeps their ring slice ring_t[g][D][B] in LDS while it (1)
+// adds the previous window's spikes arriving in the tile (tile-sorted CSR copy
+// syn_t, per-row tile offsets) and (2) advances the tile's neurons.
+//
+// Segments: contiguous id ranges with no synapse between them (independent
+// networks of a batch; one segment for a single network).  Every tile lies in
+// one segment, each segment has its own spike lists, and a source row of
+// segment q holds offsets for q's tiles only: offs[seg_obase[q] +
+// (src - seg_lo[q]) * (Gq + 1) + local tile], Gq = tiles of q.
 struct TiledArgs {
     int32_t n, D, B, G;
     int32_t chunk;          // synapses per work item in the propagation phase
+    int32_t nseg;
+    int32_t window;         // spike-list entries reserved per neuron (steps per window)
+    const int32_t* tile_lo;     // [G+1]
+    const uint16_t* tile_seg;   // [G]
+    const int32_t* seg_lo;      // [nseg+1] first neuron of each segment
+    const int32_t* seg_tile0;   // [nseg+1] first tile of each segment
+    const uint64_t* seg_obase;  // [nseg+1] first offset entry of each segment
+    uint32_t* tcnt;             // [3][nseg] entries in each segment's spike lists
     const uint8_t* type;
     const uint8_t* cset;
     const uint16_t* tbl;
@@ -178,6 +191,8 @@ hipError_t launch_prop(int mode, int wide_kind, const PropArgs& a, int32_t off, 
 hipError_t launch_stdp(const StdpArgs& a, int32_t off, int32_t wk, int grid, hipStream_t st);
 hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
 constexpr int kTiledBatch = 1024;   // spikes staged in LDS per propagation batch
+constexpr int kTiledItems = 8192;   // work items staged in LDS per propagation round
+constexpr int kTiledStaticLds = 256; // bytes reserved for the fused kernel's static LDS
 size_t tiled_lds_bytes(int D, int B);
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st);

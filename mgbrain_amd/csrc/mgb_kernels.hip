@@ -391,9 +391,10 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* wsum) 
     return v;
 }
 
-// One window of the fused tiled step (CLASS format).  Workgroup g:
+// One window of the fused tiled step (CLASS format).  Workgroup g (targets
+// [tile_lo[g], tile_lo[g+1]) of segment tile_seg[g]):
 //   1. loads its ring slice ring_t[g] (D x B words) into LDS;
-//   2. adds the previous window's spikes: for each spiker, the sub-row of its
+//   2. adds the previous window's spikes of its segment: for each spiker, the sub-row of its
 //      tile-sorted CSR row that targets this tile, split into work items of
 //      `chunk` synapses balanced over the block, one LDS atomic per synapse;
 //   3. advances the tile's neurons through this window's steps reading and
@@ -410,19 +411,26 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const int32_t D = a.D, B = a.B, g = (int32_t)blockIdx.x, bd = (int32_t)blockDim.x;
     const int32_t tid = (int32_t)threadIdx.x;
     uint32_t* ring = lds;                                   // [D][B]
-    uint32_t* sp_beg = ring + (size_t)D * B;                // [kTiledBatch]
+    uint32_t* sp_beg = ring + (size_t)D * B;                // [kTiledBatch] first CSR slot
     uint32_t* sp_len = sp_beg + kTiledBatch;                // len | slot-of-spike << 26
-    uint32_t* sp_pre = sp_len + kTiledBatch;                // inclusive prefix of work items
+    uint32_t* sp_start = sp_len + kTiledBatch;              // exclusive prefix of work items
+    uint16_t* itm = reinterpret_cast<uint16_t*>(sp_start + kTiledBatch);   // [kTiledItems] spike of item
     Ctl* ctl = a.ctl;
     const int64_t s0 = ctl->base_step + off;
     const int64_t kwin = ctl->base_window + wk;
     const int bcur = (int)(kwin % 3), bprev = (int)((kwin + 2) % 3), bnext = (int)((kwin + 1) % 3);
-    const int32_t t0 = g * B;
-    const int32_t tn = min(B, a.n - t0);
-    if (g == 0 && tid == 0) {
-        ctl->tcnt[bnext] = 0u;
-        ctl->tbase[bcur] = s0;
-    }
+    const int32_t t0 = a.tile_lo[g];
+    const int32_t tn = a.tile_lo[g + 1] - t0;
+    const int32_t q = (int32_t)a.tile_seg[g];
+    const int32_t lt = g - a.seg_tile0[q];                  // tile index inside the segment
+    const int32_t slo = a.seg_lo[q];
+    const size_t G1 = (size_t)(a.seg_tile0[q + 1] - a.seg_tile0[q] + 1);
+    const size_t obase = (size_t)a.seg_obase[q] + (size_t)lt;
+    const size_t lbase = (size_t)slo * (size_t)a.window;    // segment's part of each spike list
+    uint32_t* tcnt = a.tcnt + q;                            // tcnt[b * nseg]
+    const int32_t nseg = a.nseg;
+    if (lt == 0 && tid == 0) tcnt[bnext * nseg] = 0u;
+    if (g == 0 && tid == 0) ctl->tbase[bcur] = s0;
     if (tid == 0) se_blk = 0ull;
     // 1. ring slice -> LDS
     uint32_t* gring = a.ring_t + (size_t)g * (size_t)D * (size_t)B;
@@ -430,17 +438,16 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     __syncthreads();
 
     // 2. previous window's spikes into the tile
-    const uint32_t E = ctl->tcnt[bprev];
+    const uint32_t E = tcnt[bprev * nseg];
     const int64_t sprev = ctl->tbase[bprev];
-    const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride;
-    const int32_t G1 = a.G + 1;
+    const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride + lbase;
     for (uint32_t b0 = 0; b0 < E; b0 += (uint32_t)bd) {
         const uint32_t nb = min((uint32_t)bd, E - b0);
         uint32_t items = 0u;
         if ((uint32_t)tid < nb) {
             const uint32_t ent = L[b0 + tid];
             const uint32_t src = ent & kIdMask;
-            const size_t ob = (size_t)src * (size_t)G1 + (size_t)g;
+            const size_t ob = obase + (size_t)(src - (uint32_t)slo) * G1;
             uint32_t o0, o1;
             if (a.offs16) { o0 = a.offs16[ob]; o1 = a.offs16[ob + 1]; }
             else { o0 = a.offs32[ob]; o1 = a.offs32[ob + 1]; }
@@ -451,41 +458,58 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             items = (len + CH - 1) / CH;
         }
         const uint32_t incl = block_incl_scan(items, wsum);
-        if ((uint32_t)tid < nb) sp_pre[tid] = incl;
-        __syncthreads();
-        const uint32_t total = sp_pre[nb - 1];
-        for (uint32_t it = (uint32_t)tid; it < total; it += (uint32_t)bd) {
-            uint32_t lo = 0, hi = nb - 1;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (sp_pre[mid] > it) hi = mid;
-                else lo = mid + 1;
+        const uint32_t total = wsum[(blockDim.x >> 6) - 1];
+        if ((uint32_t)tid < nb) sp_start[tid] = incl - items;
+        for (uint32_t r0 = 0; r0 < total; r0 += kTiledItems) {
+            const uint32_t rn = min((uint32_t)kTiledItems, total - r0);
+            // item table: each spike writes its own work items of this round
+            if ((uint32_t)tid < nb) {
+                const uint32_t q0 = max(incl - items, r0), q1 = min(incl, r0 + rn);
+                for (uint32_t q = q0; q < q1; ++q) itm[q - r0] = (uint16_t)tid;
             }
-            const uint32_t c = it - (lo ? sp_pre[lo - 1] : 0u);
-            const uint32_t meta = sp_len[lo];
-            const uint32_t len = meta & kIdMask;
-            const int32_t sD = (int32_t)(meta >> 26);
-            const uint32_t kb = sp_beg[lo] + c * CH;
-            const uint32_t cnt = min((uint32_t)CH, len - c * CH);
-            uint32_t x[CH];
+            __syncthreads();
+            for (uint32_t it = (uint32_t)tid; it < rn; it += 2u * (uint32_t)bd) {
+                const uint32_t it2 = it + (uint32_t)bd;
+                uint32_t cnt[2] = {0u, 0u}, kb[2] = {0u, 0u};
+                int32_t sD[2] = {0, 0};
 #pragma unroll
-            for (int u = 0; u < CH; ++u)
-                if ((uint32_t)u < cnt) x[u] = __builtin_nontemporal_load(&a.syn_t[kb + u]);
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                if ((uint32_t)u < cnt) {
-                    int32_t slot = sD + (int32_t)((x[u] >> 26) & 31u) + 1;
-                    slot -= (slot >= D) ? D : 0;
-                    const int32_t loc = (int32_t)(x[u] & kIdMask) - t0;
-                    atomicAdd(&ring[slot * B + loc], (x[u] >> 31) ? 0x10000u : 1u);
+                for (int h = 0; h < 2; ++h) {
+                    const uint32_t q = h ? it2 : it;
+                    if (q < rn) {
+                        const uint32_t j = itm[q];
+                        const uint32_t c = r0 + q - sp_start[j];
+                        const uint32_t meta = sp_len[j];
+                        const uint32_t len = meta & kIdMask;
+                        sD[h] = (int32_t)(meta >> 26);
+                        kb[h] = sp_beg[j] + c * CH;
+                        cnt[h] = min((uint32_t)CH, len - c * CH);
+                    }
                 }
+                uint32_t x[2][CH];
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int u = 0; u < CH; ++u)
+                        if ((uint32_t)u < cnt[h]) x[h][u] = __builtin_nontemporal_load(&a.syn_t[kb[h] + u]);
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int u = 0; u < CH; ++u) {
+                        if ((uint32_t)u < cnt[h]) {
+                            int32_t slot = sD[h] + (int32_t)((x[h][u] >> 26) & 31u) + 1;
+                            slot -= (slot >= D) ? D : 0;
+                            const int32_t loc = (int32_t)(x[h][u] & kIdMask) - t0;
+                            atomicAdd(&ring[slot * B + loc], (x[h][u] >> 31) ? 0x10000u : 1u);
+                        }
+                    }
             }
+            __syncthreads();
         }
-        __syncthreads();
+        __syncthreads();   // wsum / sp_* are rewritten by the next batch
     }
 
     // 3. neurons of the tile through this window's steps
-    uint32_t* wbuf = a.win_buf + (size_t)bcur * a.win_stride;
+    uint32_t* wbuf = a.win_buf + (size_t)bcur * a.win_stride + lbase;
     const int32_t slot0 = (int32_t)(s0 % D);
     unsigned long long se_local = 0ull;
     for (int32_t j0 = 0; j0 < B; j0 += bd) {
@@ -520,7 +544,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                 uint32_t base = 0u;
                 unsigned long long rb = 0ull;
                 if (lane_id() == leader) {
-                    base = atomicAdd(&ctl->tcnt[bcur], pc);
+                    base = atomicAdd(&tcnt[bcur * nseg], pc);
                     rb = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)pc);
                 }
                 base = __shfl(base, (int)leader);
@@ -594,7 +618,8 @@ hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t
 }
 
 size_t tiled_lds_bytes(int D, int B) {
-    return sizeof(uint32_t) * ((size_t)D * (size_t)B + 3 * (size_t)kTiledBatch);
+    return sizeof(uint32_t) * ((size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
+           sizeof(uint16_t) * (size_t)kTiledItems;
 }
 
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,

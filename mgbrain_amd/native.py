@@ -22,7 +22,8 @@ PROP_AUTO, PROP_DIRECT, PROP_TILED = range(3)
 CONN_BERNOULLI, CONN_FIXED_INDEGREE = 1, 2
 F_V, F_IEXC, F_IINH, F_REFR, F_LAST, F_CNT = range(6)
 
-EXPORTS = ["mgb_abi_version", "mgb_last_error", "mgb_create", "mgb_create_generated", "mgb_run",
+EXPORTS = ["mgb_abi_version", "mgb_last_error", "mgb_create", "mgb_create_generated",
+           "mgb_create_batch_generated", "mgb_run",
            "mgb_sync", "mgb_set_launch", "mgb_get_info", "mgb_read_spikes", "mgb_clear_spikes",
            "mgb_read_state", "mgb_fire_counts", "mgb_read_csr", "mgb_read_weights", "mgb_destroy"]
 
@@ -72,7 +73,7 @@ class Info(C.Structure):
                 ("spikes", C.c_int64), ("neuron_ms", C.c_double), ("prop_ms", C.c_double),
                 ("stdp_ms", C.c_double), ("neuron_launches", C.c_int64), ("prop_launches", C.c_int64),
                 ("stdp_launches", C.c_int64), ("propagation", C.c_int32), ("tile_size", C.c_int32),
-                ("tiles", C.c_int32), ("block", C.c_int32)]
+                ("tiles", C.c_int32), ("block", C.c_int32), ("segments", C.c_int32)]
 
 
 _lib = None
@@ -97,6 +98,7 @@ def lib():
                 getattr(L, name).restype = C.c_int
         L.mgb_create.argtypes = [C.POINTER(NetDesc), C.POINTER(Opts), C.POINTER(_p)]
         L.mgb_create_generated.argtypes = [C.POINTER(GenDesc), C.POINTER(Opts), C.POINTER(_p)]
+        L.mgb_create_batch_generated.argtypes = [C.POINTER(GenDesc), C.c_int32, C.POINTER(Opts), C.POINTER(_p)]
         L.mgb_run.argtypes = [_p, C.c_int32]
         L.mgb_sync.argtypes = [_p]
         L.mgb_set_launch.argtypes = [_p, C.c_int32, C.c_int32]
@@ -203,11 +205,10 @@ class Simulator:
         _check(lib().mgb_create(C.byref(d), C.byref(o), C.byref(h)))
         return cls(h)
 
-    @classmethod
-    def generated(cls, pops, projs, lifconst, step=1e-4, dt=1e-4, dw=0.01, stdpconst=None, stdp=False,
-                  seed=0, stream_seed=None, net_seed=None, **opts):
-        """pops: list of dicts (num, type, rate, v_min, v_max, cset, stream);
-        projs: list of dicts (src, tar, kind, p, k, wmin, wmax, dmin, dmax, plastic, index, seed)."""
+    @staticmethod
+    def _gen_desc(g, keep, pops, projs, lifconst, step=1e-4, dt=1e-4, dw=0.01, stdpconst=None, stdp=False,
+                  seed=0, stream_seed=None, net_seed=None):
+        """Fills GenDesc g; the arrays it points to are appended to keep."""
         P = (GenPop * len(pops))()
         for i, q in enumerate(pops):
             P[i] = GenPop(q["num"], q["type"], q.get("rate", 0.0), q.get("v_min", 0.0), q.get("v_max", 0.0),
@@ -222,7 +223,7 @@ class Simulator:
         sc = np.ascontiguousarray(stdpconst if stdpconst is not None else [0.1, -0.01, 17, 34, 40, 0],
                                   dtype=np.float32)
         ss = None if stream_seed is None else np.ascontiguousarray(stream_seed, dtype=np.uint64)
-        g = GenDesc()
+        keep.extend([P, J, lc, sc, ss])
         g.n_pops = len(pops)
         g.pops = C.cast(P, _p)
         g.n_projs = len(projs)
@@ -238,9 +239,32 @@ class Simulator:
         g.stdp = int(bool(stdp))
         g.seed = seed
         g.net_seed = seed if net_seed is None else net_seed
+
+    @classmethod
+    def generated(cls, pops, projs, lifconst, step=1e-4, dt=1e-4, dw=0.01, stdpconst=None, stdp=False,
+                  seed=0, stream_seed=None, net_seed=None, **opts):
+        """pops: list of dicts (num, type, rate, v_min, v_max, cset, stream);
+        projs: list of dicts (src, tar, kind, p, k, wmin, wmax, dmin, dmax, plastic, index, seed)."""
+        g, keep = GenDesc(), []
+        cls._gen_desc(g, keep, pops, projs, lifconst, step, dt, dw, stdpconst, stdp, seed, stream_seed,
+                      net_seed)
         o = make_opts(**opts)
         h = _p()
         _check(lib().mgb_create_generated(C.byref(g), C.byref(o), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def batch_generated(cls, nets, **opts):
+        """nets: list of dicts with the keyword arguments of generated() (pops, projs,
+        lifconst, step, dt, dw, stdpconst, stdp, seed, stream_seed, net_seed).  Neuron
+        ids of network b follow those of network b-1."""
+        G = (GenDesc * len(nets))()
+        keep = []
+        for b, net in enumerate(nets):
+            cls._gen_desc(G[b], keep, **net)
+        o = make_opts(**opts)
+        h = _p()
+        _check(lib().mgb_create_batch_generated(G, len(nets), C.byref(o), C.byref(h)))
         return cls(h)
 
     # -- stepping -------------------------------------------------------------

@@ -17,8 +17,8 @@ import math
 import time
 
 
-def tile_candidates(n, ring_len, lds_bytes=160 * 1024, batch_words=3 * 1024):
-    bmax = max(1, (lds_bytes // 4 - batch_words) // max(1, ring_len))
+def tile_candidates(n, ring_len, lds_bytes=160 * 1024, batch_words=3 * 1024 + 4096, static_bytes=256):
+    bmax = max(1, ((lds_bytes - static_bytes) // 4 - batch_words) // max(1, ring_len))
     out = []
     for g in (128, 256, 512, 1024, 2048):
         b = min(bmax, max(32, math.ceil(n / g)))
