@@ -157,6 +157,66 @@ def batched_leg(args, shapes=None, log=None):
     }
 
 
+def stdp_leg(args, log):
+    """Brunel 100k (K = 10,000) with pair-based STDP on the E->E synapses (window of one
+    step; k_neurons + k_prop + k_stdp per step).  Static weights in the format chosen
+    by --precision (bf16 default), plastic weights FP32."""
+    pops, projs, desc, ntot = W.brunel_fixed_k(args.scale, seed=args.seed, plastic_ee=True)
+    sc, lc = W.stdp_consts(), W.lifb_consts()
+    wf = N.WF_BF16 if args.precision == "bf16" else N.WF_F32
+    t0 = time.perf_counter()
+    sim = N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed, stdp=True, stdpconst=sc,
+                                weight_format=wf, use_graph=args.graph, profile=0 if args.no_profile else 1)
+    sim.sync()
+    setup_s = time.perf_counter() - t0
+    info0 = sim.info()
+    log(f"# stdp: setup {setup_s:.2f} s, {info0['n_synapses']} synapses, window {info0['window']}, "
+        f"{info0['device_bytes'] / 2**30:.2f} GiB")
+    sim.run(args.warmup)
+    sim.sync()
+    a = sim.info()
+    t1 = time.perf_counter()
+    sim.run(args.steps)
+    sim.sync()
+    wall = time.perf_counter() - t1
+    b = sim.info()
+    se = b["syn_events"] - a["syn_events"]
+    spikes = b["spikes"] - a["spikes"]
+    log(f"# stdp: {args.steps} steps in {wall:.2f} s, {se:.4g} events, {se / wall:.4g} SE/s, RTF "
+        f"{wall / (args.steps * DT):.3f}; neuron {b['neuron_ms']:.0f} ms, prop {b['prop_ms']:.0f} ms, "
+        f"stdp {b['stdp_ms']:.0f} ms")
+    rp, col, _dl, w = sim.csr()
+    sim.close()
+    nP, nE = pops[0]["num"], pops[1]["num"]
+    lo, hi = int(rp[nP]), int(rp[nP + nE])
+    c, ww = col[lo:hi], w[lo:hi]
+    wee = ww[(c >= nP) & (c < nP + nE)]
+    del rp, col, w, c, ww
+    hist, edges = np.histogram(wee, bins=20, range=(0.0, float(sc[4])))
+    roof = None
+    if b["prop_launches"]:
+        # k_prop per event: target 4 B + delay 1 B + weight (2 B static BF16 / 4 B plastic
+        # FP32, plastic share of events taken as the plastic share of synapses); 12 B per spike
+        pl = info0.get("n_plastic", 0) or len(wee)
+        wbytes = 4.0 if wf == N.WF_F32 else 2.0 + 2.0 * pl / max(1, info0["n_synapses"])
+        alg = (5.0 + wbytes) * se + 12.0 * spikes
+        avg_s = b["prop_ms"] / 1e3 / b["prop_launches"]
+        ach = alg / b["prop_launches"] / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": "k_prop (wide format)", "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "avg_launch_us": round(avg_s * 1e6, 3), "traffic": None}
+    return {
+        "workload": "brunel100k_K10000_stdp_ee", "description": desc + "; STDP on E->E (A+ 0.01, A- 0.0105 "
+        "x W_max, tau 20 ms, W in [0, 0.2 mV])", "value": se / wall, "unit": "syn_events/s",
+        "rtf": wall / (args.steps * DT), "simulated_s": args.steps * DT, "static_weights": args.precision,
+        "plastic_weights": "f32", "neuron_ms": b["neuron_ms"], "prop_ms": b["prop_ms"], "stdp_ms": b["stdp_ms"],
+        "mean_rate_hz": spikes / ntot / (args.steps * DT), "setup_s": round(setup_s, 2),
+        "ee_weights": {"n": int(wee.size), "mean": float(wee.mean()) if wee.size else 0.0,
+                       "hist_edges": [float(x) for x in edges], "hist": [int(x) for x in hist]},
+        "roofline": roof,
+    }
+
+
 def tune_batched(args, log):
     """Autotunes the batched sweep's launch shapes (each tile size builds the batch anew)."""
     _, nets = W.sweep_nets(args.nets)
@@ -167,18 +227,20 @@ def tune_batched(args, log):
         return None
     cands = T.tile_candidates(max(W.net_size(x) for x in nets), pinfo["ring_len"])
     best, res = T.autotune(lambda b: N.Simulator.batch_generated(nets, tile_size=b), cands,
-                           blocks=(256, 512, 1024), chunks=(4, 8), log=lambda m: log("# " + m))
+                           blocks=(256, 512, 1024), chunks=(1, 2, 4), log=lambda m: log("# " + m))
     return dict(best, candidates=len(res)) if best else None
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default 10,000; 50,000 for --config 4)")
     ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 5),
+    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 4, 5),
                     help="1: Brunel 10k Bernoulli; 2: Brunel 100k K=10000 (default, plus the batched leg); "
-                         "5: batched sweep only")
+                         "4: config 2 with STDP on E->E; 5: batched sweep only")
+    ap.add_argument("--precision", default="bf16", choices=("bf16", "f32"),
+                    help="static weight storage of --config 4")
     ap.add_argument("--nets", type=int, default=64, help="networks in the batched sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg of the default run")
     ap.add_argument("--scale", type=float, default=1.0)
@@ -199,6 +261,21 @@ def main(argv=None):
         return 2
 
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if not args.steps:
+        args.steps = 50000 if args.config == 4 else 10000
+    if args.config == 4:
+        r = stdp_leg(args, log)
+        out = {
+            "metric": "syn. events/s & realtime factor, 1 MI355X; plain vs batched-BF16 @<=2% rate err",
+            "value": r["value"], "unit": "syn_events/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": r["rtf"] * DT * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (network generated on the device from seed %d)" % args.seed,
+            "config": {"workload": r["workload"], "description": r["description"], "simulated_s": r["simulated_s"]},
+            "rtf": r["rtf"], "stdp": r, "roofline": r["roofline"], "cpu_baseline": None,
+        }
+        print(json.dumps(out))
+        return 0
     table = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
     tuned_table = json.load(open(table)) if os.path.exists(table) else {}
     batch_shapes = None if args.tune else tuned_table.get(f"brunel_sweep{args.nets}")

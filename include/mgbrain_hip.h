@@ -62,7 +62,7 @@ typedef struct mgb_opts {
     int32_t profile;           /* time every kernel with hipEvents (direct launches only) */
     int32_t propagation;       /* MGB_PROP_* */
     int32_t tile_size;         /* TILED: target neurons per workgroup; 0 = tuned default */
-    int32_t chunk;             /* TILED: synapses per propagation work item (4, 8, 16) */
+    int32_t chunk;             /* TILED: synapses per propagation work item (1, 2, 4, 8, 16; 0 = 8) */
     int32_t reserved[4];
 } mgb_opts;
 
@@ -164,7 +164,9 @@ typedef struct mgb_info {
     double last_run_ms;        /* device time of the last mgb_run (hipEvents) */
     int64_t launches;          /* kernel launches issued so far (a graph replay counts once) */
     int64_t spikes;            /* spikes emitted so far */
-    /* per-kernel device time of the last mgb_run when opts.profile is set */
+    /* per-kernel device time of the last mgb_run when opts.profile is set; with
+     * use_graph on the fused path a replay's time counts as its windows' fused
+     * launches (the graph's one finalize launch included) */
     double neuron_ms, prop_ms, stdp_ms;
     int64_t neuron_launches, prop_launches, stdp_launches;
     int32_t propagation;       /* MGB_PROP_* in use (TILED: the fused kernel's time is prop_ms) */

@@ -50,6 +50,7 @@ struct mgb_sim {
     int stdp = 0;
     int use_graph = 0;
     int prop_grid = 0, prop_block = 256;
+    int32_t split = 1;                 // waves per spike row in k_prop / k_stdp
     int64_t rec_cap = 0;
     int64_t launches = 0;
     float scale_e = 1.f, scale_i = 1.f;
@@ -97,7 +98,9 @@ struct mgb_sim {
     uint64_t* seg_obase = nullptr;     // [nseg+1]
     uint64_t n_offs = 0, n_keys = 0;
     uint32_t* tcnt = nullptr;          // [3][nseg]
+    int tkind = kTClass;               // synapse word kind of the fused kernel
     uint32_t* syn_t = nullptr;
+    float* w_t = nullptr;              // kTF32 weights in tile order
     uint16_t* offs16 = nullptr;
     uint32_t* offs32 = nullptr;
     uint32_t* ring_t = nullptr;
@@ -116,6 +119,7 @@ struct mgb_sim {
     struct Span {
         int kind;
         hipEvent_t a, b;
+        int32_t count;      // launches of the span's kind it covers (graph replays: windows)
     };
     std::vector<Span> spans;
     // bookkeeping
@@ -318,8 +322,11 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D) {
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
     lds_max = std::min(lds_max, 160 * 1024);
     // the kernel's static LDS (block scan sums, event counter) shares the limit
-    int32_t bmax = (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0) - kTiledStaticLds) / (4 * (int64_t)D));
-    bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, 1 << 14));
+    const int rw = s->tkind == kTClass ? 1 : 2;       // ring words per (slot, target)
+    int32_t bmax = (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0, s->tkind) - kTiledStaticLds) /
+                             (4 * (int64_t)rw * D));
+    // CLASS words hold the global target id; wide words an 11-bit tile-local one
+    bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, s->tkind == kTClass ? (1 << 14) : 2048));
     int32_t B = (o && o->tile_size > 0) ? o->tile_size : std::max<int32_t>(256, (n + 255) / 256);
     B = std::max<int32_t>(1, std::min(B, bmax));
     const std::vector<int32_t>& lo = s->h_seg_lo;
@@ -328,7 +335,7 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D) {
     for (int32_t q = 0; q < nseg; ++q) seg_max = std::max(seg_max, lo[q + 1] - lo[q]);
     B = std::min(B, std::max(seg_max, 1));
     s->B = B;
-    s->chunk = (o && (o->chunk == 4 || o->chunk == 16)) ? o->chunk : 8;
+    s->chunk = (o && (o->chunk == 1 || o->chunk == 2 || o->chunk == 4 || o->chunk == 16)) ? o->chunk : 8;
     s->tblock = (o && o->prop_block >= 64 && o->prop_block <= 1024) ? (o->prop_block / 64) * 64 : 512;
     std::vector<int32_t> tlo, tile0(nseg + 1, 0);
     std::vector<uint16_t> tseg;
@@ -375,8 +382,38 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D) {
     MGB_HIP_TRY(hipMemcpyAsync(s->seg_kbase, kbase.data(), 4 * kbase.size(), hipMemcpyHostToDevice, st));
     MGB_HIP_TRY(hipMemcpyAsync(s->seg_obase, obase.data(), 8 * obase.size(), hipMemcpyHostToDevice, st));
     MGB_HIP_TRY(hipStreamSynchronize(st));   // host vectors go out of scope
-    const size_t lds = tiled_lds_bytes(D, B);
-    if (lds > 65536) MGB_HIP_TRY(set_tiled_lds_limit(lds));
+    const size_t lds = tiled_lds_bytes(D, B, s->tkind);
+    if (lds > 65536) MGB_HIP_TRY(set_tiled_lds_limit(s->tkind, lds));
+    return MGB_OK;
+}
+
+TileTabs tile_tabs(const mgb_sim* s, const uint16_t* seg_dev) {
+    TileTabs tt{};
+    tt.nseg = s->nseg; tt.B = s->B;
+    tt.seg_of = s->nseg > 1 ? seg_dev : nullptr;
+    tt.seg_lo = s->seg_lo; tt.seg_tile0 = s->seg_tile0;
+    tt.seg_kbase = s->seg_kbase; tt.seg_obase = s->seg_obase;
+    return tt;
+}
+
+// Per-(source, tile) offsets of the tile-sorted copy (sorted_keys: its sort keys).
+mgb_status build_tile_offsets(mgb_sim* s, int32_t n, const uint32_t* sorted_keys, const TileTabs& tt) {
+    hipStream_t st = s->stream;
+    uint32_t max_row = 0;
+    {
+        std::vector<int64_t> rp((size_t)n + 1);
+        MGB_HIP_TRY(hipMemcpyAsync(rp.data(), s->row_ptr, 8 * ((size_t)n + 1), hipMemcpyDeviceToHost, st));
+        MGB_HIP_TRY(hipStreamSynchronize(st));
+        for (int32_t m = 0; m < n; ++m) max_row = std::max<uint32_t>(max_row, (uint32_t)(rp[m + 1] - rp[m]));
+    }
+    if (max_row >= (1u << 26)) {
+        set_error("tiled propagation: a CSR row of %u synapses exceeds 2^26", max_row);
+        return MGB_ERR_UNSUPPORTED;
+    }
+    const size_t no = (size_t)s->n_offs;
+    if (max_row <= 0xFFFFu) MGB_TRY(dalloc_n(s, &s->offs16, no));
+    else MGB_TRY(dalloc_n(s, &s->offs32, no));
+    MGB_HIP_TRY(tile_offsets(sorted_keys, s->row_ptr, tt, s->n_offs, s->offs16, s->offs32, st));
     return MGB_OK;
 }
 
@@ -479,6 +516,8 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
     const int32_t max_d = S ? hmm[1] : 0;
     s->min_delay = S ? hmm[0] : 0;
     s->D = max_d + 1;                                   // mgsim.cpp:71
+    // spike rows of ~256 synapses per wave in the direct kernels
+    s->split = (int32_t)std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)(S / (uint64_t)n) / 256));
     s->n_plastic = (int64_t)hnpl;
     uint32_t max_deg_cls = 0, max_deg_tot = 0;
     for (int32_t t = 0; t < n; ++t) {
@@ -583,15 +622,12 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         MGB_HIP_TRY(hipMemcpyAsync(s->seq, seq.data(), 4 * seq.size(), hipMemcpyHostToDevice, st));
         const int prop = o ? o->propagation : MGB_PROP_AUTO;
         s->tiled = (prop == MGB_PROP_AUTO || prop == MGB_PROP_TILED) ? 1 : 0;
+        s->tkind = kTClass;
         if (s->tiled) {
             MGB_TRY(choose_tiles(s, o, n, s->D));
             // tile-sorted copy: stable sort by (source, target tile) keeps list order
             // inside every (source, tile) sub-row
-            TileTabs tt{};
-            tt.nseg = s->nseg; tt.B = s->B;
-            tt.seg_of = s->nseg > 1 ? seg_dev : nullptr;
-            tt.seg_lo = s->seg_lo; tt.seg_tile0 = s->seg_tile0;
-            tt.seg_kbase = s->seg_kbase; tt.seg_obase = s->seg_obase;
+            const TileTabs tt = tile_tabs(s, seg_dev);
             MGB_HIP_TRY(make_tile_keys(L.src, L.tar, L.weight, dq, tt, kin, vin, S, st));
             const int tbits = bits_for(s->n_keys);
             if (S > 0) {
@@ -606,23 +642,7 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
             }
             MGB_TRY(dalloc_n(s, &s->syn_t, S));
             if (S) MGB_HIP_TRY(hipMemcpyAsync(s->syn_t, vout, S * 4, hipMemcpyDeviceToDevice, st));
-            uint32_t max_row = 0;
-            {
-                std::vector<int64_t> rp((size_t)n + 1);
-                MGB_HIP_TRY(hipMemcpyAsync(rp.data(), s->row_ptr, 8 * ((size_t)n + 1),
-                                           hipMemcpyDeviceToHost, st));
-                MGB_HIP_TRY(hipStreamSynchronize(st));
-                for (int32_t m = 0; m < n; ++m)
-                    max_row = std::max<uint32_t>(max_row, (uint32_t)(rp[m + 1] - rp[m]));
-            }
-            if (max_row >= (1u << 26)) {
-                set_error("tiled propagation: a CSR row of %u synapses exceeds 2^26", max_row);
-                return MGB_ERR_UNSUPPORTED;
-            }
-            const size_t no = (size_t)s->n_offs;
-            if (max_row <= 0xFFFFu) MGB_TRY(dalloc_n(s, &s->offs16, no));
-            else MGB_TRY(dalloc_n(s, &s->offs32, no));
-            MGB_HIP_TRY(tile_offsets(kout, s->row_ptr, tt, s->n_offs, s->offs16, s->offs32, st));
+            MGB_TRY(build_tile_offsets(s, n, kout, tt));
         }
         MGB_HIP_TRY(hipStreamSynchronize(st));
     } else {
@@ -661,6 +681,38 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         g.col = s->col; g.dly = s->dly; g.w32 = s->w32; g.w16 = s->w16; g.w8 = s->w8;
         g.row_scale = s->row_scale; g.pos_of = pos_of;
         MGB_HIP_TRY(gather_wide(g, st));
+        // fused path for static networks whose delays fit the 5-bit field
+        const int prop = o ? o->propagation : MGB_PROP_AUTO;
+        const bool tiled_ok = !h.stdp && max_d <= kClassMaxDelay;
+        if (prop == MGB_PROP_TILED && !tiled_ok) {
+            set_error("tiled propagation of the wide formats needs a network without STDP and delays <= %d "
+                      "steps", kClassMaxDelay);
+            return MGB_ERR_UNSUPPORTED;
+        }
+        s->tiled = (tiled_ok && prop != MGB_PROP_DIRECT) ? 1 : 0;
+        if (s->tiled) {
+            s->tkind = s->wide_kind == kWideBF16 ? kTBF16 : (s->wide_kind == kWideFP8 ? kTFP8 : kTF32);
+            MGB_TRY(choose_tiles(s, o, n, s->D));
+            const TileTabs tt = tile_tabs(s, seg_dev);
+            // kout holds each CSR slot's source (the keys of the sort by source)
+            MGB_HIP_TRY(make_tile_keys_csr(kout, s->col, tt, kin, vin, S, st));
+            const int tbits = bits_for(s->n_keys);
+            if (S > 0) {
+                size_t tb2 = 0;
+                MGB_HIP_TRY(sort_pairs(nullptr, &tb2, kin, kout, vin, vout, S, tbits, st));
+                if (tb2 > tb) {
+                    X.release(temp);
+                    MGB_TRY(X.get(reinterpret_cast<uint8_t**>(&temp), tb2));
+                    tb = tb2;
+                }
+                MGB_HIP_TRY(sort_pairs(temp, &tb2, kin, kout, vin, vout, S, tbits, st));
+            }
+            MGB_TRY(dalloc_n(s, &s->syn_t, S));
+            if (s->tkind == kTF32) MGB_TRY(dalloc_n(s, &s->w_t, S));
+            MGB_HIP_TRY(tile_gather_wide(vout, S, s->col, s->dly, s->w16, s->w8, s->w32, tt, s->tkind, s->syn_t,
+                                         s->w_t, st));
+            MGB_TRY(build_tile_offsets(s, n, kout, tt));
+        }
         if (h.stdp) {
             // incoming CSR of plastic synapses in list order (dend_offs/dend_refs)
             const int ibits = bits_for((uint64_t)n);
@@ -735,7 +787,7 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
                                hipMemcpyHostToDevice, st));
     // ring
     if (s->tiled) {
-        const size_t rt = (size_t)s->G * (size_t)s->D * (size_t)s->B;
+        const size_t rt = (size_t)s->G * (size_t)s->D * (size_t)s->B * (s->tkind == kTClass ? 1u : 2u);
         MGB_TRY(dalloc_n(s, &s->ring_t, rt));
         MGB_HIP_TRY(hipMemsetAsync(s->ring_t, 0, 4 * rt, st));
     } else if (s->mode == kRingClass) {
@@ -796,7 +848,7 @@ NeuArgs neu_args(const mgb_sim* s) {
 
 PropArgs prop_args(const mgb_sim* s) {
     PropArgs a{};
-    a.n = s->n; a.npad = s->npad; a.D = s->D;
+    a.n = s->n; a.npad = s->npad; a.D = s->D; a.split = s->split;
     a.row_ptr = s->row_ptr; a.syn = s->syn; a.col = s->col; a.dly = s->dly;
     a.w32 = s->w32; a.w16 = s->w16; a.w8 = s->w8; a.row_scale = s->row_scale;
     a.ring = s->ring; a.ring_e = s->ring_e; a.ring_i = s->ring_i;
@@ -808,7 +860,8 @@ PropArgs prop_args(const mgb_sim* s) {
 
 StdpArgs stdp_args(const mgb_sim* s) {
     StdpArgs a{};
-    a.n = s->n; a.row_ptr = s->row_ptr; a.col = s->col; a.dly = s->dly; a.w32 = s->w32;
+    a.n = s->n; a.split = s->split;
+    a.row_ptr = s->row_ptr; a.col = s->col; a.dly = s->dly; a.w32 = s->w32;
     a.in_ptr = s->in_ptr; a.in_pos = s->in_pos; a.in_src = s->in_src; a.last = s->last;
     a.ltp = s->ltp; a.ltd = s->ltd; a.lut_n = s->lut_n;
     a.w_max = s->sc[4]; a.w_min = s->sc[5];
@@ -819,6 +872,10 @@ StdpArgs stdp_args(const mgb_sim* s) {
 TiledArgs tiled_args(const mgb_sim* s) {
     TiledArgs a{};
     a.n = s->n; a.D = s->D; a.B = s->B; a.G = s->G; a.chunk = s->chunk;
+    a.kind = s->tkind;
+    a.scale_e = s->scale_e; a.scale_i = s->scale_i;
+    a.inv_e = 1.0f / s->scale_e; a.inv_i = 1.0f / s->scale_i;
+    a.w_t = s->w_t; a.row_scale = s->row_scale;
     a.nseg = s->nseg; a.window = s->window;
     a.tile_lo = s->tile_lo; a.tile_seg = s->tile_seg; a.seg_lo = s->seg_lo;
     a.seg_tile0 = s->seg_tile0; a.seg_obase = s->seg_obase; a.tcnt = s->tcnt;
@@ -835,14 +892,14 @@ TiledArgs tiled_args(const mgb_sim* s) {
 
 int prop_grid(const mgb_sim* s, int32_t nst) {
     if (s->prop_grid > 0) return s->prop_grid;
-    // waves >= expected spikes; capped so an idle window costs little
+    // waves >= expected spikes x split; capped so an idle window costs little
     const int64_t waves_per_block = s->prop_block / 64;
-    int64_t spikes_max = (int64_t)s->n * (int64_t)nst;
+    int64_t spikes_max = (int64_t)s->n * (int64_t)nst * (int64_t)s->split;
     int64_t g = (spikes_max + waves_per_block - 1) / waves_per_block;
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
 }
 
-enum { kSpanNeuron = 0, kSpanProp = 1, kSpanStdp = 2 };
+enum { kSpanNeuron = 0, kSpanProp = 1, kSpanStdp = 2, kSpanGraph = 3 };
 
 mgb_status next_event(mgb_sim* s, hipEvent_t* e) {
     if (s->ev_used == s->ev_pool.size()) {
@@ -859,6 +916,7 @@ struct SpanGuard {
     mgb_sim* s;
     int kind;
     bool on;
+    int32_t count = 1;
     hipEvent_t a = nullptr;
     mgb_status begin() {
         if (!on) return MGB_OK;
@@ -871,7 +929,7 @@ struct SpanGuard {
         hipEvent_t b = nullptr;
         MGB_TRY(next_event(s, &b));
         MGB_HIP_TRY(hipEventRecord(b, s->stream));
-        s->spans.push_back({kind, a, b});
+        s->spans.push_back({kind, a, b, count});
         return MGB_OK;
     }
 };
@@ -1355,7 +1413,11 @@ mgb_status mgb_run(mgb_sim* s, int32_t nsteps) {
         if (nsteps >= blk) {
             MGB_TRY(ensure_graph(s, blk));
             while (nsteps - done >= blk) {
+                // a fused-path replay is its windows' fused launches (+ one finalize)
+                SpanGuard gg{s, kSpanGraph, s->profile != 0 && s->tiled != 0, blk / s->window};
+                MGB_TRY(gg.begin());
                 MGB_HIP_TRY(hipGraphLaunch(s->gexec, s->stream));
+                MGB_TRY(gg.end());
                 s->launches += 1;
                 done += blk;
             }
@@ -1379,7 +1441,7 @@ mgb_status mgb_sync(mgb_sim* s) {
 
 mgb_status mgb_set_launch(mgb_sim* s, int32_t block, int32_t chunk) {
     if (!s || block < 0 || (block > 0 && (block < 64 || block > 1024 || block % 64 != 0)) ||
-        !(chunk == 0 || chunk == 4 || chunk == 8 || chunk == 16)) {
+        !(chunk == 0 || chunk == 1 || chunk == 2 || chunk == 4 || chunk == 8 || chunk == 16)) {
         set_error("invalid launch shape (block %d, chunk %d)", block, chunk);
         return MGB_ERR_INVALID;
     }
@@ -1441,9 +1503,9 @@ mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
         if (sp.kind == kSpanNeuron) {
             out->neuron_ms += ms;
             out->neuron_launches += 1;
-        } else if (sp.kind == kSpanProp) {
+        } else if (sp.kind == kSpanProp || sp.kind == kSpanGraph) {
             out->prop_ms += ms;
-            out->prop_launches += 1;
+            out->prop_launches += sp.count;
         } else {
             out->stdp_ms += ms;
             out->stdp_launches += 1;

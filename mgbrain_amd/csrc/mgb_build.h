@@ -81,6 +81,11 @@ hipError_t make_sort_input(const int32_t* src, const int32_t* tar, const float* 
                            hipStream_t st);
 hipError_t make_tile_keys(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
                           const TileTabs& tt, uint32_t* keys, uint32_t* vals, uint64_t S, hipStream_t st);
+hipError_t make_tile_keys_csr(const uint32_t* src, const int32_t* col, const TileTabs& tt, uint32_t* keys,
+                              uint32_t* vals, uint64_t S, hipStream_t st);
+hipError_t tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col, const uint8_t* dly,
+                            const uint16_t* w16, const uint8_t* w8, const float* w32, const TileTabs& tt, int kind,
+                            uint32_t* syn_t, float* w_t, hipStream_t st);
 hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, const TileTabs& tt,
                         uint64_t total, uint16_t* offs16, uint32_t* offs32, hipStream_t st);
 hipError_t fp8_row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n,

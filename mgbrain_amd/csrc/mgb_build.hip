@@ -115,6 +115,42 @@ __global__ void k_tile_keys(const int32_t* src, const int32_t* tar, const float*
     }
 }
 
+// Tile keys of the CSR slots of a wide-format network: slot k (source src[k],
+// target col[k]); the value is the slot itself.
+__global__ void k_tile_keys_csr(const uint32_t* src, const int32_t* col, TileTabs tt, uint32_t* keys,
+                                uint32_t* vals, uint64_t S) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const int32_t s = (int32_t)src[k], t = col[k];
+        const int32_t q = tt.seg_of ? (int32_t)tt.seg_of[s] : 0;
+        const int32_t lo = tt.seg_lo[q];
+        const uint32_t gq = (uint32_t)(tt.seg_tile0[q + 1] - tt.seg_tile0[q]);
+        keys[k] = tt.seg_kbase[q] + (uint32_t)(s - lo) * gq + (uint32_t)((t - lo) / tt.B);
+        vals[k] = (uint32_t)k;
+    }
+}
+
+// Fused-path words of the wide formats in tile order (slot[k]: the CSR slot at
+// tile-order position k): local target | delay-1 << 11 | BF16 / FP8 bits << 16;
+// FP32 weights go to w_t.
+__global__ void k_tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col, const uint8_t* dly,
+                                   const uint16_t* w16, const uint8_t* w8, const float* w32, TileTabs tt,
+                                   int kind, uint32_t* syn_t, float* w_t) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = slot[k];
+        const int32_t t = col[p];
+        const int32_t q = tt.seg_of ? (int32_t)tt.seg_of[t] : 0;
+        const uint32_t loc = (uint32_t)((t - tt.seg_lo[q]) % tt.B);
+        const uint32_t d = (uint32_t)(dly[p] & 127u);
+        uint32_t payload = 0u;
+        if (kind == kTBF16) payload = w16[p];
+        else if (kind == kTFP8) payload = w8[p];
+        syn_t[k] = loc | ((d - 1u) << 11) | (payload << 16);
+        if (kind == kTF32) w_t[k] = w32[p];
+    }
+}
+
 // Offset entry e = seg_obase[q] + (m - seg_lo[q]) * (Gq + 1) + t, t in [0, Gq]:
 // first slot of (source m, local tile t) relative to row m (t = Gq: the row length).
 template <class T>
@@ -320,6 +356,22 @@ hipError_t make_tile_keys(const int32_t* src, const int32_t* tar, const float* w
     if (S == 0) return hipSuccess;
     hipLaunchKernelGGL(k_tile_keys, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, tar, w, dq, tt, keys,
                        vals, S);
+    return hipGetLastError();
+}
+
+hipError_t make_tile_keys_csr(const uint32_t* src, const int32_t* col, const TileTabs& tt, uint32_t* keys,
+                              uint32_t* vals, uint64_t S, hipStream_t st) {
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_keys_csr, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, col, tt, keys, vals, S);
+    return hipGetLastError();
+}
+
+hipError_t tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col, const uint8_t* dly,
+                            const uint16_t* w16, const uint8_t* w8, const float* w32, const TileTabs& tt, int kind,
+                            uint32_t* syn_t, float* w_t, hipStream_t st) {
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_gather_wide, dim3(blocks_for(S, 256)), dim3(256), 0, st, slot, S, col, dly, w16, w8,
+                       w32, tt, kind, syn_t, w_t);
     return hipGetLastError();
 }
 

@@ -92,6 +92,7 @@ struct NeuArgs {
 
 struct PropArgs {
     int32_t n, npad, D;
+    int32_t split;             // waves sharing one spike's row
     const int64_t* row_ptr;
     const uint32_t* syn;       // CLASS format
     const int32_t* col;        // wide formats
@@ -113,6 +114,7 @@ struct PropArgs {
 
 struct StdpArgs {
     int32_t n;
+    int32_t split;             // waves sharing one spike's out- / in-row
     const int64_t* row_ptr;
     const int32_t* col;
     const uint8_t* dly;
@@ -140,9 +142,16 @@ struct StdpArgs {
 // one segment, each segment has its own spike lists, and a source row of
 // segment q holds offsets for q's tiles only: offs[seg_obase[q] +
 // (src - seg_lo[q]) * (Gq + 1) + local tile], Gq = tiles of q.
+// Synapse word kinds of the fused kernel.
+enum TiledKind { kTClass = 0, kTBF16 = 1, kTFP8 = 2, kTF32 = 3 };
+
 struct TiledArgs {
     int32_t n, D, B, G;
     int32_t chunk;          // synapses per work item in the propagation phase
+    int32_t kind;           // TiledKind
+    float scale_e, scale_i, inv_e, inv_i;   // fixed-point scales of the wide kinds
+    const float* w_t;       // kTF32: weights in tile order
+    const float* row_scale; // kTFP8: per-source scale
     int32_t nseg;
     int32_t window;         // spike-list entries reserved per neuron (steps per window)
     const int32_t* tile_lo;     // [G+1]
@@ -193,10 +202,10 @@ hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t
 constexpr int kTiledBatch = 1024;   // spikes staged in LDS per propagation batch
 constexpr int kTiledItems = 8192;   // work items staged in LDS per propagation round
 constexpr int kTiledStaticLds = 256; // bytes reserved for the fused kernel's static LDS
-size_t tiled_lds_bytes(int D, int B);
+size_t tiled_lds_bytes(int D, int B, int kind);
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st);
 hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
-hipError_t set_tiled_lds_limit(size_t bytes);
+hipError_t set_tiled_lds_limit(int kind, size_t bytes);
 
 }  // namespace mgb

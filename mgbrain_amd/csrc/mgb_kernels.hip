@@ -261,14 +261,19 @@ __global__ void k_prop(PropArgs a, int32_t off, int32_t wk) {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const int32_t D = a.D;
+    // each spike's row is shared by `split` waves: part f takes lanes' chunks
+    // beg + f*64 + lane, stepping split*64
+    const uint32_t F = (uint32_t)a.split;
+    const int64_t stride = 64 * (int64_t)F;
     unsigned long long se_local = 0ull;
-    for (uint32_t e = wave; e < nspk; e += nwaves) {
+    for (uint32_t vw = wave; vw < nspk * F; vw += nwaves) {
+        const uint32_t e = vw / F, f = vw - e * F;
         const uint32_t ent = wbuf[e];
         const int32_t i = (int32_t)(ent & kIdMask);
         const int32_t s = (int32_t)(s0 + (ent >> 26));
-        const int64_t beg = a.row_ptr[i], end = a.row_ptr[i + 1];
-        if (lane == 0) {
-            se_local += (unsigned long long)(end - beg);
+        const int64_t beg = a.row_ptr[i] + 64 * (int64_t)f, end = a.row_ptr[i + 1];
+        if (lane == 0 && f == 0) {
+            se_local += (unsigned long long)(end - beg);   // f == 0: the whole row
             if (a.rec) {
                 const uint64_t r = rec_base + e;
                 if (r < a.rec_cap) a.rec[r] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)i;
@@ -278,10 +283,10 @@ __global__ void k_prop(PropArgs a, int32_t off, int32_t wk) {
         const int32_t sD = s % D;
         if (MODE == kRingClass) {
             int64_t k = beg + lane;
-            for (; k + 192 < end; k += 256) {
+            for (; k + 3 * stride < end; k += 4 * stride) {
                 uint32_t x[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) x[u] = __builtin_nontemporal_load(&a.syn[k + 64 * u]);
+                for (int u = 0; u < 4; ++u) x[u] = __builtin_nontemporal_load(&a.syn[k + stride * u]);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     int32_t slot = sD + (int32_t)((x[u] >> 26) & 31u) + 1;
@@ -290,7 +295,7 @@ __global__ void k_prop(PropArgs a, int32_t off, int32_t wk) {
                               (x[u] >> 31) ? 0x10000u : 1u);
                 }
             }
-            for (; k < end; k += 64) {
+            for (; k < end; k += stride) {
                 const uint32_t x = __builtin_nontemporal_load(&a.syn[k]);
                 int32_t slot = sD + (int32_t)((x >> 26) & 31u) + 1;
                 slot -= (slot >= D) ? D : 0;
@@ -299,7 +304,7 @@ __global__ void k_prop(PropArgs a, int32_t off, int32_t wk) {
             }
         } else {
             const float rs = (WK == kWideFP8) ? a.row_scale[i] : 1.0f;
-            for (int64_t k = beg + lane; k < end; k += 64) {
+            for (int64_t k = beg + lane; k < end; k += stride) {
                 const int32_t tar = a.col[k];
                 const uint32_t dl = a.dly[k];
                 int32_t slot = sD + (int32_t)(dl & 127u);
@@ -338,17 +343,20 @@ __global__ void k_stdp(StdpArgs a, int32_t off, int32_t wk) {
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t e = wave; e < nspk; e += nwaves) {
+    const uint32_t F = (uint32_t)a.split;           // waves per spike, as in k_prop
+    const int64_t stride = 64 * (int64_t)F;
+    for (uint32_t vw = wave; vw < nspk * F; vw += nwaves) {
+        const uint32_t e = vw / F, f = vw - e * F;
         const int32_t i = (int32_t)(wbuf[e] & kIdMask);
         const int32_t li = a.last[i];
-        const int64_t beg = a.row_ptr[i], end = a.row_ptr[i + 1];
-        for (int64_t k = beg + lane; k < end; k += 64) {             // axon loop :372-391
+        const int64_t beg = a.row_ptr[i] + 64 * (int64_t)f, end = a.row_ptr[i + 1];
+        for (int64_t k = beg + lane; k < end; k += stride) {         // axon loop :372-391
             if (!(a.dly[k] & 128u)) continue;
             const int32_t dt = li - a.last[a.col[k]];
             a.w32[k] = clipw(a.w32[k] + stdp_lut(a, dt), a.w_max, a.w_min);
         }
-        const int64_t pb = a.in_ptr[i], pe = a.in_ptr[i + 1];
-        for (int64_t p = pb + lane; p < pe; p += 64) {               // dendrite loop :392-411
+        const int64_t pb = a.in_ptr[i] + 64 * (int64_t)f, pe = a.in_ptr[i + 1];
+        for (int64_t p = pb + lane; p < pe; p += stride) {           // dendrite loop :392-411
             const uint32_t k = a.in_pos[p];
             const int32_t dt = a.last[a.in_src[p]] - li;
             a.w32[k] = clipw(a.w32[k] + stdp_lut(a, dt), a.w_max, a.w_min);
@@ -403,18 +411,27 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* wsum) 
 //   4. writes the ring slice back.
 // Spike lists rotate over three buffers: window k writes k%3, reads (k-1)%3
 // and clears the counter of (k+1)%3, which no workgroup touches in window k.
-template <int CH>
+//
+// WK selects the synapse word.  kTClass: target | delay-1 << 26 | inhibitory
+// << 31, arrivals counted per class (16 + 16 bits of one ring word).  Wide
+// formats: local target (11 bits) | delay-1 << 11 | BF16 (kTBF16) or FP8
+// (kTFP8, per-source scale) weight bits << 16, or the weight in w_t (kTF32);
+// arrivals add rintf(w * scale) into int32 exc / inh rings, as k_prop does.
+template <int CH, int WK>
 __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_t wk, int32_t nsteps) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t wsum[16];
-    __shared__ unsigned long long se_blk;
+    __shared__ unsigned long long se_blk, lsp_r;
+    __shared__ uint32_t lsp_n, lsp_g;
     const int32_t D = a.D, B = a.B, g = (int32_t)blockIdx.x, bd = (int32_t)blockDim.x;
     const int32_t tid = (int32_t)threadIdx.x;
-    uint32_t* ring = lds;                                   // [D][B]
-    uint32_t* sp_beg = ring + (size_t)D * B;                // [kTiledBatch] first CSR slot
+    constexpr int RW = (WK == kTClass) ? 1 : 2;             // ring words per (slot, target)
+    uint32_t* ring = lds;                                   // [RW][D][B]
+    uint32_t* sp_beg = ring + (size_t)RW * D * B;           // [kTiledBatch] first CSR slot
     uint32_t* sp_len = sp_beg + kTiledBatch;                // len | slot-of-spike << 26
     uint32_t* sp_start = sp_len + kTiledBatch;              // exclusive prefix of work items
     uint16_t* itm = reinterpret_cast<uint16_t*>(sp_start + kTiledBatch);   // [kTiledItems] spike of item
+    float* sp_rs = reinterpret_cast<float*>(itm + kTiledItems);            // [kTiledBatch] FP8 row scale
     Ctl* ctl = a.ctl;
     const int64_t s0 = ctl->base_step + off;
     const int64_t kwin = ctl->base_window + wk;
@@ -431,15 +448,37 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const int32_t nseg = a.nseg;
     if (lt == 0 && tid == 0) tcnt[bnext * nseg] = 0u;
     if (g == 0 && tid == 0) ctl->tbase[bcur] = s0;
-    if (tid == 0) se_blk = 0ull;
-    // 1. ring slice -> LDS
-    uint32_t* gring = a.ring_t + (size_t)g * (size_t)D * (size_t)B;
-    for (int32_t k = tid; k < D * B; k += bd) ring[k] = gring[k];
+    if (tid == 0) {
+        se_blk = 0ull;
+        lsp_n = 0u;
+    }
+    // 1. ring slice -> LDS.  The slots the previous window consumed ([sprev, s0)
+    // mod D) were cleared by it and not stored back (step 4): they are zero here.
+    const int64_t sprev = ctl->tbase[bprev];
+    const int32_t nprev = (int32_t)(s0 - sprev);           // 0 before the first window
+    const int32_t rprev = (int32_t)(sprev % D);
+    uint32_t* gring = a.ring_t + (size_t)g * (size_t)RW * (size_t)D * (size_t)B;
+#pragma unroll
+    for (int hh = 0; hh < RW; ++hh) {
+        // rows [rprev, rprev + nprev) mod D are zero, the rest are loaded
+        uint32_t* rr = ring + (size_t)hh * D * B;
+        const uint32_t* gr = gring + (size_t)hh * D * B;
+        const int32_t e = rprev + nprev;
+        const int32_t z0 = (e <= D) ? rprev : 0, z1 = (e <= D) ? e : e - D;   // zero rows [z0, z1)
+        if (e <= D) {
+            for (int32_t k = tid; k < z0 * B; k += bd) rr[k] = gr[k];
+            for (int32_t k = z0 * B + tid; k < z1 * B; k += bd) rr[k] = 0u;
+            for (int32_t k = z1 * B + tid; k < D * B; k += bd) rr[k] = gr[k];
+        } else {
+            for (int32_t k = tid; k < z1 * B; k += bd) rr[k] = 0u;
+            for (int32_t k = z1 * B + tid; k < rprev * B; k += bd) rr[k] = gr[k];
+            for (int32_t k = rprev * B + tid; k < D * B; k += bd) rr[k] = 0u;
+        }
+    }
     __syncthreads();
 
     // 2. previous window's spikes into the tile
     const uint32_t E = tcnt[bprev * nseg];
-    const int64_t sprev = ctl->tbase[bprev];
     const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride + lbase;
     for (uint32_t b0 = 0; b0 < E; b0 += (uint32_t)bd) {
         const uint32_t nb = min((uint32_t)bd, E - b0);
@@ -455,6 +494,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             const int32_t sD = (int32_t)((sprev + (int64_t)(ent >> 26)) % D);
             sp_beg[tid] = (uint32_t)a.row_ptr[src] + o0;
             sp_len[tid] = len | ((uint32_t)sD << 26);
+            if (WK == kTFP8) sp_rs[tid] = a.row_scale[src];
             items = (len + CH - 1) / CH;
         }
         const uint32_t incl = block_incl_scan(items, wsum);
@@ -468,13 +508,20 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                 for (uint32_t q = q0; q < q1; ++q) itm[q - r0] = (uint16_t)tid;
             }
             __syncthreads();
-            for (uint32_t it = (uint32_t)tid; it < rn; it += 2u * (uint32_t)bd) {
-                const uint32_t it2 = it + (uint32_t)bd;
-                uint32_t cnt[2] = {0u, 0u}, kb[2] = {0u, 0u};
-                int32_t sD[2] = {0, 0};
+            // U items per thread in flight: U * CH synapse words loaded before the
+            // first LDS atomic
+            constexpr int U = CH >= 16 ? 1 : 16 / CH;
+            for (uint32_t it = (uint32_t)tid; it < rn; it += (uint32_t)(U * bd)) {
+                uint32_t cnt[U], kb[U];
+                int32_t sD[U];
+                float rs[U];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const uint32_t q = h ? it2 : it;
+                for (int h = 0; h < U; ++h) {
+                    const uint32_t q = it + (uint32_t)(h * bd);
+                    cnt[h] = 0u;
+                    kb[h] = 0u;
+                    sD[h] = 0;
+                    rs[h] = 1.0f;
                     if (q < rn) {
                         const uint32_t j = itm[q];
                         const uint32_t c = r0 + q - sp_start[j];
@@ -483,23 +530,42 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                         sD[h] = (int32_t)(meta >> 26);
                         kb[h] = sp_beg[j] + c * CH;
                         cnt[h] = min((uint32_t)CH, len - c * CH);
+                        if (WK == kTFP8) rs[h] = sp_rs[j];
                     }
                 }
-                uint32_t x[2][CH];
+                uint32_t x[U][CH];
+                float wv[U][CH];
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
+                for (int h = 0; h < U; ++h)
 #pragma unroll
                     for (int u = 0; u < CH; ++u)
-                        if ((uint32_t)u < cnt[h]) x[h][u] = __builtin_nontemporal_load(&a.syn_t[kb[h] + u]);
+                        if ((uint32_t)u < cnt[h]) {
+                            x[h][u] = __builtin_nontemporal_load(&a.syn_t[kb[h] + u]);
+                            if (WK == kTF32) wv[h][u] = __builtin_nontemporal_load(&a.w_t[kb[h] + u]);
+                        }
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
+                for (int h = 0; h < U; ++h)
 #pragma unroll
                     for (int u = 0; u < CH; ++u) {
                         if ((uint32_t)u < cnt[h]) {
-                            int32_t slot = sD[h] + (int32_t)((x[h][u] >> 26) & 31u) + 1;
-                            slot -= (slot >= D) ? D : 0;
-                            const int32_t loc = (int32_t)(x[h][u] & kIdMask) - t0;
-                            atomicAdd(&ring[slot * B + loc], (x[h][u] >> 31) ? 0x10000u : 1u);
+                            const uint32_t xw = x[h][u];
+                            if (WK == kTClass) {
+                                int32_t slot = sD[h] + (int32_t)((xw >> 26) & 31u) + 1;
+                                slot -= (slot >= D) ? D : 0;
+                                const int32_t loc = (int32_t)(xw & kIdMask) - t0;
+                                atomicAdd(&ring[slot * B + loc], (xw >> 31) ? 0x10000u : 1u);
+                            } else {
+                                int32_t slot = sD[h] + (int32_t)((xw >> 11) & 31u) + 1;
+                                slot -= (slot >= D) ? D : 0;
+                                const int32_t loc = (int32_t)(xw & 0x7FFu);
+                                float w;
+                                if (WK == kTBF16) w = bf16_to_f32((uint16_t)(xw >> 16));
+                                else if (WK == kTFP8) w = fp8e4m3_to_f32((uint8_t)(xw >> 16)) * rs[h];
+                                else w = wv[h][u];
+                                int32_t* rq = reinterpret_cast<int32_t*>(ring);
+                                if (w > 0.f) atomicAdd(&rq[slot * B + loc], (int32_t)rintf(w * a.scale_e));
+                                else atomicAdd(&rq[(D + slot) * B + loc], (int32_t)rintf(w * a.scale_i));
+                            }
                         }
                     }
             }
@@ -508,7 +574,11 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         __syncthreads();   // wsum / sp_* are rewritten by the next batch
     }
 
-    // 3. neurons of the tile through this window's steps
+    // 3. neurons of the tile through this window's steps.  Spikes go to an LDS
+    // list (the staging area of step 2, free now; one LDS atomic per wave and
+    // step) that joins the segment's window list with one global atomic at the
+    // end; lanes that find it full append to HBM directly.
+    uint32_t* lsp = sp_beg;
     uint32_t* wbuf = a.win_buf + (size_t)bcur * a.win_stride + lbase;
     const int32_t slot0 = (int32_t)(s0 % D);
     unsigned long long se_local = 0ull;
@@ -518,7 +588,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         const int32_t i = t0 + j;
         const int type = active ? (int)a.type[i] : T_RAW;
         NState st;
-        if (active) neuron_load(a, i, type, true, st);
+        if (active) neuron_load(a, i, type, WK == kTClass, st);
         const bool integ = integrates(type);
         int32_t outdeg = 0;
         if (active && type != T_RAW && type != T_LIF0)
@@ -530,34 +600,54 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             if (type == T_POISSON) {
                 fired = poisson_step(st, s, w);
             } else if (integ) {
-                uint32_t* r = &ring[slot * B + j];
-                const uint32_t x = *r;
-                *r = 0u;                                              // clearInput
                 float be, bi;
-                class_input(x, a.seq, st, be, bi);
+                uint32_t* r = &ring[slot * B + j];
+                if (WK == kTClass) {
+                    const uint32_t x = *r;
+                    *r = 0u;                                          // clearInput
+                    class_input(x, a.seq, st, be, bi);
+                } else {
+                    uint32_t* ri = r + (size_t)D * B;
+                    const int32_t qe = (int32_t)*r, qi = (int32_t)*ri;
+                    *r = 0u;
+                    *ri = 0u;
+                    be = (float)qe * a.inv_e;
+                    bi = (float)qi * a.inv_i;
+                }
                 fired = integrate_step(type, st, s, be, bi);
             }
             const uint64_t m = __ballot(fired);
             if (m != 0ull) {
                 const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
                 const uint32_t pc = (uint32_t)__popcll(m);
-                uint32_t base = 0u;
-                unsigned long long rb = 0ull;
-                if (lane_id() == leader) {
-                    base = atomicAdd(&tcnt[bcur * nseg], pc);
-                    rb = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)pc);
-                }
-                base = __shfl(base, (int)leader);
-                rb = __shfl(rb, (int)leader);
-                if (fired) {
-                    const uint32_t r = rank_below(m);
-                    wbuf[base + r] = (uint32_t)i | ((uint32_t)w << 26);
-                    se_local += (unsigned long long)outdeg;
-                    if (a.rec) {
-                        const uint64_t at = rb + r;
-                        if (at < a.rec_cap) a.rec[at] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)i;
-                        else ctl->rec_overflow = 1;
+                uint32_t lb = 0u;
+                if (lane_id() == leader) lb = atomicAdd(&lsp_n, pc);
+                lb = __shfl(lb, (int)leader);
+                const uint32_t cap = (uint32_t)kTiledBatch;
+                const uint32_t over0 = max(lb, cap);                 // first index past the LDS list
+                if (lb + pc > cap) {
+                    uint32_t gb = 0u;
+                    unsigned long long rb = 0ull;
+                    if (lane_id() == leader) {
+                        gb = atomicAdd(&tcnt[bcur * nseg], lb + pc - over0);
+                        rb = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)(lb + pc - over0));
                     }
+                    gb = __shfl(gb, (int)leader);
+                    rb = __shfl(rb, (int)leader);
+                    const uint32_t o = lb + rank_below(m) - over0;
+                    if (fired && lb + rank_below(m) >= cap) {
+                        wbuf[gb + o] = (uint32_t)i | ((uint32_t)w << 26);
+                        if (a.rec) {
+                            const uint64_t at = rb + o;
+                            if (at < a.rec_cap) a.rec[at] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)i;
+                            else ctl->rec_overflow = 1;
+                        }
+                    }
+                }
+                if (fired) {
+                    const uint32_t r = lb + rank_below(m);
+                    if (r < cap) lsp[r] = (uint32_t)i | ((uint32_t)w << 26);
+                    se_local += (unsigned long long)outdeg;
                 }
             }
             slot = (slot + 1 == D) ? 0 : slot + 1;
@@ -566,9 +656,40 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     }
     if (se_local) atomicAdd(&se_blk, se_local);
     __syncthreads();
-    if (tid == 0 && se_blk) atomicAdd((unsigned long long*)&ctl->se, se_blk);
-    // 4. LDS -> ring slice
-    for (int32_t k = tid; k < D * B; k += bd) gring[k] = ring[k];
+    const uint32_t nl = min(lsp_n, (uint32_t)kTiledBatch);
+    if (tid == 0) {
+        if (se_blk) atomicAdd((unsigned long long*)&ctl->se, se_blk);
+        if (nl) {
+            lsp_g = atomicAdd(&tcnt[bcur * nseg], nl);
+            lsp_r = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)nl);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = (uint32_t)tid; k < nl; k += (uint32_t)bd) {
+        const uint32_t ent = lsp[k];
+        wbuf[lsp_g + k] = ent;
+        if (a.rec) {
+            const uint64_t at = lsp_r + k;
+            if (at < a.rec_cap)
+                a.rec[at] = ((uint64_t)(uint32_t)(s0 + (int64_t)(ent >> 26)) << 32) | (ent & kIdMask);
+            else ctl->rec_overflow = 1;
+        }
+    }
+    // 4. LDS -> ring slice, except the slots this window consumed (cleared above,
+    // or never read for neurons that do not integrate)
+    const int32_t rcur = (int32_t)(s0 % D);
+    const int32_t e = rcur + nsteps;
+#pragma unroll
+    for (int hh = 0; hh < RW; ++hh) {
+        const uint32_t* rr = ring + (size_t)hh * D * B;
+        uint32_t* gr = gring + (size_t)hh * D * B;
+        if (e <= D) {            // rows [rcur, e) consumed
+            for (int32_t k = tid; k < rcur * B; k += bd) gr[k] = rr[k];
+            for (int32_t k = e * B + tid; k < D * B; k += bd) gr[k] = rr[k];
+        } else {                 // rows [rcur, D) and [0, e - D) consumed
+            for (int32_t k = (e - D) * B + tid; k < rcur * B; k += bd) gr[k] = rr[k];
+        }
+    }
 }
 
 __global__ void k_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows) {
@@ -617,33 +738,70 @@ hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t
     return hipGetLastError();
 }
 
-size_t tiled_lds_bytes(int D, int B) {
-    return sizeof(uint32_t) * ((size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
-           sizeof(uint16_t) * (size_t)kTiledItems;
+size_t tiled_lds_bytes(int D, int B, int kind) {
+    const size_t rw = (kind == kTClass) ? 1 : 2;
+    return sizeof(uint32_t) * (rw * (size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
+           sizeof(uint16_t) * (size_t)kTiledItems + (kind == kTFP8 ? sizeof(float) * kTiledBatch : 0);
 }
+
+namespace {
+template <int WK>
+void launch_tiled_kind(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block, size_t lds,
+                       hipStream_t st) {
+    switch (a.chunk) {
+        case 1:
+            hipLaunchKernelGGL((k_tiled<1, WK>), dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_tiled<2, WK>), dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_tiled<4, WK>), dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
+            break;
+        case 16:
+            hipLaunchKernelGGL((k_tiled<16, WK>), dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
+            break;
+        default:
+            hipLaunchKernelGGL((k_tiled<8, WK>), dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
+            break;
+    }
+}
+
+template <int WK>
+hipError_t set_lds_kind(size_t bytes) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_tiled<1, WK>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_tiled<2, WK>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_tiled<4, WK>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_tiled<8, WK>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_tiled<16, WK>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return e;
+}
+}  // namespace
 
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st) {
-    const size_t lds = tiled_lds_bytes(a.D, a.B);
-    switch (a.chunk) {
-        case 4:
-            hipLaunchKernelGGL(k_tiled<4>, dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
-            break;
-        case 16:
-            hipLaunchKernelGGL(k_tiled<16>, dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
-            break;
-        default:
-            hipLaunchKernelGGL(k_tiled<8>, dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
-            break;
+    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind);
+    switch (a.kind) {
+        case kTBF16: launch_tiled_kind<kTBF16>(a, off, wk, nsteps, block, lds, st); break;
+        case kTFP8: launch_tiled_kind<kTFP8>(a, off, wk, nsteps, block, lds, st); break;
+        case kTF32: launch_tiled_kind<kTF32>(a, off, wk, nsteps, block, lds, st); break;
+        default: launch_tiled_kind<kTClass>(a, off, wk, nsteps, block, lds, st); break;
     }
     return hipGetLastError();
 }
 
-hipError_t set_tiled_lds_limit(size_t bytes) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_tiled<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)k_tiled<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)k_tiled<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    return e;
+hipError_t set_tiled_lds_limit(int kind, size_t bytes) {
+    switch (kind) {
+        case kTBF16: return set_lds_kind<kTBF16>(bytes);
+        case kTFP8: return set_lds_kind<kTFP8>(bytes);
+        case kTF32: return set_lds_kind<kTF32>(bytes);
+        default: return set_lds_kind<kTClass>(bytes);
+    }
 }
 
 hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st) {

@@ -27,7 +27,7 @@ def tile_candidates(n, ring_len, lds_bytes=160 * 1024, batch_words=3 * 1024 + 40
     return out
 
 
-def autotune(make_sim, tiles, blocks=(256, 512, 1024), chunks=(4, 8, 16), steps=400, warmup=200,
+def autotune(make_sim, tiles, blocks=(256, 512, 1024), chunks=(1, 2, 4, 8), steps=400, warmup=200,
              budget_s=90.0, log=None):
     """make_sim(tile_size) -> Simulator.  Returns (best, results) where each
     result is a dict(tile_size, block, chunk, ms_per_step)."""

@@ -30,9 +30,10 @@ def lifb_consts(v_thresh=0.020):
     return c
 
 
-def brunel_fixed_k(scale=1.0, K=10000, rate=0.0015, seed=0):
+def brunel_fixed_k(scale=1.0, K=10000, rate=0.0015, seed=0, plastic_ee=False):
     """80k E + 20k I LIFB + 10k Poisson (x scale), fixed in-degree K from all
-    sources (split by source size), J = 0.1 mV, g = 5, delays on the 1.0-2.0 ms grid."""
+    sources (split by source size), J = 0.1 mV, g = 5, delays on the 1.0-2.0 ms grid.
+    plastic_ee marks the E->E projection plastic (pair-based STDP)."""
     nP, nE, nI = int(10000 * scale), int(80000 * scale), int(20000 * scale)
     ntot = nP + nE + nI
     Ks = [int(round(K * n / ntot)) for n in (nP, nE, nI)]
@@ -43,7 +44,8 @@ def brunel_fixed_k(scale=1.0, K=10000, rate=0.0015, seed=0):
     for tgt in (1, 2):
         for src, k, w in ((0, Ks[0], J), (1, Ks[1], J), (2, Ks[2], wi)):
             projs.append(dict(src=src, tar=tgt, kind=N.CONN_FIXED_INDEGREE, k=k, wmin=w, wmax=w, dmin=dmin,
-                              dmax=dmax, index=len(projs), seed=seed))
+                              dmax=dmax, index=len(projs), seed=seed,
+                              plastic=1 if (plastic_ee and src == 1 and tgt == 1) else 0))
     desc = (f"Brunel fixed in-degree: {nE} E + {nI} I LIFB + {nP} Poisson @ {rate / DT:.0f} Hz, "
             f"K={K}, J=0.1 mV, g=5, delay U{{1.0..2.0}} ms, dt=0.1 ms")
     return pops, projs, desc, ntot
@@ -61,6 +63,15 @@ def brunel_bernoulli(scale=1.0, p=0.1, rate=0.002, seed=0):
     desc = (f"Brunel Bernoulli p={p}: {nE} E + {nI} I LIFB + {nP} Poisson @ {rate / DT:.0f} Hz, "
             f"J=0.1 mV, g=5, delay 1.4 ms, dt=0.1 ms")
     return pops, projs, desc, nP + nE + nI
+
+
+def stdp_consts(J=1e-4):
+    """Pair-based STDP of the E->E synapses: A+ = 0.01, A- = 0.0105 in units of
+    W_max = 2J (the reference's A are absolute weight increments, mgcore.cu:351-361),
+    tau+ = tau- = 20 ms = 200 steps, W in [0, 2J].  Order: {A_LTP, A_LTD, TAU_LTP,
+    TAU_LTD, W_MAX, W_MIN} (constgen.h:140-159)."""
+    wmax = 2 * J
+    return np.array([0.01 * wmax, -0.0105 * wmax, 200.0, 200.0, wmax, 0.0], dtype=f32)
 
 
 # ---- batched parameter / seed sweep ---------------------------------------------

@@ -64,7 +64,8 @@ def test_brunel_reference_generator_class_format(prop):
     assert n > 1000
 
 
-@pytest.mark.parametrize("tile,chunk,block", [(64, 4, 256), (100, 8, 512), (333, 16, 1024), (2000, 8, 64)])
+@pytest.mark.parametrize("tile,chunk,block", [(64, 4, 256), (100, 8, 512), (333, 16, 1024), (2000, 8, 64),
+                                              (150, 1, 512), (77, 2, 1024)])
 def test_tiled_shapes(tile, chunk, block):
     _say(f"case: config1 x0.2 tiled tile={tile} chunk={chunk} block={block}")
     net = M.config1(scale=0.2, poisson_rate=0.2)
@@ -88,11 +89,12 @@ def test_tiled_split_runs():
     assert np.array_equal(g.state(0).view(np.uint32), o.state(po.F_V).astype(np.float32).view(np.uint32))
 
 
-def test_brunel_reference_generator_fixed_format():
-    _say("case: make_brunel(2e5), F32 fixed-point format vs oracle fixed mode")
+@pytest.mark.parametrize("prop", [1, 2])
+def test_brunel_reference_generator_fixed_format(prop):
+    _say(f"case: make_brunel(2e5), F32 fixed-point format vs oracle fixed mode, prop={prop}")
     net = M.brunel_ref(200_000, rate=0.01)
     syn = H.host_synapses(net, mt_seed=99)
-    n = _compare_run(net, syn, 1000, seed=3, gpu_kw=dict(weight_format=2), acc=po.ACC_FIXED)
+    n = _compare_run(net, syn, 1000, seed=3, gpu_kw=dict(weight_format=2, propagation=prop), acc=po.ACC_FIXED)
     assert n > 1000
 
 

@@ -61,18 +61,22 @@ def ranged_net():
     return net
 
 
-@pytest.mark.parametrize("wf", [N.WF_BF16, N.WF_FP8])
-def test_stored_weights_and_bit_exact_dynamics(wf):
-    _say(f"case: ranged-weight net, weight_format={wf}")
+@pytest.mark.parametrize("prop", [N.PROP_DIRECT, N.PROP_TILED])
+@pytest.mark.parametrize("wf", [N.WF_BF16, N.WF_FP8, N.WF_F32])
+def test_stored_weights_and_bit_exact_dynamics(wf, prop):
+    _say(f"case: ranged-weight net, weight_format={wf}, propagation={prop}")
     net = ranged_net()
     syn = H.host_synapses(net, mt_seed=77, rand_seed=5, dw=0.0001)
     assert len(np.unique(syn.weight)) > 20
-    g = H.gpu_from_list(net, syn, seed=9, weight_format=wf, record_capacity=1_000_000)
+    g = H.gpu_from_list(net, syn, seed=9, weight_format=wf, record_capacity=1_000_000, propagation=prop)
+    assert g.info()["propagation"] == prop
     rp, col, dl, w = g.csr()
     c = po.build_csr(net.n, syn)
     assert np.array_equal(rp, c.row_ptr) and np.array_equal(col, c.col) and np.array_equal(dl, c.dly)
     if wf == N.WF_BF16:
         want = bf16_round(c.w)
+    elif wf == N.WF_F32:
+        want = c.w.copy()
     else:
         want = np.zeros_like(c.w)
         for i in range(net.n):

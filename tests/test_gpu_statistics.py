@@ -7,11 +7,16 @@ Sources: benchdata/single/mgbrain_brunel_10000000_nvprof.txt:28 (27.1845 Hz),
 benchdata/single2/mgbrain_single_brunel_0.01_20000000_nvprof.txt:11 (156.988),
 ..._0.05_20000000_nvprof.txt:11 (450.556), ..._0.1_20000000_nvprof.txt:11
 (718.904), ..._0.01_100000000_nvprof.txt:11 (163.416).
+
+Beyond the bit-exact window, per-population rates and the CV of the
+inter-spike intervals over 1 s are compared with the FP64 oracle (2 %).
 """
 import sys
 
+import numpy as np
 import pytest
 
+from oracle import pyoracle as po
 from tests import harness as H
 from tests import netmodels as M
 
@@ -35,3 +40,48 @@ def test_brunel_rate_against_benchdata(nsyn, p, ref):
     P = net.pops[0][0]
     assert abs(cnt[:P].mean() / (p / 1e-4) - 1.0) < 0.02       # Poisson inputs at p / dt
     assert abs(rate - ref) / ref < 0.02, (rate, ref)
+
+
+def _rates_cv(steps, ids, n, bounds, seconds):
+    """Per-population mean rate (Hz) and mean CV of the inter-spike intervals
+    (neurons with >= 3 spikes)."""
+    order = np.lexsort((steps, ids))
+    s, i = steps[order], ids[order]
+    out = []
+    for lo, hi in bounds:
+        m = (i >= lo) & (i < hi)
+        si, ii = s[m], i[m]
+        rate = len(si) / (hi - lo) / seconds
+        cvs = []
+        if len(si):
+            cut = np.flatnonzero(np.diff(ii)) + 1
+            for a, b in zip(np.r_[0, cut], np.r_[cut, len(si)]):
+                if b - a >= 3:
+                    isi = np.diff(si[a:b]).astype(np.float64)
+                    cvs.append(isi.std() / isi.mean())
+        out.append((rate, float(np.mean(cvs)) if cvs else float("nan"), len(cvs)))
+    return out
+
+
+def test_population_rates_and_cv_isi_against_fp64():
+    """After the bit-exact window: per-population rates and CV of ISI of the FP32
+    HIP path over 1 s against the FP64 oracle on the same network, within 2 %."""
+    print("case: make_brunel(1e7) Poisson p=0.002, 10000 steps, FP32 GPU vs FP64 oracle", flush=True)
+    net = M.brunel_ref(10_000_000, rate=0.002)
+    syn = H.host_synapses(net, mt_seed=4242)
+    g = H.gpu_from_list(net, syn, seed=5, record_capacity=4_000_000)
+    g.run(10000)
+    gs, gi = g.spikes()
+    g.close()
+    _, o = H.oracle(net, syn, seed=5, acc=po.ACC_F64)
+    want = o.run(10000, cap=4_000_000)
+    o.close()
+    os_ = np.concatenate([np.full(len(x), k, np.int32) for k, x in enumerate(want)])
+    oi = np.concatenate(want).astype(np.int32)
+    first = net.first()
+    bounds = [(first[k], first[k] + p[0]) for k, p in enumerate(net.pops)]
+    got, ref = _rates_cv(gs, gi, net.n, bounds, 1.0), _rates_cv(os_, oi, net.n, bounds, 1.0)
+    for (r1, c1, n1), (r2, c2, n2), (num, typ, _src) in zip(got, ref, net.pops):
+        print(f"  pop n={num} type={typ}: rate {r1:.3f} vs {r2:.3f} Hz, CV-ISI {c1:.4f} vs {c2:.4f}", flush=True)
+        assert abs(r1 - r2) <= 0.02 * r2, (r1, r2)
+        assert abs(c1 - c2) <= 0.02 * c2, (c1, c2)
