@@ -39,6 +39,36 @@ brunel_fixed_k = W.brunel_fixed_k
 brunel_bernoulli = W.brunel_bernoulli
 
 
+def fused_roofline(b, info0, steps, se, spikes, n_lif, n_poisson, kname):
+    """HBM roofline of the fused kernel over the timed region.
+
+    achieved = algorithmic bytes per launch / average launch duration, the bytes being the
+    reference step's per-unit figures: per step 16 B of state per LIFB neuron and 4 B per
+    Poisson neuron, 16 B ring read + clear per receiving neuron; per spike 8 B row pointers
+    + 4 B list entry; per synaptic event 4 B target + 4 B FP32 weight + 1 B delay.
+    layout_bytes_per_launch = what this layout itself must move: word_bytes per event,
+    12 B per spike, 4 B per spike and tile of its segment, the ring rows live across
+    windows in and out, 32 B per LIF / 20 B per Poisson neuron of state in and out."""
+    launches = b["prop_launches"]
+    if not launches or b["prop_ms"] <= 0:
+        return None
+    avg_s = b["prop_ms"] / 1e3 / launches
+    alg = steps * (32.0 * n_lif + 4.0 * n_poisson) + 12.0 * spikes + 9.0 * se
+    rw = 1 if b["word_bytes"] in (2, 4) and b["weight_format"] == N.WF_CLASS else 2
+    live = max(info0["ring_len"] - info0["window"], 0)
+    g_seg = b["tiles"] / max(1, b["segments"])
+    lay = (b["word_bytes"] * se + 12.0 * spikes + 4.0 * spikes * g_seg
+           + launches * (8.0 * rw * live * b["tiles"] * b["tile_size"] + 32.0 * n_lif + 20.0 * n_poisson))
+    ach = alg / launches / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
+            "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(alg / launches),
+            "layout_bytes_per_launch": round(lay / launches),
+            "layout_frac": round(lay / launches / avg_s / 1e9 / HBM_PEAK_GBS, 4),
+            "launches": launches, "tile_size": b["tile_size"], "tiles": b["tiles"], "block": b["block"],
+            "word_bytes": b["word_bytes"]}
+
+
 def cpu_baseline(sim, budget_s=12.0, max_steps=4000, seed=0):
     """The oracle (FP32 restatement of the reference step, one thread) on the same
     network (CSR read back from the device), timed on this host."""
@@ -104,17 +134,10 @@ def batched_leg(args, shapes=None, log=None):
     se = b["syn_events"] - a["syn_events"]
     spikes = b["spikes"] - a["spikes"]
     roof = None
-    if b["propagation"] == N.PROP_TILED and b["prop_launches"] > 0:
+    if b["propagation"] == N.PROP_TILED:
         n_poisson = sum(p["num"] for x in nets for p in x["pops"] if p["type"] == N.POISSON)
-        n_lif = info0["n_neurons"] - n_poisson
-        alg = (4.0 * se + 12.0 * spikes + 4.0 * spikes * (b["tiles"] / max(1, b["segments"]))
-               + b["prop_launches"] * (8.0 * info0["ring_len"] * b["tiles"] * b["tile_size"]
-                                       + 32.0 * n_lif + 20.0 * n_poisson))
-        avg_s = b["prop_ms"] / 1e3 / b["prop_launches"]
-        ach = alg / b["prop_launches"] / avg_s / 1e9 if avg_s > 0 else 0.0
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_s * 1e6, 3),
-                "bytes_per_launch": round(alg / b["prop_launches"])}
+        roof = fused_roofline(b, info0, args.steps, se, spikes, info0["n_neurons"] - n_poisson, n_poisson,
+                              "k_tiled (fused, batched segments)")
     log(f"# batched: {args.steps} steps in {wall * 1e3:.1f} ms, {se:.4g} events, {se / wall:.4g} SE/s, "
         f"RTF {wall / (args.steps * DT):.4f}")
     # the same networks one after another, each in its own simulator
@@ -148,7 +171,7 @@ def batched_leg(args, shapes=None, log=None):
                         f"delays U{{1.0..2.0}} ms, seeds 0-{len(nets) - 1}"),
         "value": se / wall, "unit": "syn_events/s", "rtf": wall / (args.steps * DT),
         "neurons": info0["n_neurons"], "synapses": info0["n_synapses"], "segments": info0["segments"],
-        "weight_format": "f32 per-class (4 B synapse word; exact)", "graph": args.graph,
+        "weight_format": f"f32 per-class ({info0['word_bytes']} B synapse word; exact)", "graph": args.graph,
         "setup_s": round(setup_s, 2), "syn_events": se,
         "sequential_value": seq_se / seq_wall, "sequential_rtf": seq_wall / (args.steps * DT),
         "speedup_vs_sequential": (se / wall) / (seq_se / seq_wall),
@@ -245,7 +268,7 @@ def main(argv=None):
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg of the default run")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--graph", type=int, default=0)
+    ap.add_argument("--graph", type=int, default=1, help="replay the step loop as hipGraphs (default 1)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -359,41 +382,30 @@ def main(argv=None):
 
     roof = None
     if b["prop_launches"] > 0:
-        n = info0["n_neurons"]
-        D = info0["ring_len"]
         n_poisson = int(sum(p["num"] for p in pops if p["type"] == N.POISSON))
-        n_lif = n - n_poisson
-        launches = b["prop_launches"]
+        n_lif = info0["n_neurons"] - n_poisson
         if b["propagation"] == N.PROP_TILED:
-            # fused window kernel, bytes each launch must move through HBM (each datum once):
-            # synapse words 4 B/event; spike entry 4 B written + row pointers 8 B per spike;
-            # (source, tile) offset pairs 4 B per spike and tile; ring slice in + out 8 B per
-            # (slot, target); LIFB state V, refr, last, cnt read + write 32 B, Poisson rate +
-            # last + cnt 20 B per neuron
-            G = b["tiles"]
-            alg = (4.0 * se + 12.0 * spikes + 4.0 * spikes * G
-                   + launches * (8.0 * D * G * b["tile_size"] + 32.0 * n_lif + 20.0 * n_poisson))
-            kname = "k_tiled (fused neuron update + CLASS propagation)"
+            roof = fused_roofline(b, info0, args.steps, se, spikes, n_lif, n_poisson,
+                                  "k_tiled (fused neuron update + CLASS propagation)")
+            # measured HBM bytes per launch: FETCH_SIZE / WRITE_SIZE counter passes of this
+            # workload scaled by a copy kernel of known size (profiles/traffic_*.json)
+            tfile = os.path.join(ROOT, "profiles", "traffic_" + workload.replace("_fixed_indegree", "") + ".json")
+            if roof and os.path.exists(tfile):
+                t = json.load(open(tfile))
+                if (t.get("tile_size"), t.get("block"), t.get("chunk"), t.get("word_bytes")) == \
+                        (b["tile_size"], b["block"], tuned_chunk, b["word_bytes"]):
+                    roof["traffic"] = round(t["traffic_bytes_per_launch"])
         else:
-            alg = 12.0 * spikes + 4.0 * se
-            kname = "k_prop (CLASS, DIRECT)"
-        per_launch = alg / launches
-        avg_s = b["prop_ms"] / 1e3 / launches
-        ach = per_launch / avg_s / 1e9
-        # measured HBM bytes per launch: FETCH_SIZE / WRITE_SIZE counter passes of this
-        # workload scaled by a copy kernel of known size (profiles/traffic_*.json)
-        traffic = None
-        tfile = os.path.join(ROOT, "profiles", "traffic_" + workload.replace("_fixed_indegree", "") + ".json")
-        if os.path.exists(tfile):
-            t = json.load(open(tfile))
-            if (t.get("tile_size"), t.get("block"), t.get("chunk")) == (b["tile_size"], b["block"], tuned_chunk):
-                traffic = round(t["traffic_bytes_per_launch"])
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
-                "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(per_launch),
-                "launches": launches, "tile_size": b["tile_size"], "tiles": b["tiles"], "block": b["block"]}
-        print(f"# {kname}: {launches} launches, avg {avg_s * 1e6:.2f} us, {per_launch / 1e6:.3f} MB "
-              f"algorithmic -> {ach:.1f} GB/s", file=sys.stderr, flush=True)
+            avg_s = b["prop_ms"] / 1e3 / b["prop_launches"]
+            alg = 12.0 * spikes + 9.0 * se
+            ach = alg / b["prop_launches"] / avg_s / 1e9
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_prop (direct)",
+                    "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(alg / b["prop_launches"])}
+        if roof:
+            print(f"# {roof['kernel']}: {roof['launches'] if 'launches' in roof else b['prop_launches']} launches, "
+                  f"avg {roof['avg_launch_us']:.2f} us, {roof['bytes_per_launch'] / 1e6:.3f} MB algorithmic -> "
+                  f"{roof['achieved']:.1f} GB/s", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

@@ -100,6 +100,7 @@ struct mgb_sim {
     uint32_t* tcnt = nullptr;          // [3][nseg]
     int tkind = kTClass;               // synapse word kind of the fused kernel
     uint32_t* syn_t = nullptr;
+    uint16_t* syn16 = nullptr;         // kTClass16 words in tile order
     float* w_t = nullptr;              // kTF32 weights in tile order
     uint16_t* offs16 = nullptr;
     uint32_t* offs32 = nullptr;
@@ -597,7 +598,8 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         for (size_t c = 0; c < keys.size(); ++c) {
             for (int side = 0; side < 2; ++side) {
                 const uint32_t bits = side ? keys[c].second : keys[c].first;
-                const uint32_t len = (side ? tmax_i[c] : tmax_e[c]) + 1;
+                // at least kClassCache entries: the fused kernel caches that many
+                const uint32_t len = std::max<uint32_t>((side ? tmax_i[c] : tmax_e[c]) + 1, kClassCache);
                 float w = 0.f;
                 if (bits != kWSentinel) std::memcpy(&w, &bits, 4);
                 s->class_w[2 * c + side] = w;
@@ -640,8 +642,15 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
                 }
                 MGB_HIP_TRY(sort_pairs(temp, &tb2, kin, kout, vin, vout, S, tbits, st));
             }
-            MGB_TRY(dalloc_n(s, &s->syn_t, S));
-            if (S) MGB_HIP_TRY(hipMemcpyAsync(s->syn_t, vout, S * 4, hipMemcpyDeviceToDevice, st));
+            if (s->B <= 1024) {
+                // 2 B per synapse: the target is stored relative to its tile
+                s->tkind = kTClass16;
+                MGB_TRY(dalloc_n(s, &s->syn16, S));
+                MGB_HIP_TRY(class16_words(vout, S, tt, s->syn16, st));
+            } else {
+                MGB_TRY(dalloc_n(s, &s->syn_t, S));
+                if (S) MGB_HIP_TRY(hipMemcpyAsync(s->syn_t, vout, S * 4, hipMemcpyDeviceToDevice, st));
+            }
             MGB_TRY(build_tile_offsets(s, n, kout, tt));
         }
         MGB_HIP_TRY(hipStreamSynchronize(st));
@@ -884,7 +893,7 @@ TiledArgs tiled_args(const mgb_sim* s) {
     a.refr = s->refr; a.last = s->last; a.cnt = s->cnt;
     a.csets = s->csets; a.stream_base = s->stream_base; a.stream_seed = s->stream_seed;
     a.tbl_off = s->tbl_off; a.seq = s->seq; a.tbl_w = s->tbl_w;
-    a.row_ptr = s->row_ptr; a.syn_t = s->syn_t; a.offs16 = s->offs16; a.offs32 = s->offs32;
+    a.row_ptr = s->row_ptr; a.syn_t = s->syn_t; a.syn16 = s->syn16; a.offs16 = s->offs16; a.offs32 = s->offs32;
     a.ring_t = s->ring_t; a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
     a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
     return a;
@@ -1491,6 +1500,7 @@ mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
     out->tiles = s->G;
     out->block = s->tiled ? s->tblock : s->prop_block;
     out->segments = s->tiled ? s->nseg : 1;
+    out->word_bytes = !s->tiled ? 0 : (s->tkind == kTClass16 ? 2 : (s->tkind == kTF32 ? 8 : 4));
     out->last_run_ms = 0.0;
     if (s->timed) {
         float ms = 0.f;

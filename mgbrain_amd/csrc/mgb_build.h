@@ -81,6 +81,7 @@ hipError_t make_sort_input(const int32_t* src, const int32_t* tar, const float* 
                            hipStream_t st);
 hipError_t make_tile_keys(const int32_t* src, const int32_t* tar, const float* w, const uint8_t* dq,
                           const TileTabs& tt, uint32_t* keys, uint32_t* vals, uint64_t S, hipStream_t st);
+hipError_t class16_words(const uint32_t* w32, uint64_t S, const TileTabs& tt, uint16_t* out, hipStream_t st);
 hipError_t make_tile_keys_csr(const uint32_t* src, const int32_t* col, const TileTabs& tt, uint32_t* keys,
                               uint32_t* vals, uint64_t S, hipStream_t st);
 hipError_t tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col, const uint8_t* dly,

@@ -115,6 +115,19 @@ __global__ void k_tile_keys(const int32_t* src, const int32_t* tar, const float*
     }
 }
 
+// 16-bit CLASS words (tiles of at most 1024 targets): tile-local target |
+// delay-1 << 10 | inhibitory << 15, from the 32-bit words in tile order.
+__global__ void k_class16(const uint32_t* w32, uint64_t S, TileTabs tt, uint16_t* out) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t w = w32[k];
+        const int32_t t = (int32_t)(w & 0x3FFFFFFu);
+        const int32_t q = tt.seg_of ? (int32_t)tt.seg_of[t] : 0;
+        const uint32_t loc = (uint32_t)((t - tt.seg_lo[q]) % tt.B);
+        out[k] = (uint16_t)(loc | (((w >> 26) & 31u) << 10) | ((w >> 31) << 15));
+    }
+}
+
 // Tile keys of the CSR slots of a wide-format network: slot k (source src[k],
 // target col[k]); the value is the slot itself.
 __global__ void k_tile_keys_csr(const uint32_t* src, const int32_t* col, TileTabs tt, uint32_t* keys,
@@ -356,6 +369,12 @@ hipError_t make_tile_keys(const int32_t* src, const int32_t* tar, const float* w
     if (S == 0) return hipSuccess;
     hipLaunchKernelGGL(k_tile_keys, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, tar, w, dq, tt, keys,
                        vals, S);
+    return hipGetLastError();
+}
+
+hipError_t class16_words(const uint32_t* w32, uint64_t S, const TileTabs& tt, uint16_t* out, hipStream_t st) {
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_class16, dim3(blocks_for(S, 256)), dim3(256), 0, st, w32, S, tt, out);
     return hipGetLastError();
 }
 

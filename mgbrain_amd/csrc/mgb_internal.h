@@ -142,8 +142,9 @@ struct StdpArgs {
 // one segment, each segment has its own spike lists, and a source row of
 // segment q holds offsets for q's tiles only: offs[seg_obase[q] +
 // (src - seg_lo[q]) * (Gq + 1) + local tile], Gq = tiles of q.
-// Synapse word kinds of the fused kernel.
-enum TiledKind { kTClass = 0, kTBF16 = 1, kTFP8 = 2, kTF32 = 3 };
+// Synapse word kinds of the fused kernel.  kTClass16 is the CLASS word of a
+// tiling with B <= 1024: tile-local target (10 bits) | delay-1 << 10 | sign << 15.
+enum TiledKind { kTClass = 0, kTBF16 = 1, kTFP8 = 2, kTF32 = 3, kTClass16 = 4 };
 
 struct TiledArgs {
     int32_t n, D, B, G;
@@ -179,6 +180,7 @@ struct TiledArgs {
     const float* tbl_w;
     const int64_t* row_ptr;
     const uint32_t* syn_t;
+    const uint16_t* syn16;      // kTClass16 words
     const uint16_t* offs16;
     const uint32_t* offs32;
     uint32_t* ring_t;
@@ -201,7 +203,8 @@ hipError_t launch_stdp(const StdpArgs& a, int32_t off, int32_t wk, int grid, hip
 hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
 constexpr int kTiledBatch = 1024;   // spikes staged in LDS per propagation batch
 constexpr int kTiledItems = 8192;   // work items staged in LDS per propagation round
-constexpr int kTiledStaticLds = 256; // bytes reserved for the fused kernel's static LDS
+constexpr int kTiledStaticLds = 1024; // bytes reserved for the fused kernel's static LDS
+constexpr int kClassCache = 32;     // k-fold class sums cached in LDS (tables hold >= this many)
 size_t tiled_lds_bytes(int D, int B, int kind);
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st);

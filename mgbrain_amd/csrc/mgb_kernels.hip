@@ -51,6 +51,7 @@ struct NState {
     int32_t refr, last, cnt, refr_steps;
     float c3, c4, c12, c13, c14, c15, c16, c17;
     uint32_t off_e, off_i;       // CLASS seq-sum table offsets
+    uint32_t cls;                // CLASS weight class
     float we, wi;                // CLASS weights
     uint32_t key0, key1, lid;    // Poisson stream
     U32x4 rnd;
@@ -66,7 +67,7 @@ __device__ __forceinline__ void neuron_load(const A& a, int i, int type, bool cl
     st.V = st.Ie = st.Ii = st.rate = 0.f;
     st.refr = st.refr_steps = 0;
     st.c3 = st.c4 = st.c12 = st.c13 = st.c14 = st.c15 = st.c16 = st.c17 = 0.f;
-    st.off_e = st.off_i = st.key0 = st.key1 = st.lid = 0u;
+    st.off_e = st.off_i = st.key0 = st.key1 = st.lid = st.cls = 0u;
     st.we = st.wi = 0.f;
     st.rnd = U32x4{{0u, 0u, 0u, 0u}};
     st.last = a.last[i];
@@ -95,6 +96,7 @@ __device__ __forceinline__ void neuron_load(const A& a, int i, int type, bool cl
         }
         if (class_tables) {
             const uint32_t t = a.tbl[i];
+            st.cls = t;
             st.off_e = a.tbl_off[2 * t];
             st.off_i = a.tbl_off[2 * t + 1];
             st.we = a.tbl_w[2 * t];
@@ -423,9 +425,14 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     __shared__ uint32_t wsum[16];
     __shared__ unsigned long long se_blk, lsp_r;
     __shared__ uint32_t lsp_n, lsp_g;
+    // CLASS: the first kClassCache k-fold sums of the classes of the tile's
+    // last and first neurons (one class per tile in population-ordered networks)
+    __shared__ float ccache[2][2][kClassCache];
+    __shared__ uint32_t ccls[2];
     const int32_t D = a.D, B = a.B, g = (int32_t)blockIdx.x, bd = (int32_t)blockDim.x;
     const int32_t tid = (int32_t)threadIdx.x;
-    constexpr int RW = (WK == kTClass) ? 1 : 2;             // ring words per (slot, target)
+    constexpr bool kCls = WK == kTClass || WK == kTClass16;  // CLASS arrival counts
+    constexpr int RW = kCls ? 1 : 2;                        // ring words per (slot, target)
     uint32_t* ring = lds;                                   // [RW][D][B]
     uint32_t* sp_beg = ring + (size_t)RW * D * B;           // [kTiledBatch] first CSR slot
     uint32_t* sp_len = sp_beg + kTiledBatch;                // len | slot-of-spike << 26
@@ -451,6 +458,12 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     if (tid == 0) {
         se_blk = 0ull;
         lsp_n = 0u;
+    }
+    if (kCls && tid < 4 * kClassCache) {
+        const int32_t c = tid / (2 * kClassCache), side = (tid / kClassCache) & 1, k = tid % kClassCache;
+        const uint32_t cl = a.tbl[c == 0 ? t0 + tn - 1 : t0];
+        ccache[c][side][k] = a.seq[a.tbl_off[2 * cl + side] + k];
+        if (side == 0 && k == 0) ccls[c] = cl;
     }
     // 1. ring slice -> LDS.  The slots the previous window consumed ([sprev, s0)
     // mod D) were cleared by it and not stored back (step 4): they are zero here.
@@ -512,60 +525,70 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             // first LDS atomic
             constexpr int U = CH >= 16 ? 1 : 16 / CH;
             for (uint32_t it = (uint32_t)tid; it < rn; it += (uint32_t)(U * bd)) {
+                // branch-free body, so that all U * CH loads are in flight before the
+                // first wait: lanes past the round's items re-read the last item and
+                // elements past an item's end re-read its last element; both add 0
                 uint32_t cnt[U], kb[U];
                 int32_t sD[U];
                 float rs[U];
 #pragma unroll
                 for (int h = 0; h < U; ++h) {
-                    const uint32_t q = it + (uint32_t)(h * bd);
-                    cnt[h] = 0u;
-                    kb[h] = 0u;
-                    sD[h] = 0;
-                    rs[h] = 1.0f;
-                    if (q < rn) {
-                        const uint32_t j = itm[q];
-                        const uint32_t c = r0 + q - sp_start[j];
-                        const uint32_t meta = sp_len[j];
-                        const uint32_t len = meta & kIdMask;
-                        sD[h] = (int32_t)(meta >> 26);
-                        kb[h] = sp_beg[j] + c * CH;
-                        cnt[h] = min((uint32_t)CH, len - c * CH);
-                        if (WK == kTFP8) rs[h] = sp_rs[j];
-                    }
+                    const uint32_t q0 = it + (uint32_t)(h * bd);
+                    const uint32_t q = min(q0, rn - 1u);
+                    const uint32_t j = itm[q];
+                    const uint32_t c = r0 + q - sp_start[j];
+                    const uint32_t meta = sp_len[j];
+                    const uint32_t len = meta & kIdMask;
+                    sD[h] = (int32_t)(meta >> 26);
+                    kb[h] = sp_beg[j] + c * CH;
+                    cnt[h] = (q0 < rn) ? min((uint32_t)CH, len - c * CH) : 0u;
+                    rs[h] = (WK == kTFP8) ? sp_rs[j] : 1.0f;
                 }
                 uint32_t x[U][CH];
                 float wv[U][CH];
 #pragma unroll
-                for (int h = 0; h < U; ++h)
+                for (int h = 0; h < U; ++h) {
+                    const uint32_t last = max(cnt[h], 1u) - 1u;
 #pragma unroll
-                    for (int u = 0; u < CH; ++u)
-                        if ((uint32_t)u < cnt[h]) {
-                            x[h][u] = __builtin_nontemporal_load(&a.syn_t[kb[h] + u]);
-                            if (WK == kTF32) wv[h][u] = __builtin_nontemporal_load(&a.w_t[kb[h] + u]);
-                        }
+                    for (int u = 0; u < CH; ++u) {
+                        const uint32_t k = kb[h] + min((uint32_t)u, last);
+                        if (WK == kTClass16) x[h][u] = __builtin_nontemporal_load(&a.syn16[k]);
+                        else x[h][u] = __builtin_nontemporal_load(&a.syn_t[k]);
+                        if (WK == kTF32) wv[h][u] = __builtin_nontemporal_load(&a.w_t[k]);
+                    }
+                }
 #pragma unroll
                 for (int h = 0; h < U; ++h)
 #pragma unroll
                     for (int u = 0; u < CH; ++u) {
-                        if ((uint32_t)u < cnt[h]) {
-                            const uint32_t xw = x[h][u];
-                            if (WK == kTClass) {
-                                int32_t slot = sD[h] + (int32_t)((xw >> 26) & 31u) + 1;
-                                slot -= (slot >= D) ? D : 0;
-                                const int32_t loc = (int32_t)(xw & kIdMask) - t0;
-                                atomicAdd(&ring[slot * B + loc], (xw >> 31) ? 0x10000u : 1u);
-                            } else {
-                                int32_t slot = sD[h] + (int32_t)((xw >> 11) & 31u) + 1;
-                                slot -= (slot >= D) ? D : 0;
-                                const int32_t loc = (int32_t)(xw & 0x7FFu);
-                                float w;
-                                if (WK == kTBF16) w = bf16_to_f32((uint16_t)(xw >> 16));
-                                else if (WK == kTFP8) w = fp8e4m3_to_f32((uint8_t)(xw >> 16)) * rs[h];
-                                else w = wv[h][u];
-                                int32_t* rq = reinterpret_cast<int32_t*>(ring);
-                                if (w > 0.f) atomicAdd(&rq[slot * B + loc], (int32_t)rintf(w * a.scale_e));
-                                else atomicAdd(&rq[(D + slot) * B + loc], (int32_t)rintf(w * a.scale_i));
-                            }
+                        const bool valid = (uint32_t)u < cnt[h];
+                        const uint32_t xw = x[h][u];
+                        // past the round's items: a distinct entry per lane, adding 0
+                        const bool idle = cnt[h] == 0u;
+                        if (WK == kTClass) {
+                            int32_t slot = sD[h] + (int32_t)((xw >> 26) & 31u) + 1;
+                            slot -= (slot >= D) ? D : 0;
+                            const int32_t loc = (int32_t)(xw & kIdMask) - t0;
+                            const int32_t at = idle ? (tid % B) : slot * B + loc;
+                            atomicAdd(&ring[at], valid ? ((xw >> 31) ? 0x10000u : 1u) : 0u);
+                        } else if (WK == kTClass16) {
+                            int32_t slot = sD[h] + (int32_t)((xw >> 10) & 31u) + 1;
+                            slot -= (slot >= D) ? D : 0;
+                            const int32_t loc = (int32_t)(xw & 0x3FFu);
+                            const int32_t at = idle ? (tid % B) : slot * B + loc;
+                            atomicAdd(&ring[at], valid ? ((xw >> 15) ? 0x10000u : 1u) : 0u);
+                        } else {
+                            int32_t slot = sD[h] + (int32_t)((xw >> 11) & 31u) + 1;
+                            slot -= (slot >= D) ? D : 0;
+                            const int32_t loc = (int32_t)(xw & 0x7FFu);
+                            float w;
+                            if (WK == kTBF16) w = bf16_to_f32((uint16_t)(xw >> 16));
+                            else if (WK == kTFP8) w = fp8e4m3_to_f32((uint8_t)(xw >> 16)) * rs[h];
+                            else w = wv[h][u];
+                            const bool exc = w > 0.f;
+                            const int32_t v = (int32_t)rintf(w * (exc ? a.scale_e : a.scale_i));
+                            const int32_t at = idle ? (tid % B) : ((exc ? 0 : D) + slot) * B + loc;
+                            atomicAdd(reinterpret_cast<int32_t*>(ring) + at, valid ? v : 0);
                         }
                     }
             }
@@ -588,21 +611,38 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         const int32_t i = t0 + j;
         const int type = active ? (int)a.type[i] : T_RAW;
         NState st;
-        if (active) neuron_load(a, i, type, WK == kTClass, st);
+        if (active) neuron_load(a, i, type, kCls, st);
         const bool integ = integrates(type);
         int32_t outdeg = 0;
         if (active && type != T_RAW && type != T_LIF0)
             outdeg = (int32_t)(a.row_ptr[i + 1] - a.row_ptr[i]);
+        // wave-uniform fast path: every active lane an LIFB neuron of a cached class
+        int32_t csl = -1;
+        if (kCls && active && integ) csl = (st.cls == ccls[0]) ? 0 : ((st.cls == ccls[1]) ? 1 : -1);
+        const bool lean = kCls && __all(!active || (type == T_LIFB && csl >= 0));
         int32_t slot = slot0;
         for (int32_t w = 0; w < nsteps; ++w) {
             const int32_t s = (int32_t)(s0 + w);
             bool fired = false;
-            if (type == T_POISSON) {
+            if (lean) {
+                if (active) {
+                    uint32_t* r = &ring[slot * B + j];
+                    const uint32_t x = *r;
+                    *r = 0u;                                          // clearInput
+                    const uint32_t ke = x & 0xFFFFu, ki = x >> 16;
+                    const float* cc = &ccache[csl][0][0];
+                    float be = cc[min(ke, (uint32_t)kClassCache - 1u)];
+                    float bi = cc[kClassCache + min(ki, (uint32_t)kClassCache - 1u)];
+                    if (ke >= (uint32_t)kClassCache) be = a.seq[st.off_e + ke];
+                    if (ki >= (uint32_t)kClassCache) bi = a.seq[st.off_i + ki];
+                    fired = integrate_step(T_LIFB, st, s, be, bi);
+                }
+            } else if (type == T_POISSON) {
                 fired = poisson_step(st, s, w);
             } else if (integ) {
                 float be, bi;
                 uint32_t* r = &ring[slot * B + j];
-                if (WK == kTClass) {
+                if (kCls) {
                     const uint32_t x = *r;
                     *r = 0u;                                          // clearInput
                     class_input(x, a.seq, st, be, bi);
@@ -739,7 +779,7 @@ hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t
 }
 
 size_t tiled_lds_bytes(int D, int B, int kind) {
-    const size_t rw = (kind == kTClass) ? 1 : 2;
+    const size_t rw = (kind == kTClass || kind == kTClass16) ? 1 : 2;
     return sizeof(uint32_t) * (rw * (size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
            sizeof(uint16_t) * (size_t)kTiledItems + (kind == kTFP8 ? sizeof(float) * kTiledBatch : 0);
 }
@@ -790,6 +830,7 @@ hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nst
         case kTBF16: launch_tiled_kind<kTBF16>(a, off, wk, nsteps, block, lds, st); break;
         case kTFP8: launch_tiled_kind<kTFP8>(a, off, wk, nsteps, block, lds, st); break;
         case kTF32: launch_tiled_kind<kTF32>(a, off, wk, nsteps, block, lds, st); break;
+        case kTClass16: launch_tiled_kind<kTClass16>(a, off, wk, nsteps, block, lds, st); break;
         default: launch_tiled_kind<kTClass>(a, off, wk, nsteps, block, lds, st); break;
     }
     return hipGetLastError();
@@ -800,6 +841,7 @@ hipError_t set_tiled_lds_limit(int kind, size_t bytes) {
         case kTBF16: return set_lds_kind<kTBF16>(bytes);
         case kTFP8: return set_lds_kind<kTFP8>(bytes);
         case kTF32: return set_lds_kind<kTF32>(bytes);
+        case kTClass16: return set_lds_kind<kTClass16>(bytes);
         default: return set_lds_kind<kTClass>(bytes);
     }
 }

@@ -73,7 +73,8 @@ class Info(C.Structure):
                 ("spikes", C.c_int64), ("neuron_ms", C.c_double), ("prop_ms", C.c_double),
                 ("stdp_ms", C.c_double), ("neuron_launches", C.c_int64), ("prop_launches", C.c_int64),
                 ("stdp_launches", C.c_int64), ("propagation", C.c_int32), ("tile_size", C.c_int32),
-                ("tiles", C.c_int32), ("block", C.c_int32), ("segments", C.c_int32)]
+                ("tiles", C.c_int32), ("block", C.c_int32), ("segments", C.c_int32),
+                ("word_bytes", C.c_int32)]
 
 
 _lib = None

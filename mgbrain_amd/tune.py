@@ -17,7 +17,7 @@ import math
 import time
 
 
-def tile_candidates(n, ring_len, lds_bytes=160 * 1024, batch_words=3 * 1024 + 4096, static_bytes=256):
+def tile_candidates(n, ring_len, lds_bytes=160 * 1024, batch_words=3 * 1024 + 4096, static_bytes=1024):
     bmax = max(1, ((lds_bytes - static_bytes) // 4 - batch_words) // max(1, ring_len))
     out = []
     for g in (128, 256, 512, 1024, 2048):
