@@ -47,8 +47,10 @@ enum {
     MGB_WF_CLASS = 1,  /* 4 B/synapse: target|delay|sign; per-class weight, exact integer
                           counts reproduce the reference's FP32 sum of equal addends */
     MGB_WF_F32 = 2,    /* per-synapse FP32 weight, int32 fixed-point accumulation */
-    MGB_WF_BF16 = 3,   /* per-synapse BF16 weight (plastic synapses stay FP32) */
-    MGB_WF_FP8 = 4     /* per-synapse FP8-E4M3 weight with a per-population scale */
+    MGB_WF_BF16 = 3,   /* per-synapse BF16 of w / s, s = the source row's largest |w|
+                          (plastic synapses stay FP32) */
+    MGB_WF_FP8 = 4     /* per-synapse FP8-E4M3 of w / s, s = the source row's largest |w|
+                          / 448, or the caller's fp8_scale */
 };
 
 typedef struct mgb_opts {

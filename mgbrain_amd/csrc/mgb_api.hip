@@ -669,17 +669,18 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         MGB_TRY(dalloc_n(s, &s->dly, S));
         MGB_TRY(dalloc_n(s, &s->w32, S));
         if (s->wide_kind == kWideBF16) MGB_TRY(dalloc_n(s, &s->w16, S));
-        if (s->wide_kind == kWideFP8) {
-            MGB_TRY(dalloc_n(s, &s->w8, S));
+        if (s->wide_kind == kWideFP8) MGB_TRY(dalloc_n(s, &s->w8, S));
+        if (s->wide_kind != kWideF32) {
             MGB_TRY(dalloc_n(s, &s->row_scale, (size_t)n));
-            if (!h.fp8_scale.empty()) {
+            if (s->wide_kind == kWideFP8 && !h.fp8_scale.empty()) {
                 MGB_HIP_TRY(hipMemcpyAsync(s->row_scale, h.fp8_scale.data(), 4 * (size_t)n,
                                            hipMemcpyHostToDevice, st));
             } else {
-                // per-source-row scale: the row's largest |w| maps onto 448
+                // per-source-row scale: the row's largest |w| maps onto 448 (FP8) or 1 (BF16)
                 uint32_t* rmax = nullptr;
                 MGB_TRY(X.get(&rmax, (size_t)n));
-                MGB_HIP_TRY(fp8_row_scales(L.src, L.weight, S, n, rmax, s->row_scale, st));
+                MGB_HIP_TRY(row_scales(L.src, L.weight, S, n, s->wide_kind == kWideFP8 ? 448.f : 1.f, rmax,
+                                       s->row_scale, st));
             }
         }
         uint32_t* pos_of = nullptr;
@@ -1621,11 +1622,18 @@ mgb_status mgb_read_csr(mgb_sim* s, int64_t* row_ptr, int32_t* col, int32_t* del
         MGB_HIP_TRY(hipMemcpy(weight, s->w32, 4 * S, hipMemcpyDeviceToHost));
         if (s->wide_kind == kWideBF16) {
             std::vector<uint16_t> w16(S);
+            std::vector<float> rs((size_t)s->n);
+            std::vector<int64_t> rp((size_t)s->n + 1);
             MGB_HIP_TRY(hipMemcpy(w16.data(), s->w16, 2 * S, hipMemcpyDeviceToHost));
-            for (size_t k = 0; k < S; ++k)
-                if (!(dl[k] & 128u)) {
+            MGB_HIP_TRY(hipMemcpy(rs.data(), s->row_scale, 4 * (size_t)s->n, hipMemcpyDeviceToHost));
+            MGB_HIP_TRY(hipMemcpy(rp.data(), s->row_ptr, 8 * ((size_t)s->n + 1), hipMemcpyDeviceToHost));
+            for (int32_t i = 0; i < s->n; ++i)
+                for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+                    if (dl[k] & 128u) continue;
                     const uint32_t b = (uint32_t)w16[k] << 16;
-                    std::memcpy(&weight[k], &b, 4);
+                    float v;
+                    std::memcpy(&v, &b, 4);
+                    weight[k] = v * rs[i];                  // as the kernels decode it
                 }
         } else if (s->wide_kind == kWideFP8) {
             std::vector<uint8_t> w8(S);

@@ -89,8 +89,8 @@ hipError_t tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col
                             uint32_t* syn_t, float* w_t, hipStream_t st);
 hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, const TileTabs& tt,
                         uint64_t total, uint16_t* offs16, uint32_t* offs32, hipStream_t st);
-hipError_t fp8_row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n,
-                          uint32_t* rmax_tmp, float* scale, hipStream_t st);
+hipError_t row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n, float top,
+                      uint32_t* rmax_tmp, float* scale, hipStream_t st);
 hipError_t make_in_keys(const int32_t* tar, const uint8_t* plastic, int32_t n, uint32_t* keys,
                         uint32_t* vals, uint64_t S, hipStream_t st);
 hipError_t sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* kin, uint32_t* kout,

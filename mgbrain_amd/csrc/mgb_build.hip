@@ -240,7 +240,7 @@ __global__ void k_gather_wide(GatherArgs a) {
         a.dly[k] = (uint8_t)(a.dq[i] | (pl ? 0x80u : 0u));
         const float w = a.weight[i];
         a.w32[k] = w;
-        if (a.w16) a.w16[k] = f32_to_bf16_rne(w);
+        if (a.w16) a.w16[k] = f32_to_bf16_rne(w / a.row_scale[a.src[i]]);
         if (a.w8) a.w8[k] = f32_to_fp8e4m3(w / a.row_scale[a.src[i]]);
         if (a.pos_of) a.pos_of[i] = (uint32_t)k;
     }
@@ -252,12 +252,13 @@ __global__ void k_row_absmax(const int32_t* src, const float* w, uint64_t S, uin
         atomicMax(&rmax[src[i]], __float_as_uint(fabsf(w[i])));
 }
 
-// FP8 scale of each source row: its largest |w| maps onto E4M3's largest normal.
-__global__ void k_row_scale(const uint32_t* rmax, int32_t n, float* scale) {
+// Scale of each source row: its largest |w| maps onto `top` (E4M3's largest normal
+// 448 for FP8, 1 for BF16), so a row of equal weights is stored exactly.
+__global__ void k_row_scale(const uint32_t* rmax, int32_t n, float top, float* scale) {
     const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
     const float m = __uint_as_float(rmax[i]);
-    scale[i] = m > 0.f ? m / 448.f : 1.f;
+    scale[i] = m > 0.f ? m / top : 1.f;
 }
 
 __global__ void k_in_keys(const int32_t* tar, const uint8_t* plastic, int32_t n, uint32_t* keys,
@@ -406,12 +407,12 @@ hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, con
     return hipGetLastError();
 }
 
-hipError_t fp8_row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n,
-                          uint32_t* rmax_tmp, float* scale, hipStream_t st) {
+hipError_t row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n, float top,
+                      uint32_t* rmax_tmp, float* scale, hipStream_t st) {
     hipError_t e = hipMemsetAsync(rmax_tmp, 0, 4 * (size_t)n, st);
     if (e != hipSuccess) return e;
     if (S) hipLaunchKernelGGL(k_row_absmax, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, w, S, rmax_tmp);
-    hipLaunchKernelGGL(k_row_scale, dim3((n + 255) / 256), dim3(256), 0, st, rmax_tmp, n, scale);
+    hipLaunchKernelGGL(k_row_scale, dim3((n + 255) / 256), dim3(256), 0, st, rmax_tmp, n, top, scale);
     return hipGetLastError();
 }
 

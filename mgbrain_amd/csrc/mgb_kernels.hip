@@ -305,7 +305,7 @@ __global__ void k_prop(PropArgs a, int32_t off, int32_t wk) {
                           (x >> 31) ? 0x10000u : 1u);
             }
         } else {
-            const float rs = (WK == kWideFP8) ? a.row_scale[i] : 1.0f;
+            const float rs = (WK == kWideF32) ? 1.0f : a.row_scale[i];
             for (int64_t k = beg + lane; k < end; k += stride) {
                 const int32_t tar = a.col[k];
                 const uint32_t dl = a.dly[k];
@@ -313,7 +313,7 @@ __global__ void k_prop(PropArgs a, int32_t off, int32_t wk) {
                 slot -= (slot >= D) ? D : 0;
                 float w;
                 if (WK == kWideF32 || (dl & 128u)) w = a.w32[k];     // plastic weights stay FP32
-                else if (WK == kWideBF16) w = bf16_to_f32(a.w16[k]);
+                else if (WK == kWideBF16) w = bf16_to_f32(a.w16[k]) * rs;
                 else w = fp8e4m3_to_f32(a.w8[k]) * rs;
                 const size_t loc = (size_t)slot * (size_t)a.npad + (size_t)tar;
                 if (w > 0.f) atomicAdd(&a.ring_e[loc], (int32_t)rintf(w * a.scale_e));
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     uint32_t* sp_len = sp_beg + kTiledBatch;                // len | slot-of-spike << 26
     uint32_t* sp_start = sp_len + kTiledBatch;              // exclusive prefix of work items
     uint16_t* itm = reinterpret_cast<uint16_t*>(sp_start + kTiledBatch);   // [kTiledItems] spike of item
-    float* sp_rs = reinterpret_cast<float*>(itm + kTiledItems);            // [kTiledBatch] FP8 row scale
+    float* sp_rs = reinterpret_cast<float*>(itm + kTiledItems);            // [kTiledBatch] BF16/FP8 row scale
     Ctl* ctl = a.ctl;
     const int64_t s0 = ctl->base_step + off;
     const int64_t kwin = ctl->base_window + wk;
@@ -471,6 +471,27 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const int32_t nprev = (int32_t)(s0 - sprev);           // 0 before the first window
     const int32_t rprev = (int32_t)(sprev % D);
     uint32_t* gring = a.ring_t + (size_t)g * (size_t)RW * (size_t)D * (size_t)B;
+    const uint32_t E = tcnt[bprev * nseg];
+    const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride + lbase;
+    // metadata of list entry e: first slot of its sub-row in this tile, length |
+    // ring slot of its spike step << 26, BF16 / FP8 row scale
+    auto meta = [&](uint32_t e, uint32_t& beg, uint32_t& lm, float& rsc) {
+        const uint32_t ent = L[e];
+        const uint32_t src = ent & kIdMask;
+        const size_t ob = obase + (size_t)(src - (uint32_t)slo) * G1;
+        uint32_t o0, o1;
+        if (a.offs16) { o0 = a.offs16[ob]; o1 = a.offs16[ob + 1]; }
+        else { o0 = a.offs32[ob]; o1 = a.offs32[ob + 1]; }
+        const int32_t sD = (int32_t)((sprev + (int64_t)(ent >> 26)) % D);
+        beg = (uint32_t)a.row_ptr[src] + o0;
+        lm = (o1 - o0) | ((uint32_t)sD << 26);
+        rsc = (WK == kTFP8 || WK == kTBF16) ? a.row_scale[src] : 1.0f;
+    };
+    // the first batch's metadata loads are issued before the ring copy, so that
+    // their latency overlaps it
+    uint32_t pbeg = 0u, plm = 0u;
+    float prs = 1.0f;
+    if ((uint32_t)tid < min((uint32_t)bd, E)) meta((uint32_t)tid, pbeg, plm, prs);
 #pragma unroll
     for (int hh = 0; hh < RW; ++hh) {
         // rows [rprev, rprev + nprev) mod D are zero, the rest are loaded
@@ -491,24 +512,17 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     __syncthreads();
 
     // 2. previous window's spikes into the tile
-    const uint32_t E = tcnt[bprev * nseg];
-    const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride + lbase;
     for (uint32_t b0 = 0; b0 < E; b0 += (uint32_t)bd) {
         const uint32_t nb = min((uint32_t)bd, E - b0);
         uint32_t items = 0u;
         if ((uint32_t)tid < nb) {
-            const uint32_t ent = L[b0 + tid];
-            const uint32_t src = ent & kIdMask;
-            const size_t ob = obase + (size_t)(src - (uint32_t)slo) * G1;
-            uint32_t o0, o1;
-            if (a.offs16) { o0 = a.offs16[ob]; o1 = a.offs16[ob + 1]; }
-            else { o0 = a.offs32[ob]; o1 = a.offs32[ob + 1]; }
-            const uint32_t len = o1 - o0;
-            const int32_t sD = (int32_t)((sprev + (int64_t)(ent >> 26)) % D);
-            sp_beg[tid] = (uint32_t)a.row_ptr[src] + o0;
-            sp_len[tid] = len | ((uint32_t)sD << 26);
-            if (WK == kTFP8) sp_rs[tid] = a.row_scale[src];
-            items = (len + CH - 1) / CH;
+            uint32_t beg = pbeg, lm = plm;
+            float rsc = prs;
+            if (b0 > 0) meta(b0 + (uint32_t)tid, beg, lm, rsc);
+            sp_beg[tid] = beg;
+            sp_len[tid] = lm;
+            if (WK == kTFP8 || WK == kTBF16) sp_rs[tid] = rsc;
+            items = ((lm & kIdMask) + CH - 1) / CH;
         }
         const uint32_t incl = block_incl_scan(items, wsum);
         const uint32_t total = wsum[(blockDim.x >> 6) - 1];
@@ -542,7 +556,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                     sD[h] = (int32_t)(meta >> 26);
                     kb[h] = sp_beg[j] + c * CH;
                     cnt[h] = (q0 < rn) ? min((uint32_t)CH, len - c * CH) : 0u;
-                    rs[h] = (WK == kTFP8) ? sp_rs[j] : 1.0f;
+                    rs[h] = (WK == kTFP8 || WK == kTBF16) ? sp_rs[j] : 1.0f;
                 }
                 uint32_t x[U][CH];
                 float wv[U][CH];
@@ -582,7 +596,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             slot -= (slot >= D) ? D : 0;
                             const int32_t loc = (int32_t)(xw & 0x7FFu);
                             float w;
-                            if (WK == kTBF16) w = bf16_to_f32((uint16_t)(xw >> 16));
+                            if (WK == kTBF16) w = bf16_to_f32((uint16_t)(xw >> 16)) * rs[h];
                             else if (WK == kTFP8) w = fp8e4m3_to_f32((uint8_t)(xw >> 16)) * rs[h];
                             else w = wv[h][u];
                             const bool exc = w > 0.f;
@@ -781,7 +795,8 @@ hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t
 size_t tiled_lds_bytes(int D, int B, int kind) {
     const size_t rw = (kind == kTClass || kind == kTClass16) ? 1 : 2;
     return sizeof(uint32_t) * (rw * (size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
-           sizeof(uint16_t) * (size_t)kTiledItems + (kind == kTFP8 ? sizeof(float) * kTiledBatch : 0);
+           sizeof(uint16_t) * (size_t)kTiledItems +
+           ((kind == kTFP8 || kind == kTBF16) ? sizeof(float) * kTiledBatch : 0);
 }
 
 namespace {

@@ -17,17 +17,43 @@ DT = 1e-4
 f32 = np.float32
 
 
+def gen_life_const(v_init, v_rest, v_reset, cm, tau_m, tau_refrac, tau_syn_E, tau_syn_I, v_thresh, i_offset,
+                   dt=1e-4):
+    """ConstGen::gen_life_const (constgen.h:9-107) in float32 (expf via numpy float32)."""
+    dt = f32(dt)
+    v_init, v_rest, v_reset, cm, tau_m, tau_refrac = map(f32, (v_init, v_rest, v_reset, cm, tau_m, tau_refrac))
+    tau_syn_E, tau_syn_I, v_thresh, i_offset = map(f32, (tau_syn_E, tau_syn_I, v_thresh, i_offset))
+    rm = f32(1.0)
+    if abs(float(cm)) > 0.00001:
+        rm = f32(tau_m / cm)
+    Cm = np.exp(f32(-dt / tau_m), dtype=f32) if tau_m > 0 else f32(0.0)
+    CE = np.exp(f32(-dt / tau_syn_E), dtype=f32) if tau_syn_E > 0 else f32(0.0)
+    CI = np.exp(f32(-dt / tau_syn_I), dtype=f32) if tau_syn_I > 0 else f32(0.0)
+    io = f32(0.0)  # the reference reads an uninitialised _i_offset; all callers pass 0
+    v_tmp = f32(io * rm + v_rest)
+    v_tmp = f32(v_tmp * f32(f32(1) - Cm))
+    C_E = f32(f32(rm * tau_syn_E) / f32(tau_syn_E - tau_m))
+    C_I = f32(f32(rm * tau_syn_I) / f32(tau_syn_I - tau_m))
+    C_E = f32(C_E * f32(CE - Cm))
+    C_I = f32(C_I * f32(CI - Cm))
+    c = np.zeros(30, dtype=f32)
+    c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8] = (dt, v_init, v_rest, v_reset, v_thresh, cm, tau_m,
+                                                            i_offset, tau_refrac)
+    c[10], c[11], c[12], c[13], c[14], c[15], c[16], c[17] = tau_syn_E, tau_syn_I, C_E, C_I, v_tmp, CE, CI, Cm
+    c[21], c[22], c[23], c[24], c[25] = (f32(0.951229453), f32(0.967216074), f32(0.0799999982),
+                                         f32(0.0599999987), f32(0.90483743))
+    return c
+
+
+def life_const1(dt=1e-4):
+    """ConstGen::gen_life_const1 (constgen.h:135-138): the Vogels-Abbott constants."""
+    return gen_life_const(-0.06, -0.06, -0.06, 0, 0, 0.005, 1, 1, -0.05, 0, dt=dt)
+
+
 def lifb_consts(v_thresh=0.020):
     """gen_life_const(v_init 0, v_rest 0, v_reset 10 mV, cm 0, tau_m 0, t_ref 2 ms, 1, 1,
-    V_th, 0) at dt = 0.1 ms (constgen.h:9-107); LIFB reads DT, V_INIT, V_RESET,
-    V_THRESH and TAU_REFRAC only."""
-    c = np.zeros(30, dtype=f32)
-    c[0], c[3], c[4], c[8] = DT, 0.010, v_thresh, 0.002
-    c[10], c[11] = 1.0, 1.0
-    e = np.exp(f32(-DT), dtype=f32)
-    c[12], c[13], c[15], c[16] = e, e, e, e
-    c[21], c[22], c[23], c[24], c[25] = 0.951229453, 0.967216074, 0.0799999982, 0.0599999987, 0.90483743
-    return c
+    V_th, 0) at dt = 0.1 ms; LIFB reads DT, V_INIT, V_RESET, V_THRESH and TAU_REFRAC only."""
+    return gen_life_const(0.0, 0.0, 0.010, 0.0, 0.0, 0.002, 1.0, 1.0, v_thresh, 0.0, dt=DT)
 
 
 def brunel_fixed_k(scale=1.0, K=10000, rate=0.0015, seed=0, plastic_ee=False):
@@ -63,6 +89,27 @@ def brunel_bernoulli(scale=1.0, p=0.1, rate=0.002, seed=0):
     desc = (f"Brunel Bernoulli p={p}: {nE} E + {nI} I LIFB + {nP} Poisson @ {rate / DT:.0f} Hz, "
             f"J=0.1 mV, g=5, delay 1.4 ms, dt=0.1 ms")
     return pops, projs, desc, nP + nE + nI
+
+
+def vogels_abbott(scale=1.0, p=0.02, seed=0, g_exc=0.04, g_inh=0.51):
+    """64k E + 16k I conductance-based LIFE neurons (x scale), Bernoulli p, delay 0.8 ms,
+    V0 uniform on [-60, -50] mV.  g_exc / g_inh are the conductance increments in the LIFE
+    update's units, multiples of the leak conductance (the update advances V by
+    dt/tau_m * (E_L - V + g_e (E_e - V) + g_i (E_i - V)), mgcore.cu:197-233): 0.4 nS and
+    5.1 nS over g_L = 10 nS (C_m 200 pF, tau_m 20 ms) give 0.04 / 0.51.  The reference's
+    make_vogel uses 0.4 / 5.1 scaled by 16e6 / N^2 (maker.h:34-35); unscaled at 80k neurons
+    that drive saturates every neuron near its refractory limit (~190 Hz)."""
+    nE, nI = int(64000 * scale), int(16000 * scale)
+    vlo, vhi = float(f32(-0.060)), float(f32(-0.050))
+    d = float(f32(0.0008))
+    pops = [dict(num=nE, type=N.LIFE, v_min=vlo, v_max=vhi), dict(num=nI, type=N.LIFE, v_min=vlo, v_max=vhi)]
+    we, wi = float(f32(g_exc)), float(f32(-g_inh))
+    projs = [dict(src=s, tar=t, kind=N.CONN_BERNOULLI, p=p, w=w, d=d, index=k, seed=seed)
+             for k, (s, t, w) in enumerate(((0, 0, we), (0, 1, we), (1, 0, wi), (1, 1, wi)))]
+    desc = (f"Vogels-Abbott COBA: {nE} E + {nI} I LIFE, Bernoulli p={p}, g_exc {g_exc}, g_inh {g_inh} "
+            f"(x g_leak), delay 0.8 ms, "
+            f"V0 ~ U[-60, -50] mV, dt=0.1 ms")
+    return pops, projs, desc, nE + nI
 
 
 def stdp_consts(J=1e-4):

@@ -16,30 +16,7 @@ BERN, FIXK = 1, 2
 f32 = np.float32
 
 
-def gen_life_const(v_init, v_rest, v_reset, cm, tau_m, tau_refrac, tau_syn_E, tau_syn_I, v_thresh, i_offset,
-                   dt=1e-4):
-    """constgen.h:9-107 in float32 (expf via numpy float32)."""
-    dt = f32(dt)
-    v_init, v_rest, v_reset, cm, tau_m, tau_refrac = map(f32, (v_init, v_rest, v_reset, cm, tau_m, tau_refrac))
-    tau_syn_E, tau_syn_I, v_thresh, i_offset = map(f32, (tau_syn_E, tau_syn_I, v_thresh, i_offset))
-    rm = f32(1.0)
-    if abs(float(cm)) > 0.00001:
-        rm = f32(tau_m / cm)
-    Cm = np.exp(f32(-dt / tau_m), dtype=f32) if tau_m > 0 else f32(0.0)
-    CE = np.exp(f32(-dt / tau_syn_E), dtype=f32) if tau_syn_E > 0 else f32(0.0)
-    CI = np.exp(f32(-dt / tau_syn_I), dtype=f32) if tau_syn_I > 0 else f32(0.0)
-    io = f32(0.0)  # the reference reads an uninitialised _i_offset; all callers pass 0
-    v_tmp = f32(io * rm + v_rest)
-    v_tmp = f32(v_tmp * f32(f32(1) - Cm))
-    C_E = f32(f32(rm * tau_syn_E) / f32(tau_syn_E - tau_m))
-    C_I = f32(f32(rm * tau_syn_I) / f32(tau_syn_I - tau_m))
-    C_E = f32(C_E * f32(CE - Cm))
-    C_I = f32(C_I * f32(CI - Cm))
-    c = np.zeros(30, dtype=f32)
-    c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8] = dt, v_init, v_rest, v_reset, v_thresh, cm, tau_m, i_offset, tau_refrac
-    c[10], c[11], c[12], c[13], c[14], c[15], c[16], c[17] = tau_syn_E, tau_syn_I, C_E, C_I, v_tmp, CE, CI, Cm
-    c[21], c[22], c[23], c[24], c[25] = f32(0.951229453), f32(0.967216074), f32(0.0799999982), f32(0.0599999987), f32(0.90483743)
-    return c
+from mgbrain_amd.workloads import gen_life_const  # noqa: E402  (shared with bench.py)
 
 
 def life_const1(dt=1e-4):
@@ -160,13 +137,13 @@ def config2(scale=1.0, K=10000, poisson_rate=0.0015, stdp=False):
     return net
 
 
-def config3(scale=1.0, p=0.02):
-    """Vogels-Abbott COBA: 64k E + 16k I LIFE, Bernoulli p = 0.02, g_exc 0.4, g_inh 5.1 (reference
-    dimensionless units, maker.h:34-35), delay 0.8 ms, V0 ~ U[-60, -50] mV."""
+def config3(scale=1.0, p=0.02, g_exc=0.04, g_inh=0.51):
+    """Vogels-Abbott COBA: 64k E + 16k I LIFE, Bernoulli p = 0.02, delay 0.8 ms, V0 ~ U[-60, -50] mV;
+    g_exc 0.4 nS / g_inh 5.1 nS over g_L = 10 nS (see mgbrain_amd.workloads.vogels_abbott)."""
     nE, nI = int(64000 * scale), int(16000 * scale)
     d = float(f32(0.0008))
     pops = [(nE, LIFE, 0), (nI, LIFE, 0)]
-    we, wi = float(f32(0.4)), float(f32(-5.1))
+    we, wi = float(f32(g_exc)), float(f32(-g_inh))
     pr = [dict(src=s, tar=t, w=w, d=d, kind=BERN, p=p) for s, t, w in ((0, 0, we), (0, 1, we), (1, 0, wi), (1, 1, wi))]
     net = Net(pops, pr, life_const1(), name=f"config3_x{scale}")
     net.vrange = [(float(f32(-0.060)), float(f32(-0.050)))] * 2

@@ -123,7 +123,7 @@ def test_generated_fixed_indegree_delays():
 
 def test_vogels_abbott_life():
     _say("case: config3 x0.05 LIFE (COBA) with V0 ~ U[-60,-50] mV")
-    net = M.config3(scale=0.05, p=0.1)
+    net = M.config3(scale=0.05, p=0.1, g_exc=0.4, g_inh=5.1)   # make_vogel's units at 4k neurons: active
     syn = H.host_synapses(net, seed=2)
     _compare_run(net, syn, 500, seed=2)
 

@@ -62,10 +62,15 @@ def test_ee_stdp_benchmark_constants(wf):
     assert 0 < syn.plastic.sum() < syn.n
     sc = M.stdp_consts_config4(J=1e-4)
     if wf == N.WF_BF16:
-        # static weights are stored as BF16 on the device; give the oracle the same values
+        # static weights are stored as BF16 of w / s (s = largest |w| of the source row,
+        # plastic synapses included) and read as bf16 * s; give the oracle the same values
         stat = syn.plastic == 0
-        u = syn.weight.view(np.uint32).astype(np.uint64)
+        smax = np.zeros(net.n, np.float32)
+        np.maximum.at(smax, syn.src, np.abs(syn.weight))
+        sc_row = np.where(smax > 0, smax, np.float32(1.0)).astype(np.float32)[syn.src]
+        x = (syn.weight / sc_row).astype(np.float32)
+        u = x.view(np.uint32).astype(np.uint64)
         r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
-        syn.weight[stat] = r[stat]
+        syn.weight[stat] = (r * sc_row).astype(np.float32)[stat]
     changed, n = _run(net, syn, sc, 800, 6, wf)
     assert changed > 100 and n > 1000
