@@ -492,6 +492,18 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     uint32_t pbeg = 0u, plm = 0u;
     float prs = 1.0f;
     if ((uint32_t)tid < min((uint32_t)bd, E)) meta((uint32_t)tid, pbeg, plm, prs);
+    // copies [k0, k1) of a slice to LDS, 8 loads per thread in flight (clamped
+    // indices keep every load in bounds; the extra values are not stored)
+    auto copy_in = [&](uint32_t* rr, const uint32_t* gr, int32_t k0, int32_t k1) {
+        for (int32_t kb = k0 + tid; kb < k1; kb += 8 * bd) {
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = gr[min(kb + u * bd, k1 - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (kb + u * bd < k1) rr[kb + u * bd] = v[u];
+        }
+    };
 #pragma unroll
     for (int hh = 0; hh < RW; ++hh) {
         // rows [rprev, rprev + nprev) mod D are zero, the rest are loaded
@@ -500,12 +512,12 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         const int32_t e = rprev + nprev;
         const int32_t z0 = (e <= D) ? rprev : 0, z1 = (e <= D) ? e : e - D;   // zero rows [z0, z1)
         if (e <= D) {
-            for (int32_t k = tid; k < z0 * B; k += bd) rr[k] = gr[k];
+            copy_in(rr, gr, 0, z0 * B);
             for (int32_t k = z0 * B + tid; k < z1 * B; k += bd) rr[k] = 0u;
-            for (int32_t k = z1 * B + tid; k < D * B; k += bd) rr[k] = gr[k];
+            copy_in(rr, gr, z1 * B, D * B);
         } else {
             for (int32_t k = tid; k < z1 * B; k += bd) rr[k] = 0u;
-            for (int32_t k = z1 * B + tid; k < rprev * B; k += bd) rr[k] = gr[k];
+            copy_in(rr, gr, z1 * B, rprev * B);
             for (int32_t k = rprev * B + tid; k < D * B; k += bd) rr[k] = 0u;
         }
     }
@@ -638,17 +650,23 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         for (int32_t w = 0; w < nsteps; ++w) {
             const int32_t s = (int32_t)(s0 + w);
             bool fired = false;
+            // class sums of a cached class: one LDS read per side (the table for
+            // counts past the cache)
+            auto cached_input = [&](uint32_t x, float& be, float& bi) {
+                const uint32_t ke = x & 0xFFFFu, ki = x >> 16;
+                const float* cc = &ccache[csl][0][0];
+                be = cc[min(ke, (uint32_t)kClassCache - 1u)];
+                bi = cc[kClassCache + min(ki, (uint32_t)kClassCache - 1u)];
+                if (ke >= (uint32_t)kClassCache) be = a.seq[st.off_e + ke];
+                if (ki >= (uint32_t)kClassCache) bi = a.seq[st.off_i + ki];
+            };
             if (lean) {
                 if (active) {
                     uint32_t* r = &ring[slot * B + j];
                     const uint32_t x = *r;
                     *r = 0u;                                          // clearInput
-                    const uint32_t ke = x & 0xFFFFu, ki = x >> 16;
-                    const float* cc = &ccache[csl][0][0];
-                    float be = cc[min(ke, (uint32_t)kClassCache - 1u)];
-                    float bi = cc[kClassCache + min(ki, (uint32_t)kClassCache - 1u)];
-                    if (ke >= (uint32_t)kClassCache) be = a.seq[st.off_e + ke];
-                    if (ki >= (uint32_t)kClassCache) bi = a.seq[st.off_i + ki];
+                    float be, bi;
+                    cached_input(x, be, bi);
                     fired = integrate_step(T_LIFB, st, s, be, bi);
                 }
             } else if (type == T_POISSON) {
@@ -659,7 +677,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                 if (kCls) {
                     const uint32_t x = *r;
                     *r = 0u;                                          // clearInput
-                    class_input(x, a.seq, st, be, bi);
+                    if (csl >= 0) cached_input(x, be, bi);
+                    else class_input(x, a.seq, st, be, bi);
                 } else {
                     uint32_t* ri = r + (size_t)D * B;
                     const int32_t qe = (int32_t)*r, qi = (int32_t)*ri;
