@@ -246,6 +246,16 @@ mgb_status mgb_read_csr(mgb_sim* sim, int64_t* row_ptr, int32_t* col, int32_t* d
 /* Current weights in CSR order (changed by STDP). */
 mgb_status mgb_read_weights(mgb_sim* sim, float* out);
 
+/* Checkpoint of everything a run changes: neuron state, ring buffers, the
+ * pending spike lists and their counters, step counters, plastic weights and
+ * the recorded spikes (the reference keeps none; its stale network writer is
+ * test/networkio.cpp:4-24).  save writes mgb_checkpoint_size() bytes; load
+ * restores them into a simulator built from the same network and options,
+ * after which runs continue bit-identically. */
+mgb_status mgb_checkpoint_size(mgb_sim* sim, int64_t* bytes);
+mgb_status mgb_checkpoint_save(mgb_sim* sim, void* buf, int64_t bytes);
+mgb_status mgb_checkpoint_load(mgb_sim* sim, const void* buf, int64_t bytes);
+
 /* Frees everything the handle owns.  Replaces ~MGSimulator (mgsim.cpp:364-406)
  * and free_gsubnet(_gpu) (mgnet.h:29-30), without cudaDeviceReset. */
 mgb_status mgb_destroy(mgb_sim* sim);

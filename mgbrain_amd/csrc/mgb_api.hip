@@ -1587,6 +1587,143 @@ mgb_status mgb_read_state(mgb_sim* s, int32_t field, void* out) {
     return MGB_OK;
 }
 
+// ---- checkpoint -------------------------------------------------------------
+
+namespace {
+
+constexpr uint64_t kCkptMagic = 0x3154504B4342474Dull;   // "MGBCKPT1"
+
+// Layout fields a checkpoint must match (the simulator it is loaded into must
+// come from the same network and options).
+struct CkptHeader {
+    uint64_t magic;
+    int32_t abi, n, npad, D, window, tiled, tkind, B, G, nseg, mode, wide_kind, stdp;
+    int64_t S, n_rec;
+    uint64_t payload;
+};
+
+struct CkptSeg {
+    void* p;
+    size_t bytes;
+};
+
+// Every device array a run changes, in a fixed order.
+std::vector<CkptSeg> ckpt_segments(const mgb_sim* s, uint64_t n_rec) {
+    const size_t np = (size_t)s->npad;
+    std::vector<CkptSeg> v = {{s->V, 4 * np}, {s->Ie, 4 * np}, {s->Ii, 4 * np},
+                              {s->refr, 4 * np}, {s->last, 4 * np}, {s->cnt, 4 * np},
+                              {s->ctl, sizeof(Ctl)},
+                              {s->win_buf, 4 * (size_t)(s->tiled ? 3 : 2) * (size_t)s->win_stride}};
+    if (s->tiled) {
+        const size_t rw = (s->tkind == kTClass || s->tkind == kTClass16) ? 1 : 2;
+        v.push_back({s->ring_t, 4 * (size_t)s->G * (size_t)s->D * (size_t)s->B * rw});
+        v.push_back({s->tcnt, 12 * (size_t)s->nseg});
+    } else if (s->mode == kRingClass) {
+        v.push_back({s->ring, 4 * (size_t)s->D * np});
+    } else {
+        v.push_back({s->ring_e, 4 * (size_t)s->D * np});
+        v.push_back({s->ring_i, 4 * (size_t)s->D * np});
+    }
+    if (s->stdp && s->w32) v.push_back({s->w32, 4 * (size_t)s->S});
+    if (n_rec) v.push_back({s->rec, 8 * (size_t)n_rec});
+    return v;
+}
+
+CkptHeader ckpt_header(const mgb_sim* s, uint64_t n_rec) {
+    CkptHeader h{};
+    h.magic = kCkptMagic;
+    h.abi = MGB_ABI_VERSION;
+    h.n = s->n; h.npad = s->npad; h.D = s->D; h.window = s->window; h.tiled = s->tiled;
+    h.tkind = s->tkind; h.B = s->B; h.G = s->G; h.nseg = s->nseg; h.mode = s->mode;
+    h.wide_kind = s->wide_kind; h.stdp = s->stdp; h.S = s->S; h.n_rec = (int64_t)n_rec;
+    for (const CkptSeg& g : ckpt_segments(s, n_rec)) h.payload += g.bytes;
+    return h;
+}
+
+mgb_status ckpt_records(mgb_sim* s, uint64_t* n_rec) {
+    Ctl c;
+    MGB_HIP_TRY(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    *n_rec = std::min<uint64_t>(c.rec_cnt, (uint64_t)s->rec_cap);
+    return MGB_OK;
+}
+
+}  // namespace
+
+mgb_status mgb_checkpoint_size(mgb_sim* s, int64_t* bytes) {
+    if (!s || !bytes) {
+        set_error("null argument");
+        return MGB_ERR_INVALID;
+    }
+    MGB_TRY(mgb_sync(s));
+    uint64_t n_rec = 0;
+    MGB_TRY(ckpt_records(s, &n_rec));
+    *bytes = (int64_t)(sizeof(CkptHeader) + ckpt_header(s, n_rec).payload);
+    return MGB_OK;
+}
+
+mgb_status mgb_checkpoint_save(mgb_sim* s, void* buf, int64_t bytes) {
+    if (!s || !buf) {
+        set_error("null argument");
+        return MGB_ERR_INVALID;
+    }
+    MGB_TRY(mgb_sync(s));
+    uint64_t n_rec = 0;
+    MGB_TRY(ckpt_records(s, &n_rec));
+    const CkptHeader h = ckpt_header(s, n_rec);
+    if (bytes < (int64_t)(sizeof(h) + h.payload)) {
+        set_error("checkpoint buffer of %lld bytes, %llu needed", (long long)bytes,
+                  (unsigned long long)(sizeof(h) + h.payload));
+        return MGB_ERR_CAPACITY;
+    }
+    uint8_t* out = static_cast<uint8_t*>(buf);
+    std::memcpy(out, &h, sizeof(h));
+    size_t off = sizeof(h);
+    for (const CkptSeg& g : ckpt_segments(s, n_rec)) {
+        if (g.bytes) MGB_HIP_TRY(hipMemcpy(out + off, g.p, g.bytes, hipMemcpyDeviceToHost));
+        off += g.bytes;
+    }
+    return MGB_OK;
+}
+
+mgb_status mgb_checkpoint_load(mgb_sim* s, const void* buf, int64_t bytes) {
+    if (!s || !buf || bytes < (int64_t)sizeof(CkptHeader)) {
+        set_error("invalid checkpoint arguments");
+        return MGB_ERR_INVALID;
+    }
+    CkptHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (h.magic != kCkptMagic || h.abi != MGB_ABI_VERSION) {
+        set_error("not a checkpoint of this library version");
+        return MGB_ERR_INVALID;
+    }
+    if ((uint64_t)h.n_rec > (uint64_t)s->rec_cap) {
+        set_error("checkpoint holds %lld recorded spikes, record capacity is %lld", (long long)h.n_rec,
+                  (long long)s->rec_cap);
+        return MGB_ERR_CAPACITY;
+    }
+    const CkptHeader want = ckpt_header(s, (uint64_t)h.n_rec);
+    if (h.n != want.n || h.npad != want.npad || h.D != want.D || h.window != want.window ||
+        h.tiled != want.tiled || h.tkind != want.tkind || h.B != want.B || h.G != want.G ||
+        h.nseg != want.nseg || h.mode != want.mode || h.wide_kind != want.wide_kind ||
+        h.stdp != want.stdp || h.S != want.S || h.payload != want.payload) {
+        set_error("checkpoint layout differs from this simulator (build it from the same network and "
+                  "options)");
+        return MGB_ERR_INVALID;
+    }
+    if (bytes < (int64_t)(sizeof(h) + h.payload)) {
+        set_error("checkpoint truncated");
+        return MGB_ERR_INVALID;
+    }
+    MGB_TRY(mgb_sync(s));
+    const uint8_t* in = static_cast<const uint8_t*>(buf);
+    size_t off = sizeof(h);
+    for (const CkptSeg& g : ckpt_segments(s, (uint64_t)h.n_rec)) {
+        if (g.bytes) MGB_HIP_TRY(hipMemcpy(g.p, in + off, g.bytes, hipMemcpyHostToDevice));
+        off += g.bytes;
+    }
+    return MGB_OK;
+}
+
 mgb_status mgb_fire_counts(mgb_sim* s, int32_t* out) { return mgb_read_state(s, MGB_F_CNT, out); }
 
 mgb_status mgb_read_csr(mgb_sim* s, int64_t* row_ptr, int32_t* col, int32_t* delay, float* weight) {

@@ -25,7 +25,8 @@ F_V, F_IEXC, F_IINH, F_REFR, F_LAST, F_CNT = range(6)
 EXPORTS = ["mgb_abi_version", "mgb_last_error", "mgb_create", "mgb_create_generated",
            "mgb_create_batch_generated", "mgb_run",
            "mgb_sync", "mgb_set_launch", "mgb_get_info", "mgb_read_spikes", "mgb_clear_spikes",
-           "mgb_read_state", "mgb_fire_counts", "mgb_read_csr", "mgb_read_weights", "mgb_destroy"]
+           "mgb_read_state", "mgb_fire_counts", "mgb_read_csr", "mgb_read_weights", "mgb_checkpoint_size",
+           "mgb_checkpoint_save", "mgb_checkpoint_load", "mgb_destroy"]
 
 _p = C.c_void_p
 
@@ -110,6 +111,9 @@ def lib():
         L.mgb_fire_counts.argtypes = [_p, _p]
         L.mgb_read_csr.argtypes = [_p, _p, _p, _p, _p]
         L.mgb_read_weights.argtypes = [_p, _p]
+        L.mgb_checkpoint_size.argtypes = [_p, C.POINTER(C.c_int64)]
+        L.mgb_checkpoint_save.argtypes = [_p, _p, C.c_int64]
+        L.mgb_checkpoint_load.argtypes = [_p, _p, C.c_int64]
         L.mgb_destroy.argtypes = [_p]
         _lib = L
     return _lib
@@ -322,6 +326,28 @@ class Simulator:
         w = np.zeros(max(s, 1), dtype=np.float32)
         _check(lib().mgb_read_weights(self.h, _addr(w)))
         return w[:s]
+
+    # -- checkpoint -------------------------------------------------------------
+    def checkpoint(self):
+        """bytes of the simulator's mutable state (mgb_checkpoint_save)."""
+        n = C.c_int64()
+        _check(lib().mgb_checkpoint_size(self.h, C.byref(n)))
+        buf = np.zeros(max(1, n.value), dtype=np.uint8)
+        _check(lib().mgb_checkpoint_save(self.h, _addr(buf), n.value))
+        return buf[:n.value].tobytes()
+
+    def restore(self, blob):
+        """Loads a checkpoint taken from a simulator of the same network and options."""
+        buf = np.frombuffer(blob, dtype=np.uint8).copy()
+        _check(lib().mgb_checkpoint_load(self.h, _addr(buf), buf.size))
+
+    def save(self, path):
+        with open(path, "wb") as f:
+            f.write(self.checkpoint())
+
+    def load(self, path):
+        with open(path, "rb") as f:
+            self.restore(f.read())
 
     def close(self):
         if self.h:

@@ -16,6 +16,7 @@ import sys
 import numpy as np
 import pytest
 
+from mgbrain_amd import analysis as A
 from oracle import pyoracle as po
 from tests import harness as H
 from tests import netmodels as M
@@ -42,25 +43,9 @@ def test_brunel_rate_against_benchdata(nsyn, p, ref):
     assert abs(rate - ref) / ref < 0.02, (rate, ref)
 
 
-def _rates_cv(steps, ids, n, bounds, seconds):
-    """Per-population mean rate (Hz) and mean CV of the inter-spike intervals
-    (neurons with >= 3 spikes)."""
-    order = np.lexsort((steps, ids))
-    s, i = steps[order], ids[order]
-    out = []
-    for lo, hi in bounds:
-        m = (i >= lo) & (i < hi)
-        si, ii = s[m], i[m]
-        rate = len(si) / (hi - lo) / seconds
-        cvs = []
-        if len(si):
-            cut = np.flatnonzero(np.diff(ii)) + 1
-            for a, b in zip(np.r_[0, cut], np.r_[cut, len(si)]):
-                if b - a >= 3:
-                    isi = np.diff(si[a:b]).astype(np.float64)
-                    cvs.append(isi.std() / isi.mean())
-        out.append((rate, float(np.mean(cvs)) if cvs else float("nan"), len(cvs)))
-    return out
+def _rates_cv(steps, ids, bounds, seconds):
+    rates = A.population_rates(steps, ids, bounds, seconds)
+    return [(r, c, k) for r, (c, k) in zip(rates, A.cv_isi(steps, ids, bounds))]
 
 
 def test_population_rates_and_cv_isi_against_fp64():
@@ -80,7 +65,7 @@ def test_population_rates_and_cv_isi_against_fp64():
     oi = np.concatenate(want).astype(np.int32)
     first = net.first()
     bounds = [(first[k], first[k] + p[0]) for k, p in enumerate(net.pops)]
-    got, ref = _rates_cv(gs, gi, net.n, bounds, 1.0), _rates_cv(os_, oi, net.n, bounds, 1.0)
+    got, ref = _rates_cv(gs, gi, bounds, 1.0), _rates_cv(os_, oi, bounds, 1.0)
     for (r1, c1, n1), (r2, c2, n2), (num, typ, _src) in zip(got, ref, net.pops):
         print(f"  pop n={num} type={typ}: rate {r1:.3f} vs {r2:.3f} Hz, CV-ISI {c1:.4f} vs {c2:.4f}", flush=True)
         assert abs(r1 - r2) <= 0.02 * r2, (r1, r2)
