@@ -208,8 +208,7 @@ def stdp_leg(args, log):
     se = b["syn_events"] - a["syn_events"]
     spikes = b["spikes"] - a["spikes"]
     log(f"# stdp: {args.steps} steps in {wall:.2f} s, {se:.4g} events, {se / wall:.4g} SE/s, RTF "
-        f"{wall / (args.steps * DT):.3f}; neuron {b['neuron_ms']:.0f} ms, prop {b['prop_ms']:.0f} ms, "
-        f"stdp {b['stdp_ms']:.0f} ms")
+        f"{wall / (args.steps * DT):.3f}")
     rp, col, _dl, w = sim.csr()
     sim.close()
     nP, nE = pops[0]["num"], pops[1]["num"]
@@ -218,23 +217,47 @@ def stdp_leg(args, log):
     wee = ww[(c >= nP) & (c < nP + nE)]
     del rp, col, w, c, ww
     hist, edges = np.histogram(wee, bins=20, range=(0.0, float(sc[4])))
+    # per-kernel durations: a profiled pass (no graph replay) on a second simulator
     roof = None
-    if b["prop_launches"]:
-        # k_prop per event: target 4 B + delay 1 B + weight (2 B static BF16 / 4 B plastic
-        # FP32, plastic share of events taken as the plastic share of synapses); 12 B per spike
-        pl = info0.get("n_plastic", 0) or len(wee)
-        wbytes = 4.0 if wf == N.WF_F32 else 2.0 + 2.0 * pl / max(1, info0["n_synapses"])
-        alg = (5.0 + wbytes) * se + 12.0 * spikes
-        avg_s = b["prop_ms"] / 1e3 / b["prop_launches"]
-        ach = alg / b["prop_launches"] / avg_s / 1e9
-        roof = {"bound": "hbm", "kernel": "k_prop (wide format)", "achieved": round(ach, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "avg_launch_us": round(avg_s * 1e6, 3), "traffic": None}
+    if not args.no_profile:
+        prof = N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed, stdp=True, stdpconst=sc,
+                                     weight_format=wf, use_graph=0, profile=1)
+        prof.run(500)
+        pa = prof.info()
+        prof.run(1000)
+        pb = prof.info()
+        prof.close()
+        pse = pb["syn_events"] - pa["syn_events"]
+        psp = pb["spikes"] - pa["spikes"]
+        n_poisson = int(sum(p["num"] for p in pops if p["type"] == N.POISSON))
+        n_lif = ntot - n_poisson
+        plastic_share = len(wee) / max(1, info0["n_synapses"])
+        # reference step's per-unit bytes (9 B per event with FP32 weights; STDP: 8 B per plastic
+        # out-synapse of a spike, 16 B per plastic in-synapse, taken at the plastic share of
+        # the spike's out-degree for both)
+        k_bytes = {"k_neurons": 1000 * ((16.0 + 16.0) * n_lif + 4.0 * n_poisson) + 4.0 * psp,
+                   "k_prop": 12.0 * psp + 9.0 * pse,
+                   "k_stdp": (8.0 + 16.0) * plastic_share * pse}
+        k_ms = {"k_neurons": (pb["neuron_ms"], pb["neuron_launches"]), "k_prop": (pb["prop_ms"], pb["prop_launches"]),
+                "k_stdp": (pb["stdp_ms"], pb["stdp_launches"])}
+        kernels = {}
+        for k, (ms, nl) in k_ms.items():
+            if nl and ms > 0:
+                ach = k_bytes[k] / (ms / 1e3) / 1e9
+                kernels[k] = {"avg_launch_us": round(ms * 1e3 / nl, 3), "achieved": round(ach, 1),
+                              "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_launch": round(k_bytes[k] / nl)}
+        if kernels:
+            dom = max(kernels, key=lambda k: k_ms[k][0])
+            roof = dict(bound="hbm", achieved=kernels[dom]["achieved"], peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=kernels[dom]["frac"], traffic=None, kernel=dom, kernels=kernels,
+                        avg_launch_us=kernels[dom]["avg_launch_us"])
+            log(f"# stdp profile (1000 steps, no graph): " + ", ".join(
+                f"{k} {v['avg_launch_us']:.1f} us {v['achieved']:.0f} GB/s" for k, v in kernels.items()))
     return {
         "workload": "brunel100k_K10000_stdp_ee", "description": desc + "; STDP on E->E (A+ 0.01, A- 0.0105 "
         "x W_max, tau 20 ms, W in [0, 0.2 mV])", "value": se / wall, "unit": "syn_events/s",
         "rtf": wall / (args.steps * DT), "simulated_s": args.steps * DT, "static_weights": args.precision,
-        "plastic_weights": "f32", "neuron_ms": b["neuron_ms"], "prop_ms": b["prop_ms"], "stdp_ms": b["stdp_ms"],
+        "plastic_weights": "f32",
         "mean_rate_hz": spikes / ntot / (args.steps * DT), "setup_s": round(setup_s, 2),
         "ee_weights": {"n": int(wee.size), "mean": float(wee.mean()) if wee.size else 0.0,
                        "hist_edges": [float(x) for x in edges], "hist": [int(x) for x in hist]},

@@ -154,3 +154,17 @@ def stdp_consts_config4(J=1e-4):
     """A+ = 0.01, A- = 0.0105 in units of W_max = 2J; tau = 20 ms = 200 steps; W in [0, 2J]."""
     wmax = 2 * J
     return np.array([0.01 * wmax, -0.0105 * wmax, 200.0, 200.0, wmax, 0.0], dtype=f32)
+
+
+def lif_net(J=0.008, g=5.0, poisson_rate=0.02):
+    """Current-based LIF with exponential synaptic currents (LIFECONSTS from
+    gen_life_const with cm 0.25, tau_m 20 ms, tau_syn 5 ms; mgcore.cu:132-163):
+    400 Poisson + 1600 E + 400 I, Bernoulli p = 0.1, delay 1.2 ms (~22 Hz)."""
+    lc = gen_life_const(0.0, 0.0, 0.010, 0.25, 0.020, 0.002, 0.005, 0.005, 0.020, 0.0)
+    pops = [(400, POISSON, 1), (1600, LIF, 0), (400, LIF, 0)]
+    w, wi, d = float(f32(J)), float(f32(-g * J)), float(f32(0.0012))
+    pr = [dict(src=s, tar=t, w=ww, d=d, kind=BERN, p=0.1)
+          for s, t, ww in ((0, 1, w), (0, 2, w), (1, 1, w), (1, 2, w), (2, 1, wi), (2, 2, wi))]
+    net = Net(pops, pr, lc, name="lif")
+    net.rates = [poisson_rate, 0.0, 0.0]
+    return net

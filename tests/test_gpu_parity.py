@@ -128,6 +128,16 @@ def test_vogels_abbott_life():
     _compare_run(net, syn, 500, seed=2)
 
 
+@pytest.mark.parametrize("prop,wf", [(1, 1), (2, 1), (2, 2)])
+def test_lif_exponential_currents(prop, wf):
+    _say(f"case: current-based LIF network (exponential synaptic currents), prop={prop}, weight_format={wf}")
+    net = M.lif_net()
+    syn = H.host_synapses(net, seed=3)
+    n = _compare_run(net, syn, 1500, seed=3, gpu_kw=dict(propagation=prop, weight_format=wf),
+                     acc=po.ACC_FIXED if wf == 2 else po.ACC_F32)
+    assert n > 3000
+
+
 def test_graph_replay_identical():
     _say("case: config1 x0.2, hipGraph replay vs oracle")
     net = M.config1(scale=0.2, poisson_rate=0.2)
