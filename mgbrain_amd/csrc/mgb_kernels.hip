@@ -306,18 +306,32 @@ __global__ void k_prop(PropArgs a, int32_t off, int32_t wk) {
             }
         } else {
             const float rs = (WK == kWideF32) ? 1.0f : a.row_scale[i];
-            for (int64_t k = beg + lane; k < end; k += stride) {
-                const int32_t tar = a.col[k];
-                const uint32_t dl = a.dly[k];
-                int32_t slot = sD + (int32_t)(dl & 127u);
-                slot -= (slot >= D) ? D : 0;
-                float w;
-                if (WK == kWideF32 || (dl & 128u)) w = a.w32[k];     // plastic weights stay FP32
-                else if (WK == kWideBF16) w = bf16_to_f32(a.w16[k]) * rs;
-                else w = fp8e4m3_to_f32(a.w8[k]) * rs;
-                const size_t loc = (size_t)slot * (size_t)a.npad + (size_t)tar;
-                if (w > 0.f) atomicAdd(&a.ring_e[loc], (int32_t)rintf(w * a.scale_e));
-                else atomicAdd(&a.ring_i[loc], (int32_t)rintf(w * a.scale_i));
+            // four slots per lane in flight: clamped indices keep the loads in the
+            // row; slots past its end add nothing
+            for (int64_t k0 = beg + lane; k0 < end; k0 += 4 * stride) {
+                int32_t tar[4];
+                uint32_t dl[4];
+                float w32[4], wn[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t k = min(k0 + u * stride, end - 1);
+                    tar[u] = a.col[k];
+                    dl[u] = a.dly[k];
+                    w32[u] = a.w32[k];
+                    if (WK == kWideBF16) wn[u] = bf16_to_f32(a.w16[k]) * rs;
+                    else if (WK == kWideFP8) wn[u] = fp8e4m3_to_f32(a.w8[k]) * rs;
+                    else wn[u] = w32[u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (k0 + u * stride >= end) break;
+                    int32_t slot = sD + (int32_t)(dl[u] & 127u);
+                    slot -= (slot >= D) ? D : 0;
+                    const float w = (dl[u] & 128u) ? w32[u] : wn[u];   // plastic weights stay FP32
+                    const size_t loc = (size_t)slot * (size_t)a.npad + (size_t)tar[u];
+                    if (w > 0.f) atomicAdd(&a.ring_e[loc], (int32_t)rintf(w * a.scale_e));
+                    else atomicAdd(&a.ring_i[loc], (int32_t)rintf(w * a.scale_i));
+                }
             }
         }
     }
@@ -352,16 +366,36 @@ __global__ void k_stdp(StdpArgs a, int32_t off, int32_t wk) {
         const int32_t i = (int32_t)(wbuf[e] & kIdMask);
         const int32_t li = a.last[i];
         const int64_t beg = a.row_ptr[i] + 64 * (int64_t)f, end = a.row_ptr[i + 1];
-        for (int64_t k = beg + lane; k < end; k += stride) {         // axon loop :372-391
-            if (!(a.dly[k] & 128u)) continue;
-            const int32_t dt = li - a.last[a.col[k]];
-            a.w32[k] = clipw(a.w32[k] + stdp_lut(a, dt), a.w_max, a.w_min);
+        for (int64_t k0 = beg + lane; k0 < end; k0 += 4 * stride) {  // axon loop :372-391
+            int32_t lt[4];
+            float wv[4];
+            bool pl[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {                           // loads of four slots in flight
+                const int64_t k = min(k0 + u * stride, end - 1);
+                pl[u] = (k0 + u * stride < end) && (a.dly[k] & 128u);
+                lt[u] = a.last[a.col[k]];
+                wv[u] = a.w32[k];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (pl[u]) a.w32[k0 + u * stride] = clipw(wv[u] + stdp_lut(a, li - lt[u]), a.w_max, a.w_min);
         }
         const int64_t pb = a.in_ptr[i] + 64 * (int64_t)f, pe = a.in_ptr[i + 1];
-        for (int64_t p = pb + lane; p < pe; p += stride) {           // dendrite loop :392-411
-            const uint32_t k = a.in_pos[p];
-            const int32_t dt = a.last[a.in_src[p]] - li;
-            a.w32[k] = clipw(a.w32[k] + stdp_lut(a, dt), a.w_max, a.w_min);
+        for (int64_t p0 = pb + lane; p0 < pe; p0 += 4 * stride) {    // dendrite loop :392-411
+            uint32_t kk[4];
+            int32_t ls[4];
+            float wv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t p = min(p0 + u * stride, pe - 1);
+                kk[u] = a.in_pos[p];
+                ls[u] = a.last[a.in_src[p]];
+                wv[u] = a.w32[kk[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (p0 + u * stride < pe) a.w32[kk[u]] = clipw(wv[u] + stdp_lut(a, ls[u] - li), a.w_max, a.w_min);
         }
     }
 }
@@ -468,7 +502,9 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // 1. ring slice -> LDS.  The slots the previous window consumed ([sprev, s0)
     // mod D) were cleared by it and not stored back (step 4): they are zero here.
     const int64_t sprev = ctl->tbase[bprev];
-    const int32_t nprev = (int32_t)(s0 - sprev);           // 0 before the first window
+    // 0 before the first window; at most D (a network without synapses has D = 1
+    // and windows of up to kMaxWindow steps)
+    const int32_t nprev = (int32_t)min(s0 - sprev, (int64_t)D);
     const int32_t rprev = (int32_t)(sprev % D);
     uint32_t* gring = a.ring_t + (size_t)g * (size_t)RW * (size_t)D * (size_t)B;
     const uint32_t E = tcnt[bprev * nseg];
@@ -751,7 +787,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // 4. LDS -> ring slice, except the slots this window consumed (cleared above,
     // or never read for neurons that do not integrate)
     const int32_t rcur = (int32_t)(s0 % D);
-    const int32_t e = rcur + nsteps;
+    const int32_t e = rcur + min(nsteps, D);
 #pragma unroll
     for (int hh = 0; hh < RW; ++hh) {
         const uint32_t* rr = ring + (size_t)hh * D * B;
