@@ -12,6 +12,7 @@
 //  - Poisson inputs are keyed by Config::SEED (the reference seeds from the clock).
 #pragma once
 #include <stdexcept>
+#include <string>
 
 #include "../model/model.h"
 #include "../model/network.h"
@@ -43,6 +44,10 @@ class MGSimulator {
     std::vector<float> membrane_potentials();
     void spikes(std::vector<int>& steps, std::vector<int>& ids);
     long long syn_events();
+    // new: checkpoint / resume to and from a file (mgb_checkpoint_save / _load); load
+    // into a simulator built from the same network and Config
+    void save(const std::string& path);
+    void load(const std::string& path);
     mgb_sim* handle() { return sim; }
 };
 

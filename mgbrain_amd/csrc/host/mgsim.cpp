@@ -5,7 +5,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <fstream>
 #include <iostream>
+#include <iterator>
 
 namespace MGBrain {
 
@@ -237,6 +239,23 @@ long long MGSimulator::syn_events() {
     mgb_info inf;
     check(mgb_get_info(sim, &inf));
     return inf.syn_events;
+}
+
+void MGSimulator::save(const std::string& path) {
+    int64_t bytes = 0;
+    check(mgb_checkpoint_size(sim, &bytes));
+    std::vector<uint8_t> buf((size_t)bytes);
+    check(mgb_checkpoint_save(sim, buf.data(), bytes));
+    std::ofstream f(path, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(buf.data()), (std::streamsize)buf.size());
+    if (!f) throw std::runtime_error("cannot write checkpoint " + path);
+}
+
+void MGSimulator::load(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot read checkpoint " + path);
+    std::vector<uint8_t> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    check(mgb_checkpoint_load(sim, buf.data(), (int64_t)buf.size()));
 }
 
 }  // namespace MGBrain

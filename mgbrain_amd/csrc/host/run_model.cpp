@@ -10,7 +10,8 @@
 //   pop NUM TYPE SOURCE | proj S T WMIN WMAX DMIN DMAX CTYPE [STDP]
 //   bern S T WMIN WMAX DMIN DMAX P [STDP] | fixk S T WMIN WMAX DMIN DMAX K [STDP]
 //   brunel NSYN | vogel NSYN | rate POP R | vinit POP LO HI
-//   steps T | record CAP | weight_format W | device_gen 0/1 | nparts N | single_gpu
+//   steps T | record CAP | weight_format W | device_gen 0/1 | nparts N | single_gpu |
+//   checkpoint K PATH (save after K steps, finish in a new simulator loaded from PATH)
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -38,7 +39,8 @@ int main(int argc, char** argv) {
     std::vector<Population> pops;
     uint64_t clock_seed = 1;
     unsigned srand_seed = 1;
-    int steps = 100, device_gen = 0, nparts = 1;
+    int steps = 100, device_gen = 0, nparts = 1, ckpt_at = -1;
+    std::string ckpt_path;
     std::string line;
     while (std::getline(in, line)) {
         std::istringstream ss(line);
@@ -89,6 +91,7 @@ int main(int argc, char** argv) {
         else if (k == "record") { long long c; ss >> c; Config::set_record_capacity(c); }
         else if (k == "weight_format") { int w; ss >> w; Config::set_weight_format(w); }
         else if (k == "device_gen") ss >> device_gen;
+        else if (k == "checkpoint") ss >> ckpt_at >> ckpt_path;
         else if (k == "nparts") ss >> nparts;
         else if (k == "single_gpu") Config::enable_single_gpu();
         else {
@@ -99,15 +102,28 @@ int main(int argc, char** argv) {
     Network net;
     MGSimulator* sim = nullptr;
     try {
+        std::vector<int> part;
         if (device_gen) {
             sim = new MGSimulator(m, clock_seed);
         } else {
             std::srand(srand_seed);
             NetGen::gen_net(net, m, clock_seed);
-            std::vector<int> part(net.neurons.size(), 0);
+            part.assign(net.neurons.size(), 0);
             sim = new MGSimulator(net, part, nparts);
         }
-        sim->simulate(steps * Config::STEP);
+        if (ckpt_at >= 0) {
+            // run ckpt_at steps, save, and finish the run in a new simulator of the
+            // same network restored from the file
+            sim->simulate(ckpt_at * Config::STEP);
+            sim->save(ckpt_path);
+            delete sim;
+            sim = nullptr;
+            sim = device_gen ? new MGSimulator(m, clock_seed) : new MGSimulator(net, part, nparts);
+            sim->load(ckpt_path);
+            sim->simulate((steps - ckpt_at) * Config::STEP);
+        } else {
+            sim->simulate(steps * Config::STEP);
+        }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         delete sim;

@@ -21,7 +21,7 @@ PROG = os.path.join(ROOT, "mgbrain_amd", "lib", "mgb_run_model")
 def run_model(spec):
     with tempfile.TemporaryDirectory() as d:
         sp, out = os.path.join(d, "spec.txt"), os.path.join(d, "out.bin")
-        open(sp, "w").write("\n".join(spec) + "\n")
+        open(sp, "w").write("\n".join(x.replace("{dir}", d) for x in spec) + "\n")
         r = subprocess.run([PROG, sp, out], capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stdout + r.stderr
         raw = open(out, "rb").read()
@@ -112,3 +112,20 @@ def test_facade_per_projection_stdp():
                      acc=po.ACC_FIXED, scale_exc=info["fix_scale_exc"], scale_inh=info["fix_scale_inh"])
     assert H.first_mismatch(H.split_steps(r["steps"], r["ids"], 600), o.run(600)) < 0
     assert np.array_equal(r["v"].view(np.uint32), o.state(po.F_V).astype(np.float32).view(np.uint32))
+
+
+@pytest.mark.parametrize("device_gen", [0, 1])
+def test_facade_checkpoint_resume(device_gen):
+    _say(f"case: facade save after 333 steps, finish in a new MGSimulator loaded from the file, device_gen={device_gen}")
+    base = ["timestep 0.0001", "clock 7", "seed 3", "lifconst 0 0 0.01 0 0 0.002 1 1 0.02 0",
+            "pop 300 2 1", "pop 800 5 0", "pop 200 5 0", "rate 0 0.2",
+            "bern 0 1 0.0001 0.0001 0.0014 0.0014 0.1", "bern 0 2 0.0001 0.0001 0.0014 0.0014 0.1",
+            "fixk 1 1 0.0001 0.0001 0.001 0.002 80", "fixk 1 2 0.0001 0.0001 0.001 0.002 80",
+            "fixk 2 1 -0.0005 -0.0005 0.001 0.002 20", "bern 2 2 -0.0005 -0.0005 0.0014 0.0014 0.1",
+            "steps 900", "record 2000000", f"device_gen {device_gen}"]
+    whole = run_model(base)
+    resumed = run_model(base + ["checkpoint 333 {dir}/ck.bin"])
+    assert len(whole["ids"]) > 1000
+    assert np.array_equal(whole["steps"], resumed["steps"]) and np.array_equal(whole["ids"], resumed["ids"])
+    assert np.array_equal(whole["v"].view(np.uint32), resumed["v"].view(np.uint32))
+    assert whole["se"] == resumed["se"]
