@@ -183,19 +183,26 @@ def batched_leg(args, shapes=None, log=None):
 
 
 def stdp_leg(args, log):
-    """Brunel 100k (K = 10,000) with pair-based STDP on the E->E synapses (window of one
-    step; k_neurons + k_prop + k_stdp per step).  Static weights in the format chosen
-    by --precision (bf16 default), plastic weights FP32."""
+    """Brunel 100k (K = 10,000) with pair-based STDP on the E->E synapses.  --stdp-path
+    fused (default): k_tiled_stdp, one launch per window of min(min_delay, shortest
+    inter-spike interval of E neurons) = 10 steps, all weights FP32.  direct: a window of
+    one step (k_neurons + k_prop + k_stdp per step), static weights in the format chosen
+    by --precision, plastic weights FP32."""
     pops, projs, desc, ntot = W.brunel_fixed_k(args.scale, seed=args.seed, plastic_ee=True)
     sc, lc = W.stdp_consts(), W.lifb_consts()
-    wf = N.WF_BF16 if args.precision == "bf16" else N.WF_F32
+    fused = args.stdp_path == "fused"
+    wf = N.WF_F32 if fused or args.precision != "bf16" else N.WF_BF16
+    prop = N.PROP_TILED if fused else N.PROP_DIRECT
     t0 = time.perf_counter()
+    shape = dict(tile_size=args.tile, prop_block=args.block, chunk=args.chunk) if fused else {}
     sim = N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed, stdp=True, stdpconst=sc,
-                                weight_format=wf, use_graph=args.graph, profile=0 if args.no_profile else 1)
+                                weight_format=wf, use_graph=args.graph, profile=0 if args.no_profile else 1,
+                                propagation=prop, **shape)
     sim.sync()
     setup_s = time.perf_counter() - t0
     info0 = sim.info()
     log(f"# stdp: setup {setup_s:.2f} s, {info0['n_synapses']} synapses, window {info0['window']}, "
+        f"tile {info0['tile_size']} block {info0['block']}, "
         f"{info0['device_bytes'] / 2**30:.2f} GiB")
     sim.run(args.warmup)
     sim.sync()
@@ -221,7 +228,7 @@ def stdp_leg(args, log):
     roof = None
     if not args.no_profile:
         prof = N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed, stdp=True, stdpconst=sc,
-                                     weight_format=wf, use_graph=0, profile=1)
+                                     weight_format=wf, use_graph=0, profile=1, propagation=prop, **shape)
         prof.run(500)
         pa = prof.info()
         prof.run(1000)
@@ -240,6 +247,10 @@ def stdp_leg(args, log):
                    "k_stdp": (8.0 + 16.0) * plastic_share * pse}
         k_ms = {"k_neurons": (pb["neuron_ms"], pb["neuron_launches"]), "k_prop": (pb["prop_ms"], pb["prop_launches"]),
                 "k_stdp": (pb["stdp_ms"], pb["stdp_launches"])}
+        if fused:
+            # one kernel does the three kernels' work: the same per-unit bytes, summed
+            k_bytes = {"k_tiled_stdp": sum(k_bytes.values())}
+            k_ms = {"k_tiled_stdp": (pb["prop_ms"], pb["prop_launches"])}
         kernels = {}
         for k, (ms, nl) in k_ms.items():
             if nl and ms > 0:
@@ -256,8 +267,9 @@ def stdp_leg(args, log):
     return {
         "workload": "brunel100k_K10000_stdp_ee", "description": desc + "; STDP on E->E (A+ 0.01, A- 0.0105 "
         "x W_max, tau 20 ms, W in [0, 0.2 mV])", "value": se / wall, "unit": "syn_events/s",
-        "rtf": wall / (args.steps * DT), "simulated_s": args.steps * DT, "static_weights": args.precision,
-        "plastic_weights": "f32",
+        "rtf": wall / (args.steps * DT), "simulated_s": args.steps * DT,
+        "path": args.stdp_path, "window": info0["window"],
+        "static_weights": "f32" if wf == N.WF_F32 else "bf16", "plastic_weights": "f32",
         "mean_rate_hz": spikes / ntot / (args.steps * DT), "setup_s": round(setup_s, 2),
         "ee_weights": {"n": int(wee.size), "mean": float(wee.mean()) if wee.size else 0.0,
                        "hist_edges": [float(x) for x in edges], "hist": [int(x) for x in hist]},
@@ -338,6 +350,8 @@ def main(argv=None):
     ap.add_argument("--precision", default="bf16", choices=("bf16", "f32"),
                     help="static weight storage of --config 4")
     ap.add_argument("--nets", type=int, default=64, help="networks in the batched sweep")
+    ap.add_argument("--stdp-path", default="fused", choices=("fused", "direct"),
+                    help="--config 4: fused window kernel (FP32 weights) or the direct kernels")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg of the default run")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=0)

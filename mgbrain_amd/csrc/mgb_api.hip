@@ -61,7 +61,7 @@ struct mgb_sim {
     uint8_t *type = nullptr, *cset = nullptr;
     uint16_t *tbl = nullptr, *stream_idx = nullptr;
     float *rate = nullptr, *V = nullptr, *Ie = nullptr, *Ii = nullptr;
-    int32_t *refr = nullptr, *last = nullptr, *cnt = nullptr;
+    int32_t *refr = nullptr, *last = nullptr, *cnt = nullptr;   // kTStdp: last is [3][npad]
     CSet* csets = nullptr;
     int32_t* stream_base = nullptr;
     uint64_t* stream_seed = nullptr;
@@ -99,9 +99,12 @@ struct mgb_sim {
     uint64_t n_offs = 0, n_keys = 0;
     uint32_t* tcnt = nullptr;          // [3][nseg]
     int tkind = kTClass;               // synapse word kind of the fused kernel
+    uint32_t* tperm = nullptr;         // kTStdp: CSR slot of each tile-order position
+    int32_t *hist = nullptr, *tf = nullptr, *cntf = nullptr;   // kTStdp: deferred dendrite updates
+    int32_t stdp_isi = 1;              // kTStdp: shortest inter-spike interval of plastic synapses' ends
     uint32_t* syn_t = nullptr;
     uint16_t* syn16 = nullptr;         // kTClass16 words in tile order
-    float* w_t = nullptr;              // kTF32 weights in tile order
+    float* w_t = nullptr;              // kTF32 / kTStdp weights in tile order
     uint16_t* offs16 = nullptr;
     uint32_t* offs32 = nullptr;
     uint32_t* ring_t = nullptr;
@@ -323,9 +326,8 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D) {
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
     lds_max = std::min(lds_max, 160 * 1024);
     // the kernel's static LDS (block scan sums, event counter) shares the limit
-    const int rw = s->tkind == kTClass ? 1 : 2;       // ring words per (slot, target)
-    int32_t bmax = (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0, s->tkind) - kTiledStaticLds) /
-                             (4 * (int64_t)rw * D));
+    const int64_t per_b = (int64_t)(tiled_lds_bytes(D, 1, s->tkind) - tiled_lds_bytes(D, 0, s->tkind));
+    int32_t bmax = (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0, s->tkind) - kTiledStaticLds) / per_b);
     // CLASS words hold the global target id; wide words an 11-bit tile-local one
     bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, s->tkind == kTClass ? (1 << 14) : 2048));
     int32_t B = (o && o->tile_size > 0) ? o->tile_size : std::max<int32_t>(256, (n + 255) / 256);
@@ -691,17 +693,18 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         g.col = s->col; g.dly = s->dly; g.w32 = s->w32; g.w16 = s->w16; g.w8 = s->w8;
         g.row_scale = s->row_scale; g.pos_of = pos_of;
         MGB_HIP_TRY(gather_wide(g, st));
-        // fused path for static networks whose delays fit the 5-bit field
+        // fused path for delays that fit the 5-bit field; with STDP, FP32 weights
         const int prop = o ? o->propagation : MGB_PROP_AUTO;
-        const bool tiled_ok = !h.stdp && max_d <= kClassMaxDelay;
+        const bool tiled_ok = max_d <= kClassMaxDelay && (!h.stdp || s->wide_kind == kWideF32);
         if (prop == MGB_PROP_TILED && !tiled_ok) {
-            set_error("tiled propagation of the wide formats needs a network without STDP and delays <= %d "
-                      "steps", kClassMaxDelay);
+            set_error("tiled propagation of the wide formats needs delays <= %d steps (and FP32 weights with "
+                      "STDP)", kClassMaxDelay);
             return MGB_ERR_UNSUPPORTED;
         }
         s->tiled = (tiled_ok && prop != MGB_PROP_DIRECT) ? 1 : 0;
         if (s->tiled) {
-            s->tkind = s->wide_kind == kWideBF16 ? kTBF16 : (s->wide_kind == kWideFP8 ? kTFP8 : kTF32);
+            s->tkind = h.stdp ? kTStdp
+                              : (s->wide_kind == kWideBF16 ? kTBF16 : (s->wide_kind == kWideFP8 ? kTFP8 : kTF32));
             MGB_TRY(choose_tiles(s, o, n, s->D));
             const TileTabs tt = tile_tabs(s, seg_dev);
             // kout holds each CSR slot's source (the keys of the sort by source)
@@ -718,10 +721,18 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
                 MGB_HIP_TRY(sort_pairs(temp, &tb2, kin, kout, vin, vout, S, tbits, st));
             }
             MGB_TRY(dalloc_n(s, &s->syn_t, S));
-            if (s->tkind == kTF32) MGB_TRY(dalloc_n(s, &s->w_t, S));
+            if (s->tkind == kTF32 || s->tkind == kTStdp) MGB_TRY(dalloc_n(s, &s->w_t, S));
             MGB_HIP_TRY(tile_gather_wide(vout, S, s->col, s->dly, s->w16, s->w8, s->w32, tt, s->tkind, s->syn_t,
                                          s->w_t, st));
             MGB_TRY(build_tile_offsets(s, n, kout, tt));
+            if (s->tkind == kTStdp) {
+                // plastic weights live in tile order: keep the slot of each position (read-back)
+                // and map the CSR slots of the in-synapse lists below to tile positions
+                MGB_TRY(dalloc_n(s, &s->tperm, S));
+                if (S) MGB_HIP_TRY(hipMemcpyAsync(s->tperm, vout, S * 4, hipMemcpyDeviceToDevice, st));
+                MGB_HIP_TRY(invert_slots(vout, S, vin, st));
+                MGB_HIP_TRY(remap_u32(pos_of, vin, S, st));
+            }
         }
         if (h.stdp) {
             // incoming CSR of plastic synapses in list order (dend_offs/dend_refs)
@@ -742,6 +753,28 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
             MGB_TRY(dalloc_n(s, &s->in_pos, (size_t)s->n_plastic));
             MGB_TRY(dalloc_n(s, &s->in_src, (size_t)s->n_plastic));
             MGB_HIP_TRY(in_fill(vout, pos_of, L.src, (uint64_t)s->n_plastic, s->in_pos, s->in_src, st));
+            if (s->tkind == kTStdp) {
+                // window cap: the shortest inter-spike interval of any end of a plastic
+                // synapse (1 step for Poisson neurons, refractory steps + 1 for the
+                // integrating types; neurons of other types never fire)
+                std::vector<int32_t> isi((size_t)n + 1, kMaxWindow);   // isi[n]: the result
+                for (int32_t i = 0; i < n; ++i) {
+                    const int ty = h.type[i];
+                    if (ty == MGB_POISSON) {
+                        isi[i] = 1;
+                    } else if (ty == MGB_LIF || ty == MGB_LIFE || ty == MGB_LIFB) {
+                        const float* c = &h.lifconst[30 * (size_t)h.cset[i]];
+                        const int32_t refr = (int32_t)roundf(c[8] / c[0]) - 1;   // as setup_state
+                        isi[i] = std::max(refr + 1, 1);
+                    }
+                }
+                int32_t* disi = nullptr;
+                MGB_TRY(X.get(&disi, (size_t)n + 1));
+                MGB_HIP_TRY(hipMemcpyAsync(disi, isi.data(), 4 * ((size_t)n + 1), hipMemcpyHostToDevice, st));
+                MGB_HIP_TRY(plastic_isi(s->in_ptr, s->in_src, disi, n, (uint64_t)s->n_plastic, disi + n, st));
+                MGB_HIP_TRY(hipMemcpyAsync(&s->stdp_isi, disi + n, 4, hipMemcpyDeviceToHost, st));
+                MGB_HIP_TRY(hipStreamSynchronize(st));   // isi goes out of scope
+            }
         }
         MGB_HIP_TRY(hipStreamSynchronize(st));
     }
@@ -765,7 +798,8 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     MGB_TRY(dalloc_n(s, &s->Ie, np));
     MGB_TRY(dalloc_n(s, &s->Ii, np));
     MGB_TRY(dalloc_n(s, &s->refr, np));
-    MGB_TRY(dalloc_n(s, &s->last, np));
+    const size_t nlast = (s->tiled && s->tkind == kTStdp) ? 3 : 1;   // kTStdp: three rotating buffers
+    MGB_TRY(dalloc_n(s, &s->last, nlast * np));
     MGB_TRY(dalloc_n(s, &s->cnt, np));
     MGB_HIP_TRY(hipMemsetAsync(s->ctl, 0, sizeof(Ctl), st));
     MGB_HIP_TRY(hipMemcpyAsync(s->type, h.type.data(), (size_t)n, hipMemcpyHostToDevice, st));
@@ -776,7 +810,15 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     MGB_HIP_TRY(hipMemsetAsync(s->Ie, 0, 4 * np, st));
     MGB_HIP_TRY(hipMemsetAsync(s->Ii, 0, 4 * np, st));
     MGB_HIP_TRY(hipMemsetAsync(s->refr, 0, 4 * np, st));
-    MGB_HIP_TRY(hipMemsetAsync(s->last, 0, 4 * np, st));   // mgnet.cpp:16 zero-initialised
+    MGB_HIP_TRY(hipMemsetAsync(s->last, 0, 4 * nlast * np, st));   // mgnet.cpp:16 zero-initialised
+    if (nlast == 3) {
+        MGB_TRY(dalloc_n(s, &s->hist, np * kStdpHist));
+        MGB_TRY(dalloc_n(s, &s->tf, np));
+        MGB_TRY(dalloc_n(s, &s->cntf, np));
+        MGB_HIP_TRY(hipMemsetAsync(s->hist, 0, 4 * np * kStdpHist, st));
+        MGB_HIP_TRY(hipMemsetAsync(s->tf, 0xFF, 4 * np, st));     // -1: no update deferred yet
+        MGB_HIP_TRY(hipMemsetAsync(s->cntf, 0, 4 * np, st));
+    }
     MGB_HIP_TRY(hipMemsetAsync(s->cnt, 0, 4 * np, st));
     // constants: LIFECONSTS + refractory steps (mgcore.cu:153)
     const size_t ncs = h.lifconst.size() / 30;
@@ -797,7 +839,8 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
                                hipMemcpyHostToDevice, st));
     // ring
     if (s->tiled) {
-        const size_t rt = (size_t)s->G * (size_t)s->D * (size_t)s->B * (s->tkind == kTClass ? 1u : 2u);
+        const size_t rw = (s->tkind == kTClass || s->tkind == kTClass16) ? 1u : 2u;
+        const size_t rt = (size_t)s->G * (size_t)s->D * (size_t)s->B * rw;
         MGB_TRY(dalloc_n(s, &s->ring_t, rt));
         MGB_HIP_TRY(hipMemsetAsync(s->ring_t, 0, 4 * rt, st));
     } else if (s->mode == kRingClass) {
@@ -813,7 +856,9 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     int32_t win = s->S ? s->min_delay : kMaxWindow;
     if (o && o->window > 0) win = std::min(win, o->window);
     win = std::min(win, kMaxWindow);
-    if (h.stdp) win = 1;
+    // STDP: one step per window on the direct path; the fused path takes windows no
+    // longer than the shortest inter-spike interval of the plastic synapses' ends
+    if (h.stdp) win = s->tiled ? std::min(win, s->stdp_isi) : 1;
     s->window = std::max(win, 1);
     s->win_stride = (uint32_t)(np * (size_t)s->window);
     MGB_TRY(dalloc_n(s, &s->win_buf, (s->tiled ? 3 : 2) * (size_t)s->win_stride));
@@ -897,6 +942,13 @@ TiledArgs tiled_args(const mgb_sim* s) {
     a.row_ptr = s->row_ptr; a.syn_t = s->syn_t; a.syn16 = s->syn16; a.offs16 = s->offs16; a.offs32 = s->offs32;
     a.ring_t = s->ring_t; a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
     a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
+    if (s->tkind == kTStdp) {
+        a.last3 = s->last; a.last_stride = (size_t)s->npad;
+        a.hist = s->hist; a.tf = s->tf; a.cntf = s->cntf;
+        a.din_ptr = s->in_ptr; a.din_pos = s->in_pos; a.din_src = s->in_src;
+        a.ltp = s->ltp; a.ltd = s->ltd; a.lut_n = s->lut_n;
+        a.w_max = s->sc[4]; a.w_min = s->sc[5];
+    }
     return a;
 }
 
@@ -1501,7 +1553,7 @@ mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
     out->tiles = s->G;
     out->block = s->tiled ? s->tblock : s->prop_block;
     out->segments = s->tiled ? s->nseg : 1;
-    out->word_bytes = !s->tiled ? 0 : (s->tkind == kTClass16 ? 2 : (s->tkind == kTF32 ? 8 : 4));
+    out->word_bytes = !s->tiled ? 0 : (s->tkind == kTClass16 ? 2 : (s->tkind == kTF32 || s->tkind == kTStdp ? 8 : 4));
     out->last_run_ms = 0.0;
     if (s->timed) {
         float ms = 0.f;
@@ -1583,6 +1635,12 @@ mgb_status mgb_read_state(mgb_sim* s, int32_t field, void* out) {
             return MGB_ERR_INVALID;
     }
     MGB_TRY(mgb_sync(s));
+    if (field == MGB_F_LAST && s->tiled && s->tkind == kTStdp) {
+        // the next window reads buffer base_window % 3
+        Ctl c;
+        MGB_HIP_TRY(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+        src = s->last + (size_t)(c.base_window % 3) * (size_t)s->npad;
+    }
     MGB_HIP_TRY(hipMemcpy(out, src, 4 * (size_t)s->n, hipMemcpyDeviceToHost));
     return MGB_OK;
 }
@@ -1610,8 +1668,9 @@ struct CkptSeg {
 // Every device array a run changes, in a fixed order.
 std::vector<CkptSeg> ckpt_segments(const mgb_sim* s, uint64_t n_rec) {
     const size_t np = (size_t)s->npad;
+    const size_t nlast = (s->tiled && s->tkind == kTStdp) ? 3 : 1;
     std::vector<CkptSeg> v = {{s->V, 4 * np}, {s->Ie, 4 * np}, {s->Ii, 4 * np},
-                              {s->refr, 4 * np}, {s->last, 4 * np}, {s->cnt, 4 * np},
+                              {s->refr, 4 * np}, {s->last, 4 * nlast * np}, {s->cnt, 4 * np},
                               {s->ctl, sizeof(Ctl)},
                               {s->win_buf, 4 * (size_t)(s->tiled ? 3 : 2) * (size_t)s->win_stride}};
     if (s->tiled) {
@@ -1624,7 +1683,12 @@ std::vector<CkptSeg> ckpt_segments(const mgb_sim* s, uint64_t n_rec) {
         v.push_back({s->ring_e, 4 * (size_t)s->D * np});
         v.push_back({s->ring_i, 4 * (size_t)s->D * np});
     }
-    if (s->stdp && s->w32) v.push_back({s->w32, 4 * (size_t)s->S});
+    if (s->stdp) v.push_back({s->tkind == kTStdp && s->tiled ? s->w_t : s->w32, 4 * (size_t)s->S});
+    if (s->hist) {
+        v.push_back({s->hist, 4 * np * kStdpHist});
+        v.push_back({s->tf, 4 * np});
+        v.push_back({s->cntf, 4 * np});
+    }
     if (n_rec) v.push_back({s->rec, 8 * (size_t)n_rec});
     return v;
 }
@@ -1726,6 +1790,37 @@ mgb_status mgb_checkpoint_load(mgb_sim* s, const void* buf, int64_t bytes) {
 
 mgb_status mgb_fire_counts(mgb_sim* s, int32_t* out) { return mgb_read_state(s, MGB_F_CNT, out); }
 
+namespace {
+
+// kTStdp: the fused kernel applies a window's STDP in the next window's launch.
+// Writes to w32 (CSR order) the weights the reference holds after the last
+// step run: a copy of w_t with the pending window's updates applied.
+mgb_status current_weights(mgb_sim* s) {
+    const size_t S = (size_t)s->S;
+    float* tmp = nullptr;
+    hipError_t e = hipMalloc(&tmp, std::max<size_t>(S, 1) * 4);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu) for the weight read-back failed: %s", S * 4, hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? MGB_ERR_OOM : MGB_ERR_HIP;
+    }
+    TiledArgs ta = tiled_args(s);
+    ta.w_t = tmp;
+    ta.apply_only = 1;
+    e = hipMemcpyAsync(tmp, s->w_t, S * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = launch_tiled(ta, 0, 0, 1, s->tblock, s->stream);
+    if (e == hipSuccess) e = scatter_weights(s->tperm, tmp, (uint64_t)S, s->w32, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    const hipError_t f = hipFree(tmp);
+    if (e == hipSuccess) e = f;
+    if (e != hipSuccess) {
+        set_error("weight read-back: %s", hipGetErrorString(e));
+        return MGB_ERR_HIP;
+    }
+    return MGB_OK;
+}
+
+}  // namespace
+
 mgb_status mgb_read_csr(mgb_sim* s, int64_t* row_ptr, int32_t* col, int32_t* delay, float* weight) {
     if (!s) {
         set_error("null handle");
@@ -1756,6 +1851,7 @@ mgb_status mgb_read_csr(mgb_sim* s, int64_t* row_ptr, int32_t* col, int32_t* del
     if (delay)
         for (size_t k = 0; k < S; ++k) delay[k] = (int32_t)(dl[k] & 127u);
     if (weight && S) {
+        if (s->tiled && s->tkind == kTStdp) MGB_TRY(current_weights(s));
         MGB_HIP_TRY(hipMemcpy(weight, s->w32, 4 * S, hipMemcpyDeviceToHost));
         if (s->wide_kind == kWideBF16) {
             std::vector<uint16_t> w16(S);

@@ -87,6 +87,11 @@ hipError_t make_tile_keys_csr(const uint32_t* src, const int32_t* col, const Til
 hipError_t tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col, const uint8_t* dly,
                             const uint16_t* w16, const uint8_t* w8, const float* w32, const TileTabs& tt, int kind,
                             uint32_t* syn_t, float* w_t, hipStream_t st);
+hipError_t invert_slots(const uint32_t* slot, uint64_t S, uint32_t* inv, hipStream_t st);
+hipError_t remap_u32(uint32_t* a, const uint32_t* map, uint64_t np, hipStream_t st);
+hipError_t scatter_weights(const uint32_t* slot, const float* w_t, uint64_t S, float* w32, hipStream_t st);
+hipError_t plastic_isi(const int64_t* in_ptr, const int32_t* in_src, const int32_t* isi, int32_t n, uint64_t np,
+                       int32_t* out, hipStream_t st);
 hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, const TileTabs& tt,
                         uint64_t total, uint16_t* offs16, uint32_t* offs32, hipStream_t st);
 hipError_t row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n, float top,

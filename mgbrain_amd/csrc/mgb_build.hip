@@ -144,8 +144,8 @@ __global__ void k_tile_keys_csr(const uint32_t* src, const int32_t* col, TileTab
 }
 
 // Fused-path words of the wide formats in tile order (slot[k]: the CSR slot at
-// tile-order position k): local target | delay-1 << 11 | BF16 / FP8 bits << 16;
-// FP32 weights go to w_t.
+// tile-order position k): local target | delay-1 << 11 | BF16 / FP8 bits << 16
+// (kTStdp: the plastic flag << 16); FP32 weights go to w_t.
 __global__ void k_tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col, const uint8_t* dly,
                                    const uint16_t* w16, const uint8_t* w8, const float* w32, TileTabs tt,
                                    int kind, uint32_t* syn_t, float* w_t) {
@@ -159,9 +159,45 @@ __global__ void k_tile_gather_wide(const uint32_t* slot, uint64_t S, const int32
         uint32_t payload = 0u;
         if (kind == kTBF16) payload = w16[p];
         else if (kind == kTFP8) payload = w8[p];
+        else if (kind == kTStdp) payload = (uint32_t)(dly[p] >> 7);
         syn_t[k] = loc | ((d - 1u) << 11) | (payload << 16);
-        if (kind == kTF32) w_t[k] = w32[p];
+        if (kind == kTF32 || kind == kTStdp) w_t[k] = w32[p];
     }
+}
+
+// inv[slot[k]] = k: the tile-order position of every CSR slot.
+__global__ void k_invert(const uint32_t* slot, uint64_t S, uint32_t* inv) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
+         k += (uint64_t)gridDim.x * blockDim.x)
+        inv[slot[k]] = (uint32_t)k;
+}
+
+// a[p] = map[a[p]].
+__global__ void k_remap(uint32_t* a, const uint32_t* map, uint64_t np) {
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
+         p += (uint64_t)gridDim.x * blockDim.x)
+        a[p] = map[a[p]];
+}
+
+// w32[slot[k]] = w_t[k]: the CSR-order copy of tile-order weights.
+__global__ void k_scatter_w(const uint32_t* slot, const float* w_t, uint64_t S, float* w32) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
+         k += (uint64_t)gridDim.x * blockDim.x)
+        w32[slot[k]] = w_t[k];
+}
+
+// Smallest isi[] over the ends of plastic synapses (in-CSR in_ptr / in_src):
+// targets with a plastic in-synapse, then the sources of all of them.
+__global__ void k_plastic_isi(const int64_t* in_ptr, const int32_t* in_src, const int32_t* isi, int32_t n,
+                              uint64_t np, int32_t* out) {
+    int32_t m = 0x7FFFFFFF;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t j = t0; j < (uint64_t)n; j += stride)
+        if (in_ptr[j + 1] > in_ptr[j]) m = min(m, isi[j]);
+    for (uint64_t p = t0; p < np; p += stride) m = min(m, isi[in_src[p]]);
+    for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63u) == 0u && m != 0x7FFFFFFF) atomicMin(out, m);
 }
 
 // Offset entry e = seg_obase[q] + (m - seg_lo[q]) * (Gq + 1) + t, t in [0, Gq]:
@@ -392,6 +428,32 @@ hipError_t tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col
     if (S == 0) return hipSuccess;
     hipLaunchKernelGGL(k_tile_gather_wide, dim3(blocks_for(S, 256)), dim3(256), 0, st, slot, S, col, dly, w16, w8,
                        w32, tt, kind, syn_t, w_t);
+    return hipGetLastError();
+}
+
+hipError_t invert_slots(const uint32_t* slot, uint64_t S, uint32_t* inv, hipStream_t st) {
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_invert, dim3(blocks_for(S, 256)), dim3(256), 0, st, slot, S, inv);
+    return hipGetLastError();
+}
+
+hipError_t remap_u32(uint32_t* a, const uint32_t* map, uint64_t np, hipStream_t st) {
+    if (np == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_remap, dim3(blocks_for(np, 256)), dim3(256), 0, st, a, map, np);
+    return hipGetLastError();
+}
+
+hipError_t scatter_weights(const uint32_t* slot, const float* w_t, uint64_t S, float* w32, hipStream_t st) {
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_w, dim3(blocks_for(S, 256)), dim3(256), 0, st, slot, w_t, S, w32);
+    return hipGetLastError();
+}
+
+hipError_t plastic_isi(const int64_t* in_ptr, const int32_t* in_src, const int32_t* isi, int32_t n, uint64_t np,
+                       int32_t* out, hipStream_t st) {
+    const uint64_t work = std::max<uint64_t>((uint64_t)n, np);
+    hipLaunchKernelGGL(k_plastic_isi, dim3(blocks_for(std::max<uint64_t>(work, 1), 256)), dim3(256), 0, st, in_ptr,
+                       in_src, isi, n, np, out);
     return hipGetLastError();
 }
 

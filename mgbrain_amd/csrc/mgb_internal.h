@@ -144,14 +144,16 @@ struct StdpArgs {
 // (src - seg_lo[q]) * (Gq + 1) + local tile], Gq = tiles of q.
 // Synapse word kinds of the fused kernel.  kTClass16 is the CLASS word of a
 // tiling with B <= 1024: tile-local target (10 bits) | delay-1 << 10 | sign << 15.
-enum TiledKind { kTClass = 0, kTBF16 = 1, kTFP8 = 2, kTF32 = 3, kTClass16 = 4 };
+// kTStdp: the kTF32 word with a plastic flag at bit 16, FP32 weights in tile
+// order (w_t, updated in place), pair-based STDP inside the kernel.
+enum TiledKind { kTClass = 0, kTBF16 = 1, kTFP8 = 2, kTF32 = 3, kTClass16 = 4, kTStdp = 5 };
 
 struct TiledArgs {
     int32_t n, D, B, G;
     int32_t chunk;          // synapses per work item in the propagation phase
     int32_t kind;           // TiledKind
     float scale_e, scale_i, inv_e, inv_i;   // fixed-point scales of the wide kinds
-    const float* w_t;       // kTF32: weights in tile order
+    float* w_t;             // kTF32 / kTStdp: weights in tile order
     const float* row_scale; // kTFP8: per-source scale
     int32_t nseg;
     int32_t window;         // spike-list entries reserved per neuron (steps per window)
@@ -189,6 +191,27 @@ struct TiledArgs {
     uint32_t win_stride;
     uint64_t* rec;
     uint64_t rec_cap;
+    // kTStdp.  Window k reads its neurons' last spike steps from last3 buffer
+    // k % 3 and writes them to (k + 1) % 3, so buffers (k - 1) % 3 and k % 3
+    // hold window k-1's start and end values for the whole launch.  Dendrite
+    // updates are deferred: neuron j keeps its last kStdpHist spike steps
+    // (hist, ring indexed by the fire count), its in-synapses are up to date
+    // through step tf[j], and cntf[j] is its fire count at tf[j].
+    int32_t* last3;             // [3][last_stride]
+    size_t last_stride;
+    int32_t* hist;              // [n][kStdpHist]
+    int32_t* tf;                // [n]
+    int32_t* cntf;              // [n]
+    const int64_t* din_ptr;     // [n+1] plastic in-synapses of each target
+    const uint32_t* din_pos;    // their tile-order positions
+    const int32_t* din_src;     // their sources
+    const float* ltp;           // ltp[k] = A_LTP * expf(-k / TAU_LTP)
+    const float* ltd;           // ltd[k] = A_LTD * expf(-k / TAU_LTD)
+    int32_t lut_n;
+    float w_max, w_min;
+    // kTStdp read-back: apply the pending window's STDP to w_t (a copy) and
+    // touch nothing else — no ring, neuron state, lists or counters
+    int32_t apply_only;
 };
 
 enum RingMode { kRingClass = 0, kRingFixed = 1 };
@@ -205,6 +228,7 @@ constexpr int kTiledBatch = 1024;   // spikes staged in LDS per propagation batc
 constexpr int kTiledItems = 8192;   // work items staged in LDS per propagation round
 constexpr int kTiledStaticLds = 1024; // bytes reserved for the fused kernel's static LDS
 constexpr int kClassCache = 32;     // k-fold class sums cached in LDS (tables hold >= this many)
+constexpr int kStdpHist = 16;       // spike steps kept per neuron for deferred dendrite updates
 size_t tiled_lds_bytes(int D, int B, int kind);
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st);

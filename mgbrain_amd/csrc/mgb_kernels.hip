@@ -61,16 +61,18 @@ __device__ __forceinline__ bool integrates(int type) {
     return type == T_LIF || type == T_LIFB || type == T_LIFE;
 }
 
-// Loads the neuron's registers (state, constants, stream).
+// Loads the neuron's registers (state, constants, stream); `last` holds the
+// last spike steps.
 template <class A>
-__device__ __forceinline__ void neuron_load(const A& a, int i, int type, bool class_tables, NState& st) {
+__device__ __forceinline__ void neuron_load(const A& a, int i, int type, bool class_tables, NState& st,
+                                            const int32_t* last) {
     st.V = st.Ie = st.Ii = st.rate = 0.f;
     st.refr = st.refr_steps = 0;
     st.c3 = st.c4 = st.c12 = st.c13 = st.c14 = st.c15 = st.c16 = st.c17 = 0.f;
     st.off_e = st.off_i = st.key0 = st.key1 = st.lid = st.cls = 0u;
     st.we = st.wi = 0.f;
     st.rnd = U32x4{{0u, 0u, 0u, 0u}};
-    st.last = a.last[i];
+    st.last = last[i];
     st.cnt = a.cnt[i];
     if (type == T_POISSON) {
         st.rate = a.rate[i];
@@ -106,8 +108,8 @@ __device__ __forceinline__ void neuron_load(const A& a, int i, int type, bool cl
 }
 
 template <class A>
-__device__ __forceinline__ void neuron_store(const A& a, int i, int type, const NState& st) {
-    a.last[i] = st.last;
+__device__ __forceinline__ void neuron_store(const A& a, int i, int type, const NState& st, int32_t* last) {
+    last[i] = st.last;
     a.cnt[i] = st.cnt;
     if (integrates(type)) {
         a.V[i] = st.V;
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(kNeuBlock) void k_neurons(NeuArgs a, int32_t off, i
     const int type = active ? (int)a.type[i] : T_RAW;
     uint32_t* wbuf = a.win_buf + (size_t)par * a.win_stride;
     NState st;
-    if (active) neuron_load(a, i, type, MODE == kRingClass, st);
+    if (active) neuron_load(a, i, type, MODE == kRingClass, st, a.last);
     const bool integ = integrates(type);
     int32_t slot = (int32_t)(s0 % a.D);
     for (int32_t w = 0; w < nsteps; ++w) {
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(kNeuBlock) void k_neurons(NeuArgs a, int32_t off, i
         }
         slot = (slot + 1 == a.D) ? 0 : slot + 1;
     }
-    if (active) neuron_store(a, i, type, st);
+    if (active) neuron_store(a, i, type, st, a.last);
 }
 
 // Propagation: one wave per spike, lanes stride the spiker's CSR row.
@@ -340,10 +342,16 @@ __global__ void k_prop(PropArgs a, int32_t off, int32_t wk) {
     if (threadIdx.x == 0 && se_blk) atomicAdd((unsigned long long*)&ctl->se, se_blk);
 }
 
-__device__ __forceinline__ float stdp_lut(const StdpArgs& a, int32_t dt) {
-    if (dt < 0) return a.ltp[min(-dt, a.lut_n - 1)];
-    if (dt > 0) return a.ltd[min(dt, a.lut_n - 1)];
+// Weight change of a pair dt = t_pre - t_post apart (mgcore.cu:346-360): the
+// A_LTP branch for dt < 0, A_LTD for dt > 0, none at 0; tables indexed by |dt|.
+__device__ __forceinline__ float stdp_dw(const float* ltp, const float* ltd, int32_t lut_n, int32_t dt) {
+    if (dt < 0) return ltp[min(-dt, lut_n - 1)];
+    if (dt > 0) return ltd[min(dt, lut_n - 1)];
     return 0.0f;
+}
+
+__device__ __forceinline__ float stdp_lut(const StdpArgs& a, int32_t dt) {
+    return stdp_dw(a.ltp, a.ltd, a.lut_n, dt);
 }
 
 __device__ __forceinline__ float clipw(float w, float wmax, float wmin) {   // mgcore.cu:362
@@ -435,6 +443,62 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* wsum) 
     return v;
 }
 
+// One tile's ring slice (RW halves of D x B words) from HBM into LDS.  Rows
+// [r0, r0 + nz) mod D were consumed by the previous window and not stored
+// back: they are cleared instead of loaded.  8 loads per thread in flight
+// (clamped indices keep every load in bounds; the extra values are not stored).
+template <int RW>
+__device__ __forceinline__ void ring_slice_in(uint32_t* ring, const uint32_t* gring, int32_t D, int32_t B,
+                                              int32_t r0, int32_t nz) {
+    const int32_t tid = (int32_t)threadIdx.x, bd = (int32_t)blockDim.x;
+    auto copy_in = [&](uint32_t* rr, const uint32_t* gr, int32_t k0, int32_t k1) {
+        for (int32_t kb = k0 + tid; kb < k1; kb += 8 * bd) {
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = gr[min(kb + u * bd, k1 - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (kb + u * bd < k1) rr[kb + u * bd] = v[u];
+        }
+    };
+#pragma unroll
+    for (int hh = 0; hh < RW; ++hh) {
+        uint32_t* rr = ring + (size_t)hh * D * B;
+        const uint32_t* gr = gring + (size_t)hh * D * B;
+        const int32_t e = r0 + nz;
+        const int32_t z0 = (e <= D) ? r0 : 0, z1 = (e <= D) ? e : e - D;   // zero rows [z0, z1)
+        if (e <= D) {
+            copy_in(rr, gr, 0, z0 * B);
+            for (int32_t k = z0 * B + tid; k < z1 * B; k += bd) rr[k] = 0u;
+            copy_in(rr, gr, z1 * B, D * B);
+        } else {
+            for (int32_t k = tid; k < z1 * B; k += bd) rr[k] = 0u;
+            copy_in(rr, gr, z1 * B, r0 * B);
+            for (int32_t k = r0 * B + tid; k < D * B; k += bd) rr[k] = 0u;
+        }
+    }
+}
+
+// LDS -> the tile's ring slice, except the min(nsteps, D) rows from rcur that
+// this window consumed (cleared, or never read for neurons that do not integrate).
+template <int RW>
+__device__ __forceinline__ void ring_slice_out(const uint32_t* ring, uint32_t* gring, int32_t D, int32_t B,
+                                               int32_t rcur, int32_t nsteps) {
+    const int32_t tid = (int32_t)threadIdx.x, bd = (int32_t)blockDim.x;
+    const int32_t e = rcur + min(nsteps, D);
+#pragma unroll
+    for (int hh = 0; hh < RW; ++hh) {
+        const uint32_t* rr = ring + (size_t)hh * D * B;
+        uint32_t* gr = gring + (size_t)hh * D * B;
+        if (e <= D) {            // rows [rcur, e) consumed
+            for (int32_t k = tid; k < rcur * B; k += bd) gr[k] = rr[k];
+            for (int32_t k = e * B + tid; k < D * B; k += bd) gr[k] = rr[k];
+        } else {                 // rows [rcur, D) and [0, e - D) consumed
+            for (int32_t k = (e - D) * B + tid; k < rcur * B; k += bd) gr[k] = rr[k];
+        }
+    }
+}
+
 // One window of the fused tiled step (CLASS format).  Workgroup g (targets
 // [tile_lo[g], tile_lo[g+1]) of segment tile_seg[g]):
 //   1. loads its ring slice ring_t[g] (D x B words) into LDS;
@@ -453,6 +517,28 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* wsum) 
 // formats: local target (11 bits) | delay-1 << 11 | BF16 (kTBF16) or FP8
 // (kTFP8, per-source scale) weight bits << 16, or the weight in w_t (kTF32);
 // arrivals add rintf(w * scale) into int32 exc / inh rings, as k_prop does.
+//
+// kTStdp: the kTF32 word with a plastic flag at bit 16 and pair-based STDP
+// (mgcore.cu:346-412) on w_t.  A synapse k -> j lives in the tile of its
+// target, so every read and update of it happens in one workgroup.  The host
+// caps the window at the shortest inter-spike interval of the plastic
+// synapses' ends: k and j fire at most once in a window.  Axon updates (k's
+// spike at t2: dt = t2 - j's last spike at t2) are made where the weight is
+// read for propagation.  Dendrite updates (j's spike at t1: dt = k's last
+// spike at t1 - t1) are deferred: neuron j keeps its last kStdpHist spike
+// steps, and a synapse is brought up to date, replaying j's spikes in step
+// order, (a) where it is read at k's next spike t2 — j's spikes after
+// max(k's previous spike, tf[j]) and before t2, before the weight is
+// propagated, which is the reference's order (each step: propagation, then
+// STDP) — and (b) when j is flushed: after j has fired kStdpHist - 1 times since
+// tf[j], all of j's plastic in-synapses are brought up to date through the
+// previous window and tf[j] moves there, so the ring always holds the spikes
+// needed.  k's last spike 0 counts as "never" (a spike at step 0 then still
+// replays; dt = 0 only clips, which is idempotent).  Window k's neurons read
+// their last spike steps from last3 buffer k % 3 and write them to (k+1) % 3.
+// With a.apply_only the launch brings every plastic synapse up to date through
+// the pending window in a.w_t (a copy), touching nothing else: the weights
+// the reference holds after the last step run.
 template <int CH, int WK>
 __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_t wk, int32_t nsteps) {
     extern __shared__ uint32_t lds[];
@@ -466,13 +552,23 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const int32_t D = a.D, B = a.B, g = (int32_t)blockIdx.x, bd = (int32_t)blockDim.x;
     const int32_t tid = (int32_t)threadIdx.x;
     constexpr bool kCls = WK == kTClass || WK == kTClass16;  // CLASS arrival counts
+    constexpr bool kPl = WK == kTStdp;                      // STDP on w_t
+    constexpr bool kAux = WK == kTFP8 || WK == kTBF16 || kPl;
     constexpr int RW = kCls ? 1 : 2;                        // ring words per (slot, target)
     uint32_t* ring = lds;                                   // [RW][D][B]
     uint32_t* sp_beg = ring + (size_t)RW * D * B;           // [kTiledBatch] first CSR slot
     uint32_t* sp_len = sp_beg + kTiledBatch;                // len | slot-of-spike << 26
     uint32_t* sp_start = sp_len + kTiledBatch;              // exclusive prefix of work items
     uint16_t* itm = reinterpret_cast<uint16_t*>(sp_start + kTiledBatch);   // [kTiledItems] spike of item
-    float* sp_rs = reinterpret_cast<float*>(itm + kTiledItems);            // [kTiledBatch] BF16/FP8 row scale
+    // [kTiledBatch] BF16/FP8: row scale bits; kTStdp: the source's last spike before the window
+    uint32_t* sp_aux = reinterpret_cast<uint32_t*>(itm + kTiledItems);
+    // kTStdp, per tile neuron: spike history [B][kStdpHist], fire count, tf, neurons to flush
+    int32_t* hL = reinterpret_cast<int32_t*>(sp_aux + kTiledBatch);
+    int32_t* cL = hL + (size_t)B * kStdpHist;
+    int32_t* tfL = cL + B;
+    uint32_t* fl = reinterpret_cast<uint32_t*>(tfL + B);
+    __shared__ uint32_t nfl;
+    const bool full = !kPl || a.apply_only == 0;
     Ctl* ctl = a.ctl;
     const int64_t s0 = ctl->base_step + off;
     const int64_t kwin = ctl->base_window + wk;
@@ -487,11 +583,23 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const size_t lbase = (size_t)slo * (size_t)a.window;    // segment's part of each spike list
     uint32_t* tcnt = a.tcnt + q;                            // tcnt[b * nseg]
     const int32_t nseg = a.nseg;
-    if (lt == 0 && tid == 0) tcnt[bnext * nseg] = 0u;
-    if (g == 0 && tid == 0) ctl->tbase[bcur] = s0;
+    if (full && lt == 0 && tid == 0) tcnt[bnext * nseg] = 0u;
+    if (full && g == 0 && tid == 0) ctl->tbase[bcur] = s0;
     if (tid == 0) {
         se_blk = 0ull;
         lsp_n = 0u;
+        nfl = 0u;
+    }
+    const int32_t* lastA = kPl ? a.last3 + (size_t)bprev * a.last_stride : nullptr;   // window k-1 start
+    const int32_t* lastB = kPl ? a.last3 + (size_t)bcur * a.last_stride : a.last;     // window k-1 end
+    int32_t* lastC = kPl ? a.last3 + (size_t)bnext * a.last_stride : a.last;          // window k end
+    if (kPl) {
+        for (int32_t j = tid; j < tn; j += bd) {
+            cL[j] = a.cnt[t0 + j];
+            tfL[j] = a.tf[t0 + j];
+        }
+        const int32_t* hg = a.hist + (size_t)t0 * kStdpHist;
+        for (int32_t k = tid; k < tn * kStdpHist; k += bd) hL[k] = hg[k];
     }
     if (kCls && tid < 4 * kClassCache) {
         const int32_t c = tid / (2 * kClassCache), side = (tid / kClassCache) & 1, k = tid % kClassCache;
@@ -510,8 +618,9 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const uint32_t E = tcnt[bprev * nseg];
     const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride + lbase;
     // metadata of list entry e: first slot of its sub-row in this tile, length |
-    // ring slot of its spike step << 26, BF16 / FP8 row scale
-    auto meta = [&](uint32_t e, uint32_t& beg, uint32_t& lm, float& rsc) {
+    // ring slot of its spike step << 26, BF16 / FP8 row scale bits or (kTStdp)
+    // the source's last spike before the window
+    auto meta = [&](uint32_t e, uint32_t& beg, uint32_t& lm, uint32_t& aux) {
         const uint32_t ent = L[e];
         const uint32_t src = ent & kIdMask;
         const size_t ob = obase + (size_t)(src - (uint32_t)slo) * G1;
@@ -521,42 +630,15 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         const int32_t sD = (int32_t)((sprev + (int64_t)(ent >> 26)) % D);
         beg = (uint32_t)a.row_ptr[src] + o0;
         lm = (o1 - o0) | ((uint32_t)sD << 26);
-        rsc = (WK == kTFP8 || WK == kTBF16) ? a.row_scale[src] : 1.0f;
+        if (WK == kTFP8 || WK == kTBF16) aux = __float_as_uint(a.row_scale[src]);
+        else if (kPl) aux = (uint32_t)lastA[src];
+        else aux = 0u;
     };
     // the first batch's metadata loads are issued before the ring copy, so that
     // their latency overlaps it
-    uint32_t pbeg = 0u, plm = 0u;
-    float prs = 1.0f;
-    if ((uint32_t)tid < min((uint32_t)bd, E)) meta((uint32_t)tid, pbeg, plm, prs);
-    // copies [k0, k1) of a slice to LDS, 8 loads per thread in flight (clamped
-    // indices keep every load in bounds; the extra values are not stored)
-    auto copy_in = [&](uint32_t* rr, const uint32_t* gr, int32_t k0, int32_t k1) {
-        for (int32_t kb = k0 + tid; kb < k1; kb += 8 * bd) {
-            uint32_t v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = gr[min(kb + u * bd, k1 - 1)];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (kb + u * bd < k1) rr[kb + u * bd] = v[u];
-        }
-    };
-#pragma unroll
-    for (int hh = 0; hh < RW; ++hh) {
-        // rows [rprev, rprev + nprev) mod D are zero, the rest are loaded
-        uint32_t* rr = ring + (size_t)hh * D * B;
-        const uint32_t* gr = gring + (size_t)hh * D * B;
-        const int32_t e = rprev + nprev;
-        const int32_t z0 = (e <= D) ? rprev : 0, z1 = (e <= D) ? e : e - D;   // zero rows [z0, z1)
-        if (e <= D) {
-            copy_in(rr, gr, 0, z0 * B);
-            for (int32_t k = z0 * B + tid; k < z1 * B; k += bd) rr[k] = 0u;
-            copy_in(rr, gr, z1 * B, D * B);
-        } else {
-            for (int32_t k = tid; k < z1 * B; k += bd) rr[k] = 0u;
-            copy_in(rr, gr, z1 * B, rprev * B);
-            for (int32_t k = rprev * B + tid; k < D * B; k += bd) rr[k] = 0u;
-        }
-    }
+    uint32_t pbeg = 0u, plm = 0u, paux = 0u;
+    if ((uint32_t)tid < min((uint32_t)bd, E)) meta((uint32_t)tid, pbeg, plm, paux);
+    if (full) ring_slice_in<RW>(ring, gring, D, B, rprev, nprev);
     __syncthreads();
 
     // 2. previous window's spikes into the tile
@@ -564,12 +646,11 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         const uint32_t nb = min((uint32_t)bd, E - b0);
         uint32_t items = 0u;
         if ((uint32_t)tid < nb) {
-            uint32_t beg = pbeg, lm = plm;
-            float rsc = prs;
-            if (b0 > 0) meta(b0 + (uint32_t)tid, beg, lm, rsc);
+            uint32_t beg = pbeg, lm = plm, aux = paux;
+            if (b0 > 0) meta(b0 + (uint32_t)tid, beg, lm, aux);
             sp_beg[tid] = beg;
             sp_len[tid] = lm;
-            if (WK == kTFP8 || WK == kTBF16) sp_rs[tid] = rsc;
+            if (kAux) sp_aux[tid] = aux;
             items = ((lm & kIdMask) + CH - 1) / CH;
         }
         const uint32_t incl = block_incl_scan(items, wsum);
@@ -585,13 +666,14 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             __syncthreads();
             // U items per thread in flight: U * CH synapse words loaded before the
             // first LDS atomic
-            constexpr int U = CH >= 16 ? 1 : 16 / CH;
+            // (kTStdp with single-synapse items: 8, which keeps it within the registers)
+            constexpr int U = CH >= 16 ? 1 : ((kPl && CH == 1) ? 8 : 16 / CH);
             for (uint32_t it = (uint32_t)tid; it < rn; it += (uint32_t)(U * bd)) {
                 // branch-free body, so that all U * CH loads are in flight before the
                 // first wait: lanes past the round's items re-read the last item and
                 // elements past an item's end re-read its last element; both add 0
                 uint32_t cnt[U], kb[U];
-                int32_t sD[U];
+                int32_t sD[U], t2[U], la[U];
                 float rs[U];
 #pragma unroll
                 for (int h = 0; h < U; ++h) {
@@ -604,7 +686,14 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                     sD[h] = (int32_t)(meta >> 26);
                     kb[h] = sp_beg[j] + c * CH;
                     cnt[h] = (q0 < rn) ? min((uint32_t)CH, len - c * CH) : 0u;
-                    rs[h] = (WK == kTFP8 || WK == kTBF16) ? sp_rs[j] : 1.0f;
+                    rs[h] = (WK == kTFP8 || WK == kTBF16) ? __uint_as_float(sp_aux[j]) : 1.0f;
+                    if (kPl) {
+                        // the spike's step: the window is shorter than the ring, so its
+                        // slot determines it
+                        const int32_t dd = sD[h] - rprev;
+                        t2[h] = (int32_t)sprev + dd + (dd < 0 ? D : 0);
+                        la[h] = (int32_t)sp_aux[j];
+                    }
                 }
                 uint32_t x[U][CH];
                 float wv[U][CH];
@@ -617,6 +706,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                         if (WK == kTClass16) x[h][u] = __builtin_nontemporal_load(&a.syn16[k]);
                         else x[h][u] = __builtin_nontemporal_load(&a.syn_t[k]);
                         if (WK == kTF32) wv[h][u] = __builtin_nontemporal_load(&a.w_t[k]);
+                        else if (kPl) wv[h][u] = a.w_t[k];           // rewritten below: cached load
                     }
                 }
 #pragma unroll
@@ -647,16 +737,101 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             if (WK == kTBF16) w = bf16_to_f32((uint16_t)(xw >> 16)) * rs[h];
                             else if (WK == kTFP8) w = fp8e4m3_to_f32((uint8_t)(xw >> 16)) * rs[h];
                             else w = wv[h][u];
+                            const bool pl = kPl && ((xw >> 16) & 1u);
+                            int32_t lj = 0;                            // j's last spike at t2
+                            if (pl) {
+                                // j's history, newest first: its last spike at or before t2,
+                                // then the deferred dendrite updates (lo, t2), replayed oldest first
+                                const int32_t* hj = hL + loc * kStdpHist;
+                                const int32_t c = cL[loc];
+                                const int32_t lo = max(la[h] == 0 ? -1 : la[h], tfL[loc]);
+                                const int32_t nm = min(c, kStdpHist);
+                                int32_t mlo = nm, mhi = -1;
+                                bool found = false;
+                                for (int32_t m = 0; m < nm; ++m) {
+                                    const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
+                                    if (e > t2[h]) continue;           // j fired after t2 in that window
+                                    if (!found) {
+                                        lj = e;
+                                        found = true;
+                                    }
+                                    if (e == t2[h]) continue;
+                                    if (e <= lo) break;
+                                    mlo = min(mlo, m);
+                                    mhi = m;
+                                }
+                                for (int32_t m = mhi; m >= mlo; --m) {
+                                    const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
+                                    w = clipw(w + stdp_dw(a.ltp, a.ltd, a.lut_n, la[h] - e), a.w_max, a.w_min);
+                                }
+                            }
                             const bool exc = w > 0.f;
                             const int32_t v = (int32_t)rintf(w * (exc ? a.scale_e : a.scale_i));
                             const int32_t at = idle ? (tid % B) : ((exc ? 0 : D) + slot) * B + loc;
-                            atomicAdd(reinterpret_cast<int32_t*>(ring) + at, valid ? v : 0);
+                            if (full) atomicAdd(reinterpret_cast<int32_t*>(ring) + at, valid ? v : 0);
+                            if (pl && valid)                           // axon update of t2
+                                a.w_t[kb[h] + (uint32_t)u] =
+                                    clipw(w + stdp_dw(a.ltp, a.ltd, a.lut_n, t2[h] - lj), a.w_max, a.w_min);
                         }
                     }
             }
             __syncthreads();
         }
         __syncthreads();   // wsum / sp_* are rewritten by the next batch
+    }
+    if (kPl) {
+        // 2b. flush: neurons that fired kStdpHist - 1 times since tf (all that fired
+        // since tf for apply_only) bring their plastic in-synapses up to date
+        // through the previous window, 8 in-synapses per thread in flight
+        for (int32_t j = tid; j < tn; j += bd) {
+            const int32_t n_since = cL[j] - a.cntf[t0 + j];
+            if (full ? n_since >= kStdpHist - 1 : n_since > 0) fl[atomicAdd(&nfl, 1u)] = (uint32_t)j;
+        }
+        __syncthreads();
+        const int32_t hz = (int32_t)(s0 - 1);               // flush horizon: end of the previous window
+        const uint32_t nf = nfl;
+        for (uint32_t fi = 0; fi < nf; ++fi) {
+            const int32_t jl = (int32_t)fl[fi];
+            const int32_t* hj = hL + jl * kStdpHist;
+            const int32_t c = cL[jl], tfj = tfL[jl], nm = min(c, kStdpHist);
+            const int64_t pb = a.din_ptr[t0 + jl], pe = a.din_ptr[t0 + jl + 1];
+            for (int64_t p0 = pb + tid; p0 < pe; p0 += 8 * (int64_t)bd) {
+                uint32_t kk[8];
+                int32_t sr[8], lk[8];
+                float wv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int64_t pp = min(p0 + u * (int64_t)bd, pe - 1);
+                    kk[u] = a.din_pos[pp];
+                    sr[u] = a.din_src[pp];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    lk[u] = lastB[sr[u]];
+                    wv[u] = a.w_t[kk[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (p0 + u * (int64_t)bd >= pe) continue;
+                    // replay j's spikes in (max(k's last spike, tf), horizon], oldest first
+                    const int32_t lo = max(lk[u] == 0 ? -1 : lk[u], tfj);
+                    int32_t m = 0;
+                    while (m < nm && hj[(c - 1 - m) & (kStdpHist - 1)] > lo) ++m;
+                    if (m == 0) continue;
+                    float w = wv[u];
+                    for (--m; m >= 0; --m) {
+                        const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
+                        if (e <= hz) w = clipw(w + stdp_dw(a.ltp, a.ltd, a.lut_n, lk[u] - e), a.w_max, a.w_min);
+                    }
+                    a.w_t[kk[u]] = w;
+                }
+            }
+            if (full && tid == 0) {
+                a.tf[t0 + jl] = hz;
+                a.cntf[t0 + jl] = c;
+            }
+        }
+        if (!full) return;
     }
 
     // 3. neurons of the tile through this window's steps.  Spikes go to an LDS
@@ -673,7 +848,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         const int32_t i = t0 + j;
         const int type = active ? (int)a.type[i] : T_RAW;
         NState st;
-        if (active) neuron_load(a, i, type, kCls, st);
+        if (active) neuron_load(a, i, type, kCls, st, lastB);
         const bool integ = integrates(type);
         int32_t outdeg = 0;
         if (active && type != T_RAW && type != T_LIF0)
@@ -725,6 +900,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                 }
                 fired = integrate_step(type, st, s, be, bi);
             }
+            if (kPl && fired) a.hist[(size_t)i * kStdpHist + ((st.cnt - 1) & (kStdpHist - 1))] = s;
             const uint64_t m = __ballot(fired);
             if (m != 0ull) {
                 const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
@@ -761,7 +937,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             }
             slot = (slot + 1 == D) ? 0 : slot + 1;
         }
-        if (active) neuron_store(a, i, type, st);
+        if (active) neuron_store(a, i, type, st, lastC);
     }
     if (se_local) atomicAdd(&se_blk, se_local);
     __syncthreads();
@@ -786,19 +962,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     }
     // 4. LDS -> ring slice, except the slots this window consumed (cleared above,
     // or never read for neurons that do not integrate)
-    const int32_t rcur = (int32_t)(s0 % D);
-    const int32_t e = rcur + min(nsteps, D);
-#pragma unroll
-    for (int hh = 0; hh < RW; ++hh) {
-        const uint32_t* rr = ring + (size_t)hh * D * B;
-        uint32_t* gr = gring + (size_t)hh * D * B;
-        if (e <= D) {            // rows [rcur, e) consumed
-            for (int32_t k = tid; k < rcur * B; k += bd) gr[k] = rr[k];
-            for (int32_t k = e * B + tid; k < D * B; k += bd) gr[k] = rr[k];
-        } else {                 // rows [rcur, D) and [0, e - D) consumed
-            for (int32_t k = (e - D) * B + tid; k < rcur * B; k += bd) gr[k] = rr[k];
-        }
-    }
+    ring_slice_out<RW>(ring, gring, D, B, (int32_t)(s0 % D), nsteps);
 }
 
 __global__ void k_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows) {
@@ -851,7 +1015,8 @@ size_t tiled_lds_bytes(int D, int B, int kind) {
     const size_t rw = (kind == kTClass || kind == kTClass16) ? 1 : 2;
     return sizeof(uint32_t) * (rw * (size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
            sizeof(uint16_t) * (size_t)kTiledItems +
-           ((kind == kTFP8 || kind == kTBF16) ? sizeof(float) * kTiledBatch : 0);
+           ((kind == kTFP8 || kind == kTBF16 || kind == kTStdp) ? sizeof(uint32_t) * kTiledBatch : 0) +
+           (kind == kTStdp ? (kStdpHist + 3) * sizeof(int32_t) * (size_t)B : 0);
 }
 
 namespace {
@@ -901,6 +1066,7 @@ hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nst
         case kTFP8: launch_tiled_kind<kTFP8>(a, off, wk, nsteps, block, lds, st); break;
         case kTF32: launch_tiled_kind<kTF32>(a, off, wk, nsteps, block, lds, st); break;
         case kTClass16: launch_tiled_kind<kTClass16>(a, off, wk, nsteps, block, lds, st); break;
+        case kTStdp: launch_tiled_kind<kTStdp>(a, off, wk, nsteps, block, lds, st); break;
         default: launch_tiled_kind<kTClass>(a, off, wk, nsteps, block, lds, st); break;
     }
     return hipGetLastError();
@@ -912,6 +1078,7 @@ hipError_t set_tiled_lds_limit(int kind, size_t bytes) {
         case kTFP8: return set_lds_kind<kTFP8>(bytes);
         case kTF32: return set_lds_kind<kTF32>(bytes);
         case kTClass16: return set_lds_kind<kTClass16>(bytes);
+        case kTStdp: return set_lds_kind<kTStdp>(bytes);
         default: return set_lds_kind<kTClass>(bytes);
     }
 }

@@ -56,15 +56,16 @@ def test_resume_bit_identical(prop, wf, graph):
                                           record_capacity=1_000_000))
 
 
-def test_resume_stdp():
-    _say("case: fixed in-degree net with E->E STDP, checkpoint mid-run")
+@pytest.mark.parametrize("prop", [N.PROP_DIRECT, N.PROP_TILED])
+def test_resume_stdp(prop):
+    _say(f"case: fixed in-degree net with E->E STDP, checkpoint mid-run, propagation={prop}")
     net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
     syn = H.host_synapses(net, seed=6)
     sc = M.stdp_consts_config4(J=1e-4)
 
     def make():
         return H.gpu_from_list(net, syn, seed=6, stdp=True, stdpconst=sc, weight_format=N.WF_F32,
-                               record_capacity=1_000_000)
+                               record_capacity=1_000_000, propagation=prop)
     _check_resume(make)
 
 

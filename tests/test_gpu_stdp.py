@@ -7,6 +7,12 @@ benchmark's constants (A+ = 0.01 W_max, A- = -0.0105 W_max, tau = 200 steps,
 W in [0, 2J]).  The oracle processes spikers sequentially in id order; the GPU
 updates all spikers' rows in parallel: each synapse receives at most one
 non-zero update per step, so both give the same weights.
+
+Both GPU paths are checked: the direct kernels (one step per window) and the
+fused tiled kernel, whose windows span up to min(min_delay, shortest
+inter-spike interval of the plastic synapses' ends) steps — one step when
+Poisson neurons have plastic synapses (a), ten steps for E->E plasticity with
+a 2 ms refractory period and 1-2 ms delays (b).
 """
 import sys
 
@@ -21,10 +27,12 @@ from tests import netmodels as M
 pytestmark = pytest.mark.gpu
 
 
-def _run(net, syn, sc, nsteps, seed, wf):
-    g = H.gpu_from_list(net, syn, seed=seed, stdp=True, stdpconst=sc, weight_format=wf, record_capacity=4_000_000)
+def _run(net, syn, sc, nsteps, seed, wf, prop=N.PROP_DIRECT, window=1):
+    g = H.gpu_from_list(net, syn, seed=seed, stdp=True, stdpconst=sc, weight_format=wf, record_capacity=4_000_000,
+                        propagation=prop)
     info = g.info()
-    assert info["window"] == 1
+    assert info["window"] == window
+    assert info["propagation"] == (N.PROP_DIRECT if prop == N.PROP_DIRECT else N.PROP_TILED)
     _, o = H.oracle(net, syn, seed=seed, acc=po.ACC_FIXED, stdp=True, stdpconst=sc,
                     scale=(info["fix_scale_exc"], info["fix_scale_inh"]))
     w0 = g.weights()
@@ -43,19 +51,21 @@ def _run(net, syn, sc, nsteps, seed, wf):
     return changed, sum(len(x) for x in want)
 
 
-def test_reference_global_stdp():
-    print("case: make_brunel(2e5) global STDP, default STDPCONSTS", flush=True)
+@pytest.mark.parametrize("prop", [N.PROP_DIRECT, N.PROP_TILED])
+def test_reference_global_stdp(prop):
+    print(f"case: make_brunel(2e5) global STDP, default STDPCONSTS, propagation={prop}", flush=True)
     sys.stdout.flush()
     net = M.brunel_ref(200_000, rate=0.01)
     syn = H.host_synapses(net, mt_seed=31)
     syn.plastic[:] = 1
-    changed, n = _run(net, syn, M.STDP_DEFAULT, 600, 3, N.WF_F32)
+    changed, n = _run(net, syn, M.STDP_DEFAULT, 600, 3, N.WF_F32, prop=prop, window=1)
     assert changed > 1000 and n > 1000
 
 
-@pytest.mark.parametrize("wf", [N.WF_F32, N.WF_BF16])
-def test_ee_stdp_benchmark_constants(wf):
-    print(f"case: fixed in-degree net, E->E STDP, weight_format={wf}", flush=True)
+@pytest.mark.parametrize("wf,prop", [(N.WF_F32, N.PROP_DIRECT), (N.WF_BF16, N.PROP_DIRECT),
+                                     (N.WF_F32, N.PROP_TILED), (N.WF_F32, N.PROP_AUTO)])
+def test_ee_stdp_benchmark_constants(wf, prop):
+    print(f"case: fixed in-degree net, E->E STDP, weight_format={wf}, propagation={prop}", flush=True)
     sys.stdout.flush()
     net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
     syn = H.host_synapses(net, seed=6)
@@ -72,5 +82,41 @@ def test_ee_stdp_benchmark_constants(wf):
         u = x.view(np.uint32).astype(np.uint64)
         r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
         syn.weight[stat] = (r * sc_row).astype(np.float32)[stat]
-    changed, n = _run(net, syn, sc, 800, 6, wf)
+    window = 1 if prop == N.PROP_DIRECT else 10     # min_delay 10, refractory interval 20
+    changed, n = _run(net, syn, sc, 800, 6, wf, prop=prop, window=window)
     assert changed > 100 and n > 1000
+
+
+@pytest.mark.parametrize("tau_ref,window", [(0.002, 10), (0.0006, 6)])
+def test_fused_stdp_weights_between_runs(tau_ref, window):
+    """Fused path: weights read between runs (the pending window's updates applied to a
+    copy) equal the oracle's at every read, and reading them changes nothing; a 0.6 ms
+    refractory period (inter-spike interval >= 6 steps) caps the window below min_delay."""
+    print(f"case: E->E STDP, tau_ref={tau_ref}, runs of 257 / 300 / 243 steps", flush=True)
+    sys.stdout.flush()
+    net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
+    net.lifconst = M.gen_life_const(0.0, 0.0, 0.010, 0.0, 0.0, tau_ref, 1.0, 1.0, 0.020, 0.0)
+    syn = H.host_synapses(net, seed=9)
+    sc = M.stdp_consts_config4(J=1e-4)
+    g = H.gpu_from_list(net, syn, seed=9, stdp=True, stdpconst=sc, weight_format=N.WF_F32,
+                        record_capacity=4_000_000, propagation=N.PROP_TILED)
+    info = g.info()
+    assert info["window"] == window
+    _, o = H.oracle(net, syn, seed=9, acc=po.ACC_FIXED, stdp=True, stdpconst=sc,
+                    scale=(info["fix_scale_exc"], info["fix_scale_inh"]))
+    w0 = g.weights()
+    done, want = 0, []
+    for n in (257, 300, 243):
+        g.run(n)
+        want += o.run(n)
+        done += n
+        wo = o.weights().astype(np.float32)
+        assert np.array_equal(g.weights().view(np.uint32), wo.view(np.uint32)), f"weights differ after {done}"
+        assert np.array_equal(g.weights().view(np.uint32), wo.view(np.uint32))   # a second read agrees
+    st, ids = g.spikes()
+    assert H.first_mismatch(H.split_steps(st, ids, done), want) < 0
+    assert np.array_equal(g.fire_counts(), o.state(po.F_CNT).astype(np.int32))
+    assert np.array_equal(g.state(N.F_LAST), o.state(po.F_LAST).astype(np.int32))
+    assert int(np.sum(g.weights() != w0)) > 100
+    g.close()
+    o.close()
