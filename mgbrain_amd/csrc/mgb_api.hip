@@ -83,6 +83,7 @@ struct mgb_sim {
     float* tbl_w = nullptr;
     float *ltp = nullptr, *ltd = nullptr;
     int32_t lut_n = 1;
+    int32_t lut_lds = 0;               // kTStdp: table entries the fused kernel keeps in LDS
     uint32_t* win_buf = nullptr;
     uint32_t win_stride = 0;
     uint64_t* rec = nullptr;
@@ -101,6 +102,8 @@ struct mgb_sim {
     int tkind = kTClass;               // synapse word kind of the fused kernel
     uint32_t* tperm = nullptr;         // kTStdp: CSR slot of each tile-order position
     int32_t *hist = nullptr, *tf = nullptr, *cntf = nullptr;   // kTStdp: deferred dendrite updates
+    int32_t* flist = nullptr;          // kTStdp: [2][npad][3] neurons to flush
+    uint32_t* fcnt = nullptr;          // [2]
     int32_t stdp_isi = 1;              // kTStdp: shortest inter-spike interval of plastic synapses' ends
     uint32_t* syn_t = nullptr;
     uint16_t* syn16 = nullptr;         // kTClass16 words in tile order
@@ -818,6 +821,9 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
         MGB_HIP_TRY(hipMemsetAsync(s->hist, 0, 4 * np * kStdpHist, st));
         MGB_HIP_TRY(hipMemsetAsync(s->tf, 0xFF, 4 * np, st));     // -1: no update deferred yet
         MGB_HIP_TRY(hipMemsetAsync(s->cntf, 0, 4 * np, st));
+        MGB_TRY(dalloc_n(s, &s->flist, 2 * 3 * np));
+        MGB_TRY(dalloc_n(s, &s->fcnt, 2));
+        MGB_HIP_TRY(hipMemsetAsync(s->fcnt, 0, 8, st));
     }
     MGB_HIP_TRY(hipMemsetAsync(s->cnt, 0, 4 * np, st));
     // constants: LIFECONSTS + refractory steps (mgcore.cu:153)
@@ -878,6 +884,16 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
         MGB_TRY(dalloc_n(s, &s->ltd, ltd.size()));
         MGB_HIP_TRY(hipMemcpyAsync(s->ltp, ltp.data(), 4 * ltp.size(), hipMemcpyHostToDevice, st));
         MGB_HIP_TRY(hipMemcpyAsync(s->ltd, ltd.data(), 4 * ltd.size(), hipMemcpyHostToDevice, st));
+        if (s->tiled && s->tkind == kTStdp) {
+            // the LDS left beside the tile's data holds the tables' leading entries
+            int lds_max = 65536;
+            (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
+            lds_max = std::min(lds_max, 160 * 1024);
+            const int64_t room = (int64_t)lds_max - kTiledStaticLds - (int64_t)tiled_lds_bytes(s->D, s->B, s->tkind);
+            s->lut_lds = (int32_t)std::max<int64_t>(0, std::min<int64_t>(s->lut_n, room / 8));
+            const size_t lds = tiled_lds_bytes(s->D, s->B, s->tkind, s->lut_lds);
+            if (lds > 65536) MGB_HIP_TRY(set_tiled_lds_limit(s->tkind, lds));
+        }
     }
     s->use_graph = o ? o->use_graph : 0;
     s->profile = o ? o->profile : 0;
@@ -945,8 +961,9 @@ TiledArgs tiled_args(const mgb_sim* s) {
     if (s->tkind == kTStdp) {
         a.last3 = s->last; a.last_stride = (size_t)s->npad;
         a.hist = s->hist; a.tf = s->tf; a.cntf = s->cntf;
+        a.flist = s->flist; a.fcnt = s->fcnt; a.flist_cap = (uint32_t)s->npad;
         a.din_ptr = s->in_ptr; a.din_pos = s->in_pos; a.din_src = s->in_src;
-        a.ltp = s->ltp; a.ltd = s->ltd; a.lut_n = s->lut_n;
+        a.ltp = s->ltp; a.ltd = s->ltd; a.lut_n = s->lut_n; a.lut_lds = s->lut_lds;
         a.w_max = s->sc[4]; a.w_min = s->sc[5];
     }
     return a;
@@ -1012,6 +1029,13 @@ mgb_status issue_windows(mgb_sim* s, int32_t nsteps, bool finalize, bool capturi
             MGB_HIP_TRY(launch_tiled(ta, off, wk, nst, s->tblock, s->stream));
             MGB_TRY(g1.end());
             s->launches += 1;
+            if (s->tkind == kTStdp) {
+                SpanGuard g2{s, kSpanStdp, prof};
+                MGB_TRY(g2.begin());
+                MGB_HIP_TRY(launch_stdp_flush(ta, off, wk, s->stream));
+                MGB_TRY(g2.end());
+                s->launches += 1;
+            }
             off += nst;
             ++wk;
         }
@@ -1688,6 +1712,7 @@ std::vector<CkptSeg> ckpt_segments(const mgb_sim* s, uint64_t n_rec) {
         v.push_back({s->hist, 4 * np * kStdpHist});
         v.push_back({s->tf, 4 * np});
         v.push_back({s->cntf, 4 * np});
+        v.push_back({s->fcnt, 8});
     }
     if (n_rec) v.push_back({s->rec, 8 * (size_t)n_rec});
     return v;
@@ -1808,6 +1833,9 @@ mgb_status current_weights(mgb_sim* s) {
     ta.apply_only = 1;
     e = hipMemcpyAsync(tmp, s->w_t, S * 4, hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = launch_tiled(ta, 0, 0, 1, s->tblock, s->stream);
+    if (e == hipSuccess) e = launch_stdp_flush(ta, 0, 0, s->stream);
+    // the list this used is empty before the next window appends to it
+    if (e == hipSuccess) e = hipMemsetAsync(s->fcnt, 0, 8, s->stream);
     if (e == hipSuccess) e = scatter_weights(s->tperm, tmp, (uint64_t)S, s->w32, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     const hipError_t f = hipFree(tmp);

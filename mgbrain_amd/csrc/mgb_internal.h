@@ -202,12 +202,18 @@ struct TiledArgs {
     int32_t* hist;              // [n][kStdpHist]
     int32_t* tf;                // [n]
     int32_t* cntf;              // [n]
+    // neurons to flush after window k: list k % 2 of (neuron, fire count, old tf)
+    // triples, fcnt[k % 2] entries; window k clears fcnt[(k + 1) % 2]
+    int32_t* flist;             // [2][flist_cap][3]
+    uint32_t* fcnt;             // [2]
+    uint32_t flist_cap;
     const int64_t* din_ptr;     // [n+1] plastic in-synapses of each target
     const uint32_t* din_pos;    // their tile-order positions
     const int32_t* din_src;     // their sources
     const float* ltp;           // ltp[k] = A_LTP * expf(-k / TAU_LTP)
     const float* ltd;           // ltd[k] = A_LTD * expf(-k / TAU_LTD)
     int32_t lut_n;
+    int32_t lut_lds;            // leading entries of each table copied to LDS
     float w_max, w_min;
     // kTStdp read-back: apply the pending window's STDP to w_t (a copy) and
     // touch nothing else — no ring, neuron state, lists or counters
@@ -229,10 +235,14 @@ constexpr int kTiledItems = 8192;   // work items staged in LDS per propagation 
 constexpr int kTiledStaticLds = 1024; // bytes reserved for the fused kernel's static LDS
 constexpr int kClassCache = 32;     // k-fold class sums cached in LDS (tables hold >= this many)
 constexpr int kStdpHist = 16;       // spike steps kept per neuron for deferred dendrite updates
-size_t tiled_lds_bytes(int D, int B, int kind);
+// Dynamic LDS of the fused kernel; kTStdp also caches lut_lds entries of each STDP table.
+size_t tiled_lds_bytes(int D, int B, int kind, int lut_lds = 0);
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st);
 hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
 hipError_t set_tiled_lds_limit(int kind, size_t bytes);
+// kTStdp: brings the plastic in-synapses of the neurons the fused kernel listed
+// in window base_window + wk up to date (flush of deferred dendrite updates).
+hipError_t launch_stdp_flush(const TiledArgs& a, int32_t off, int32_t wk, hipStream_t st);
 
 }  // namespace mgb

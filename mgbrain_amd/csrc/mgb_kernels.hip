@@ -559,15 +559,23 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     uint32_t* sp_beg = ring + (size_t)RW * D * B;           // [kTiledBatch] first CSR slot
     uint32_t* sp_len = sp_beg + kTiledBatch;                // len | slot-of-spike << 26
     uint32_t* sp_start = sp_len + kTiledBatch;              // exclusive prefix of work items
-    uint16_t* itm = reinterpret_cast<uint16_t*>(sp_start + kTiledBatch);   // [kTiledItems] spike of item
+    constexpr uint32_t kItems = kTiledItems;   // work items staged per round
+    uint16_t* itm = reinterpret_cast<uint16_t*>(sp_start + kTiledBatch);   // [kItems] spike of item
     // [kTiledBatch] BF16/FP8: row scale bits; kTStdp: the source's last spike before the window
-    uint32_t* sp_aux = reinterpret_cast<uint32_t*>(itm + kTiledItems);
-    // kTStdp, per tile neuron: spike history [B][kStdpHist], fire count, tf, neurons to flush
+    uint32_t* sp_aux = reinterpret_cast<uint32_t*>(itm + kItems);
+    // kTStdp, per tile neuron: spike history [B][kStdpHist], fire count, tf
     int32_t* hL = reinterpret_cast<int32_t*>(sp_aux + kTiledBatch);
     int32_t* cL = hL + (size_t)B * kStdpHist;
     int32_t* tfL = cL + B;
-    uint32_t* fl = reinterpret_cast<uint32_t*>(tfL + B);
-    __shared__ uint32_t nfl;
+    // the first lut_lds entries of the LTP and LTD tables
+    float* lutL = reinterpret_cast<float*>(tfL + B);
+    // STDP weight change for dt = t_pre - t_post (stdp_dw) from the LDS copy where it holds |dt|
+    auto dw = [&](int32_t dt) -> float {
+        if (dt == 0) return 0.0f;
+        const int32_t k = dt < 0 ? -dt : dt;
+        if (k < a.lut_lds) return lutL[(dt < 0 ? 0 : a.lut_lds) + k];
+        return (dt < 0 ? a.ltp : a.ltd)[min(k, a.lut_n - 1)];
+    };
     const bool full = !kPl || a.apply_only == 0;
     Ctl* ctl = a.ctl;
     const int64_t s0 = ctl->base_step + off;
@@ -584,11 +592,11 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     uint32_t* tcnt = a.tcnt + q;                            // tcnt[b * nseg]
     const int32_t nseg = a.nseg;
     if (full && lt == 0 && tid == 0) tcnt[bnext * nseg] = 0u;
+    if (kPl && full && g == 0 && tid == 0) a.fcnt[(kwin + 1) & 1] = 0u;
     if (full && g == 0 && tid == 0) ctl->tbase[bcur] = s0;
     if (tid == 0) {
         se_blk = 0ull;
         lsp_n = 0u;
-        nfl = 0u;
     }
     const int32_t* lastA = kPl ? a.last3 + (size_t)bprev * a.last_stride : nullptr;   // window k-1 start
     const int32_t* lastB = kPl ? a.last3 + (size_t)bcur * a.last_stride : a.last;     // window k-1 end
@@ -600,6 +608,10 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         }
         const int32_t* hg = a.hist + (size_t)t0 * kStdpHist;
         for (int32_t k = tid; k < tn * kStdpHist; k += bd) hL[k] = hg[k];
+        for (int32_t k = tid; k < a.lut_lds; k += bd) {
+            lutL[k] = a.ltp[k];
+            lutL[a.lut_lds + k] = a.ltd[k];
+        }
     }
     if (kCls && tid < 4 * kClassCache) {
         const int32_t c = tid / (2 * kClassCache), side = (tid / kClassCache) & 1, k = tid % kClassCache;
@@ -656,8 +668,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         const uint32_t incl = block_incl_scan(items, wsum);
         const uint32_t total = wsum[(blockDim.x >> 6) - 1];
         if ((uint32_t)tid < nb) sp_start[tid] = incl - items;
-        for (uint32_t r0 = 0; r0 < total; r0 += kTiledItems) {
-            const uint32_t rn = min((uint32_t)kTiledItems, total - r0);
+        for (uint32_t r0 = 0; r0 < total; r0 += kItems) {
+            const uint32_t rn = min(kItems, total - r0);
             // item table: each spike writes its own work items of this round
             if ((uint32_t)tid < nb) {
                 const uint32_t q0 = max(incl - items, r0), q1 = min(incl, r0 + rn);
@@ -666,8 +678,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             __syncthreads();
             // U items per thread in flight: U * CH synapse words loaded before the
             // first LDS atomic
-            // (kTStdp with single-synapse items: 8, which keeps it within the registers)
-            constexpr int U = CH >= 16 ? 1 : ((kPl && CH == 1) ? 8 : 16 / CH);
+            // (kTStdp: at most 8 synapses, whose STDP state then fits in the registers)
+            constexpr int U = CH >= 16 ? 1 : (kPl ? (CH >= 8 ? 1 : (CH == 1 ? 4 : 8 / CH)) : 16 / CH);
             for (uint32_t it = (uint32_t)tid; it < rn; it += (uint32_t)(U * bd)) {
                 // branch-free body, so that all U * CH loads are in flight before the
                 // first wait: lanes past the round's items re-read the last item and
@@ -709,6 +721,50 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                         else if (kPl) wv[h][u] = a.w_t[k];           // rewritten below: cached load
                     }
                 }
+                // kTStdp, per plastic element: j's history newest first gives its last spike
+                // at or before t2 (for the axon update) and the range [mlo, mhi] of
+                // deferred dendrite updates in (lo, t2); the axon table loads of all
+                // elements are then issued together
+                int32_t rng[U][CH];      // mlo | mhi << 8 (mhi < mlo: none); -1: not plastic
+                int32_t adt[U][CH];      // dt of the axon update
+                float ax[U][CH];
+                if (kPl) {
+#pragma unroll
+                    for (int h = 0; h < U; ++h)
+#pragma unroll
+                        for (int u = 0; u < CH; ++u) {
+                            const uint32_t xw = x[h][u];
+                            const int32_t loc = (int32_t)(xw & 0x7FFu);
+                            int32_t lj = 0, mlo = kStdpHist, mhi = 0;
+                            rng[h][u] = -1;
+                            if ((xw >> 16) & 1u) {
+                                const int32_t* hj = hL + loc * kStdpHist;
+                                const int32_t c = cL[loc];
+                                const int32_t lo = max(la[h] == 0 ? -1 : la[h], tfL[loc]);
+                                const int32_t nm = min(c, kStdpHist);
+                                bool found = false;
+                                for (int32_t m = 0; m < nm; ++m) {
+                                    const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
+                                    if (e > t2[h]) continue;           // j fired after t2 in that window
+                                    if (!found) {
+                                        lj = e;
+                                        found = true;
+                                    }
+                                    if (e == t2[h]) continue;
+                                    if (e <= lo) break;
+                                    mlo = min(mlo, m);
+                                    mhi = m;
+                                }
+                                rng[h][u] = mlo | (mhi << 8);
+                            }
+                            adt[h][u] = t2[h] - lj;
+                        }
+#pragma unroll
+                    for (int h = 0; h < U; ++h)
+#pragma unroll
+                        for (int u = 0; u < CH; ++u)
+                            ax[h][u] = rng[h][u] >= 0 ? dw(adt[h][u]) : 0.0f;
+                }
 #pragma unroll
                 for (int h = 0; h < U; ++h)
 #pragma unroll
@@ -737,32 +793,15 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             if (WK == kTBF16) w = bf16_to_f32((uint16_t)(xw >> 16)) * rs[h];
                             else if (WK == kTFP8) w = fp8e4m3_to_f32((uint8_t)(xw >> 16)) * rs[h];
                             else w = wv[h][u];
-                            const bool pl = kPl && ((xw >> 16) & 1u);
-                            int32_t lj = 0;                            // j's last spike at t2
+                            const bool pl = kPl && rng[h][u] >= 0;
                             if (pl) {
-                                // j's history, newest first: its last spike at or before t2,
-                                // then the deferred dendrite updates (lo, t2), replayed oldest first
+                                // deferred dendrite updates before t2, oldest first
                                 const int32_t* hj = hL + loc * kStdpHist;
                                 const int32_t c = cL[loc];
-                                const int32_t lo = max(la[h] == 0 ? -1 : la[h], tfL[loc]);
-                                const int32_t nm = min(c, kStdpHist);
-                                int32_t mlo = nm, mhi = -1;
-                                bool found = false;
-                                for (int32_t m = 0; m < nm; ++m) {
-                                    const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
-                                    if (e > t2[h]) continue;           // j fired after t2 in that window
-                                    if (!found) {
-                                        lj = e;
-                                        found = true;
-                                    }
-                                    if (e == t2[h]) continue;
-                                    if (e <= lo) break;
-                                    mlo = min(mlo, m);
-                                    mhi = m;
-                                }
+                                const int32_t mlo = rng[h][u] & 0xFF, mhi = rng[h][u] >> 8;
                                 for (int32_t m = mhi; m >= mlo; --m) {
                                     const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
-                                    w = clipw(w + stdp_dw(a.ltp, a.ltd, a.lut_n, la[h] - e), a.w_max, a.w_min);
+                                    w = clipw(w + dw(la[h] - e), a.w_max, a.w_min);
                                 }
                             }
                             const bool exc = w > 0.f;
@@ -770,8 +809,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             const int32_t at = idle ? (tid % B) : ((exc ? 0 : D) + slot) * B + loc;
                             if (full) atomicAdd(reinterpret_cast<int32_t*>(ring) + at, valid ? v : 0);
                             if (pl && valid)                           // axon update of t2
-                                a.w_t[kb[h] + (uint32_t)u] =
-                                    clipw(w + stdp_dw(a.ltp, a.ltd, a.lut_n, t2[h] - lj), a.w_max, a.w_min);
+                                a.w_t[kb[h] + (uint32_t)u] = clipw(w + ax[h][u], a.w_max, a.w_min);
                         }
                     }
             }
@@ -780,55 +818,21 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         __syncthreads();   // wsum / sp_* are rewritten by the next batch
     }
     if (kPl) {
-        // 2b. flush: neurons that fired kStdpHist - 1 times since tf (all that fired
-        // since tf for apply_only) bring their plastic in-synapses up to date
-        // through the previous window, 8 in-synapses per thread in flight
+        // 2b. neurons that fired kStdpHist - 1 times since tf (for apply_only: all that
+        // fired since tf) go to this window's flush list (k_stdp_flush, launched next)
+        uint32_t* fc = a.fcnt + (kwin & 1);
+        int32_t* fli = a.flist + (size_t)(kwin & 1) * a.flist_cap * 3;
         for (int32_t j = tid; j < tn; j += bd) {
             const int32_t n_since = cL[j] - a.cntf[t0 + j];
-            if (full ? n_since >= kStdpHist - 1 : n_since > 0) fl[atomicAdd(&nfl, 1u)] = (uint32_t)j;
-        }
-        __syncthreads();
-        const int32_t hz = (int32_t)(s0 - 1);               // flush horizon: end of the previous window
-        const uint32_t nf = nfl;
-        for (uint32_t fi = 0; fi < nf; ++fi) {
-            const int32_t jl = (int32_t)fl[fi];
-            const int32_t* hj = hL + jl * kStdpHist;
-            const int32_t c = cL[jl], tfj = tfL[jl], nm = min(c, kStdpHist);
-            const int64_t pb = a.din_ptr[t0 + jl], pe = a.din_ptr[t0 + jl + 1];
-            for (int64_t p0 = pb + tid; p0 < pe; p0 += 8 * (int64_t)bd) {
-                uint32_t kk[8];
-                int32_t sr[8], lk[8];
-                float wv[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int64_t pp = min(p0 + u * (int64_t)bd, pe - 1);
-                    kk[u] = a.din_pos[pp];
-                    sr[u] = a.din_src[pp];
+            if (full ? n_since >= kStdpHist - 1 : n_since > 0) {
+                const uint32_t e = atomicAdd(fc, 1u);
+                fli[3 * e] = t0 + j;
+                fli[3 * e + 1] = cL[j];
+                fli[3 * e + 2] = tfL[j];
+                if (full) {
+                    a.tf[t0 + j] = (int32_t)(s0 - 1);
+                    a.cntf[t0 + j] = cL[j];
                 }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    lk[u] = lastB[sr[u]];
-                    wv[u] = a.w_t[kk[u]];
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    if (p0 + u * (int64_t)bd >= pe) continue;
-                    // replay j's spikes in (max(k's last spike, tf), horizon], oldest first
-                    const int32_t lo = max(lk[u] == 0 ? -1 : lk[u], tfj);
-                    int32_t m = 0;
-                    while (m < nm && hj[(c - 1 - m) & (kStdpHist - 1)] > lo) ++m;
-                    if (m == 0) continue;
-                    float w = wv[u];
-                    for (--m; m >= 0; --m) {
-                        const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
-                        if (e <= hz) w = clipw(w + stdp_dw(a.ltp, a.ltd, a.lut_n, lk[u] - e), a.w_max, a.w_min);
-                    }
-                    a.w_t[kk[u]] = w;
-                }
-            }
-            if (full && tid == 0) {
-                a.tf[t0 + jl] = hz;
-                a.cntf[t0 + jl] = c;
             }
         }
         if (!full) return;
@@ -965,6 +969,82 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     ring_slice_out<RW>(ring, gring, D, B, (int32_t)(s0 % D), nsteps);
 }
 
+// Flush of deferred dendrite updates (kTStdp): for each neuron j the fused
+// kernel listed in window k, every plastic in-synapse k -> j replays j's spikes
+// in (max(k's last spike, old tf[j]), end of window k-1], oldest first, with
+// dt = k's last spike - t1 (k's last spike 0 counts as "never", as in the fused
+// kernel).  The list is staged in LDS 256 entries at a time; their in-synapses
+// are cut into units of one per thread, dealt round-robin over the grid.
+__global__ __launch_bounds__(256) void k_stdp_flush(TiledArgs a, int32_t off, int32_t wk) {
+    __shared__ int32_t ej[256], ec[256], etf[256];
+    __shared__ int64_t epb[256], epe[256];
+    __shared__ uint32_t eui[256], wsum[16];                // inclusive prefix of units
+    const Ctl* ctl = a.ctl;
+    const int64_t kwin = ctl->base_window + wk;
+    const uint32_t nf = a.fcnt[kwin & 1];
+    const int32_t* fli = a.flist + (size_t)(kwin & 1) * a.flist_cap * 3;
+    const int32_t hz = (int32_t)(ctl->base_step + off - 1);
+    const int32_t* lastB = a.last3 + (size_t)(kwin % 3) * a.last_stride;   // end of window k-1
+    const uint32_t bd = blockDim.x, G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
+    uint64_t ubase = 0;                                     // units of the chunks before
+    for (uint32_t f0 = 0; f0 < nf; f0 += bd) {
+        const uint32_t nc = min(bd, nf - f0);
+        uint32_t nu = 0;
+        if (tid < nc) {
+            const int32_t j = fli[3 * (f0 + tid)];
+            ej[tid] = j;
+            ec[tid] = fli[3 * (f0 + tid) + 1];
+            etf[tid] = fli[3 * (f0 + tid) + 2];
+            const int64_t pb = a.din_ptr[j], pe = a.din_ptr[j + 1];
+            epb[tid] = pb;
+            epe[tid] = pe;
+            nu = (uint32_t)((pe - pb + bd - 1) / bd);
+        }
+        const uint32_t incl = block_incl_scan(nu, wsum);
+        eui[tid] = incl;
+        const uint32_t total = wsum[(bd >> 6) - 1];
+        __syncthreads();
+        for (uint32_t u = (uint32_t)((b + G - (uint32_t)(ubase % G)) % G); u < total; u += G) {
+            uint32_t lo_f = 0, hi_f = nc - 1;               // entry f: eui[f-1] <= u < eui[f]
+            while (lo_f < hi_f) {
+                const uint32_t mid = (lo_f + hi_f) >> 1;
+                if (eui[mid] <= u) lo_f = mid + 1;
+                else hi_f = mid;
+            }
+            const uint32_t f = lo_f;
+            const uint32_t uf = u - (f ? eui[f - 1] : 0u);
+            const int64_t p = epb[f] + (int64_t)uf * bd + tid;
+            if (p >= epe[f]) continue;
+            const int32_t j = ej[f], c = ec[f];
+            const uint32_t kk = a.din_pos[p];
+            const int32_t lk = lastB[a.din_src[p]];
+            const int32_t lo = max(lk == 0 ? -1 : lk, etf[f]);
+            const int32_t* hj = a.hist + (size_t)j * kStdpHist;
+            const int32_t nm = min(c, kStdpHist);
+            // the newest nr entries are after lo; their table values are loaded together,
+            // then applied oldest first
+            int32_t hv[kStdpHist];
+#pragma unroll
+            for (int m = 0; m < kStdpHist; ++m) hv[m] = hj[(c - 1 - m) & (kStdpHist - 1)];
+            int32_t nr = 0;
+#pragma unroll
+            for (int m = 0; m < kStdpHist; ++m) nr += (m < nm && hv[m] > lo && m == nr) ? 1 : 0;
+            if (nr == 0) continue;
+            float dv[kStdpHist];
+#pragma unroll
+            for (int m = 0; m < kStdpHist; ++m)
+                dv[m] = (m < nr && hv[m] <= hz) ? stdp_dw(a.ltp, a.ltd, a.lut_n, lk - hv[m]) : 0.0f;
+            float w = a.w_t[kk];
+#pragma unroll
+            for (int m = kStdpHist - 1; m >= 0; --m)
+                if (m < nr && hv[m] <= hz) w = clipw(w + dv[m], a.w_max, a.w_min);
+            a.w_t[kk] = w;
+        }
+        ubase += total;
+        __syncthreads();                                    // the staging arrays are reused
+    }
+}
+
 __global__ void k_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows) {
     ctl->base_step += steps;
     ctl->base_window += windows;
@@ -1011,12 +1091,13 @@ hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t
     return hipGetLastError();
 }
 
-size_t tiled_lds_bytes(int D, int B, int kind) {
+size_t tiled_lds_bytes(int D, int B, int kind, int lut_lds) {
     const size_t rw = (kind == kTClass || kind == kTClass16) ? 1 : 2;
+    const bool pl = kind == kTStdp;
     return sizeof(uint32_t) * (rw * (size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
            sizeof(uint16_t) * (size_t)kTiledItems +
-           ((kind == kTFP8 || kind == kTBF16 || kind == kTStdp) ? sizeof(uint32_t) * kTiledBatch : 0) +
-           (kind == kTStdp ? (kStdpHist + 3) * sizeof(int32_t) * (size_t)B : 0);
+           ((kind == kTFP8 || kind == kTBF16 || pl) ? sizeof(uint32_t) * kTiledBatch : 0) +
+           (pl ? (kStdpHist + 2) * sizeof(int32_t) * (size_t)B + 2 * sizeof(float) * (size_t)lut_lds : 0);
 }
 
 namespace {
@@ -1060,7 +1141,7 @@ hipError_t set_lds_kind(size_t bytes) {
 
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st) {
-    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind);
+    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.kind == kTStdp ? a.lut_lds : 0);
     switch (a.kind) {
         case kTBF16: launch_tiled_kind<kTBF16>(a, off, wk, nsteps, block, lds, st); break;
         case kTFP8: launch_tiled_kind<kTFP8>(a, off, wk, nsteps, block, lds, st); break;
@@ -1081,6 +1162,11 @@ hipError_t set_tiled_lds_limit(int kind, size_t bytes) {
         case kTStdp: return set_lds_kind<kTStdp>(bytes);
         default: return set_lds_kind<kTClass>(bytes);
     }
+}
+
+hipError_t launch_stdp_flush(const TiledArgs& a, int32_t off, int32_t wk, hipStream_t st) {
+    hipLaunchKernelGGL(k_stdp_flush, dim3(1024), dim3(256), 0, st, a, off, wk);
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st) {

@@ -89,10 +89,12 @@ def test_ee_stdp_benchmark_constants(wf, prop):
 
 @pytest.mark.parametrize("tau_ref,window", [(0.002, 10), (0.0006, 6)])
 def test_fused_stdp_weights_between_runs(tau_ref, window):
-    """Fused path: weights read between runs (the pending window's updates applied to a
-    copy) equal the oracle's at every read, and reading them changes nothing; a 0.6 ms
-    refractory period (inter-spike interval >= 6 steps) caps the window below min_delay."""
-    print(f"case: E->E STDP, tau_ref={tau_ref}, runs of 257 / 300 / 243 steps", flush=True)
+    """Fused path: weights read between runs (the deferred updates applied to a copy)
+    equal the oracle's at every read, and reading them changes nothing; a 0.6 ms
+    refractory period (inter-spike interval >= 6 steps) caps the window below min_delay.
+    The runs are long enough for neurons to fire well over 16 times, so the flushes of
+    the 16-entry spike history (deferred dendrite updates) are exercised."""
+    print(f"case: E->E STDP, tau_ref={tau_ref}, runs of 457 / 600 / 443 steps", flush=True)
     sys.stdout.flush()
     net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
     net.lifconst = M.gen_life_const(0.0, 0.0, 0.010, 0.0, 0.0, tau_ref, 1.0, 1.0, 0.020, 0.0)
@@ -106,7 +108,7 @@ def test_fused_stdp_weights_between_runs(tau_ref, window):
                     scale=(info["fix_scale_exc"], info["fix_scale_inh"]))
     w0 = g.weights()
     done, want = 0, []
-    for n in (257, 300, 243):
+    for n in (457, 600, 443):
         g.run(n)
         want += o.run(n)
         done += n
@@ -118,5 +120,8 @@ def test_fused_stdp_weights_between_runs(tau_ref, window):
     assert np.array_equal(g.fire_counts(), o.state(po.F_CNT).astype(np.int32))
     assert np.array_equal(g.state(N.F_LAST), o.state(po.F_LAST).astype(np.int32))
     assert int(np.sum(g.weights() != w0)) > 100
+    counts = g.fire_counts()
+    print(f"fire counts: max {counts.max()}, neurons above 31: {int(np.sum(counts > 31))}", flush=True)
+    assert int(np.sum(counts > 31)) > 50        # each of these was flushed at least twice
     g.close()
     o.close()
