@@ -194,7 +194,13 @@ def stdp_leg(args, log):
     wf = N.WF_F32 if fused or args.precision != "bf16" else N.WF_BF16
     prop = N.PROP_TILED if fused else N.PROP_DIRECT
     t0 = time.perf_counter()
-    shape = dict(tile_size=args.tile, prop_block=args.block, chunk=args.chunk) if fused else {}
+    shape = {}
+    if fused:
+        shape = dict(tile_size=args.tile, prop_block=args.block, chunk=args.chunk)
+        table = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
+        tuned = json.load(open(table)).get("brunel100k_K10000_stdp_ee") if os.path.exists(table) else None
+        if tuned and args.scale == 1.0 and not (args.tile or args.block or args.chunk):
+            shape = dict(tile_size=tuned["tile_size"], prop_block=tuned["block"], chunk=tuned["chunk"])
     sim = N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed, stdp=True, stdpconst=sc,
                                 weight_format=wf, use_graph=args.graph, profile=0 if args.no_profile else 1,
                                 propagation=prop, **shape)
