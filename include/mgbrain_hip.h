@@ -175,7 +175,7 @@ typedef struct mgb_info {
     int32_t tile_size;
     int32_t tiles;
     int32_t block;             /* workgroup size of the propagation / fused kernel */
-    int32_t segments;          /* independent segments of the fused path (networks of a batch) */
+    int32_t segments;          /* independent segments (networks of a batch) */
     int32_t word_bytes;        /* bytes per synapse the fused kernel reads (0: direct path) */
 } mgb_info;
 

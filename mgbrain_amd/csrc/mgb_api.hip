@@ -101,6 +101,8 @@ struct mgb_sim {
     uint32_t* tcnt = nullptr;          // [3][nseg]
     int tkind = kTClass;               // synapse word kind of the fused kernel
     uint32_t* tperm = nullptr;         // kTStdp: CSR slot of each tile-order position
+    float2* seg_scale = nullptr;       // fixed-point formats, several segments: (scale, 1 / scale)
+    uint16_t* seg_of = nullptr;        // segment of each neuron (with seg_scale)
     int32_t *hist = nullptr, *tf = nullptr, *cntf = nullptr;   // kTStdp: deferred dendrite updates
     int32_t* flist = nullptr;          // kTStdp: [2][npad][3] neurons to flush
     uint32_t* fcnt = nullptr;          // [2]
@@ -467,10 +469,11 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
     MGB_TRY(X.get(&wcls, 2 * (size_t)n));
     MGB_TRY(X.get(&indeg, 2 * (size_t)n));
     MGB_TRY(X.get(&in_pl, (size_t)n));
-    MGB_TRY(X.get(&absmax, 1));
     MGB_TRY(X.get(&npl, 1));
     std::vector<uint16_t> run_of;
     std::vector<int32_t> runs = stream_runs(h, run_of);
+    const size_t nruns = runs.size() - 1;                  // max |w| per run of streams
+    MGB_TRY(X.get(&absmax, std::max<size_t>(nruns, 1)));
     uint16_t* seg_dev = nullptr;
     if (runs.size() > 2) {
         MGB_TRY(X.get(&seg_dev, (size_t)n));
@@ -482,24 +485,25 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
     MGB_HIP_TRY(hipMemsetAsync(wcls, 0xFF, 8 * (size_t)n, st));
     MGB_HIP_TRY(hipMemsetAsync(indeg, 0, 8 * (size_t)n, st));
     MGB_HIP_TRY(hipMemsetAsync(in_pl, 0, 4 * (size_t)n, st));
-    MGB_HIP_TRY(hipMemsetAsync(absmax, 0, 4, st));
+    MGB_HIP_TRY(hipMemsetAsync(absmax, 0, 4 * std::max<size_t>(nruns, 1), st));
     MGB_HIP_TRY(hipMemsetAsync(npl, 0, 8, st));
     if (S > 0) {
         StatArgs a{};
         a.S = S; a.n = n; a.step = h.step;
         a.src = L.src; a.tar = L.tar; a.weight = L.weight; a.delay = L.delay;
         a.plastic = h.stdp ? L.plastic : nullptr;
-        a.seg = seg_dev;
+        a.seg = seg_dev;                                   // null for one run: absmax[0]
         a.dq = dq; a.flags = flags; a.min_d = mm; a.max_d = mm + 1;
         a.wcls = wcls; a.indeg = indeg; a.in_pl = in_pl; a.absmax = absmax; a.n_plastic = npl;
         MGB_HIP_TRY(run_stats(a, st));
     }
-    uint32_t hflags = 0, habs = 0;
+    uint32_t hflags = 0;
+    std::vector<uint32_t> habs(std::max<size_t>(nruns, 1), 0u);
     int32_t hmm[2];
     unsigned long long hnpl = 0;
     MGB_HIP_TRY(hipMemcpyAsync(&hflags, flags, 4, hipMemcpyDeviceToHost, st));
     MGB_HIP_TRY(hipMemcpyAsync(hmm, mm, 8, hipMemcpyDeviceToHost, st));
-    MGB_HIP_TRY(hipMemcpyAsync(&habs, absmax, 4, hipMemcpyDeviceToHost, st));
+    MGB_HIP_TRY(hipMemcpyAsync(habs.data(), absmax, 4 * habs.size(), hipMemcpyDeviceToHost, st));
     MGB_HIP_TRY(hipMemcpyAsync(&hnpl, npl, 8, hipMemcpyDeviceToHost, st));
     std::vector<uint32_t> h_wcls(2 * (size_t)n), h_indeg(2 * (size_t)n);
     MGB_HIP_TRY(hipMemcpyAsync(h_wcls.data(), wcls, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
@@ -660,16 +664,33 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         }
         MGB_HIP_TRY(hipStreamSynchronize(st));
     } else {
-        // fixed-point scale: |ring value| <= in-degree * max |w| < 2^31
-        float wabs = 0.f;
-        std::memcpy(&wabs, &habs, 4);
-        if (h.stdp) wabs = std::max({wabs, std::fabs(h.stdpconst[4]), std::fabs(h.stdpconst[5])});
-        if (s->wide_kind != kWideF32) wabs *= 1.0625f;   // BF16 / FP8 rounding headroom
-        const double bound = (double)std::max<uint32_t>(max_deg_tot, 1) * (double)wabs;
-        int e = 30;
-        if (bound > 0) e = (int)std::floor(std::log2(2147483520.0 / bound));
-        e = std::max(-100, std::min(e, 100));
-        s->scale_e = s->scale_i = std::ldexp(1.0f, e);
+        // fixed-point scale per segment (a batched network gets the scale it has
+        // alone): |ring value| <= in-degree * max |w| < 2^31
+        const std::vector<int32_t>& slo = s->h_seg_lo;
+        const size_t nseg = slo.size() - 1;
+        std::vector<float2> seg_sc(nseg);
+        for (size_t q = 0; q < nseg; ++q) {
+            uint32_t deg = 0, abits = 0;
+            for (int32_t t = slo[q]; t < slo[q + 1]; ++t) deg = std::max(deg, h_indeg[t] + h_indeg[(size_t)n + t]);
+            if (nseg == nruns) abits = habs[q];
+            else for (uint32_t b : habs) abits = std::max(abits, b);      // one segment
+            float wabs = 0.f;
+            std::memcpy(&wabs, &abits, 4);
+            if (h.stdp) wabs = std::max({wabs, std::fabs(h.stdpconst[4]), std::fabs(h.stdpconst[5])});
+            if (s->wide_kind != kWideF32) wabs *= 1.0625f;   // BF16 / FP8 rounding headroom
+            const double bound = (double)std::max<uint32_t>(deg, 1) * (double)wabs;
+            int e = 30;
+            if (bound > 0) e = (int)std::floor(std::log2(2147483520.0 / bound));
+            e = std::max(-100, std::min(e, 100));
+            seg_sc[q] = make_float2(std::ldexp(1.0f, e), std::ldexp(1.0f, -e));
+        }
+        s->scale_e = s->scale_i = seg_sc[0].x;
+        if (nseg > 1) {
+            MGB_TRY(dalloc_n(s, &s->seg_scale, nseg));
+            MGB_TRY(dalloc_n(s, &s->seg_of, (size_t)n));
+            MGB_HIP_TRY(hipMemcpyAsync(s->seg_scale, seg_sc.data(), sizeof(float2) * nseg, hipMemcpyHostToDevice, st));
+            MGB_HIP_TRY(hipMemcpyAsync(s->seg_of, run_of.data(), 2 * (size_t)n, hipMemcpyHostToDevice, st));
+        }
         MGB_TRY(dalloc_n(s, &s->col, S));
         MGB_TRY(dalloc_n(s, &s->dly, S));
         MGB_TRY(dalloc_n(s, &s->w32, S));
@@ -913,6 +934,7 @@ NeuArgs neu_args(const mgb_sim* s) {
     a.ring = s->ring; a.tbl_off = s->tbl_off; a.seq = s->seq; a.tbl_w = s->tbl_w;
     a.ring_e = s->ring_e; a.ring_i = s->ring_i;
     a.inv_e = 1.0f / s->scale_e; a.inv_i = 1.0f / s->scale_i;
+    a.seg_scale = s->seg_scale; a.seg_of = s->seg_of;
     a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
     return a;
 }
@@ -924,6 +946,7 @@ PropArgs prop_args(const mgb_sim* s) {
     a.w32 = s->w32; a.w16 = s->w16; a.w8 = s->w8; a.row_scale = s->row_scale;
     a.ring = s->ring; a.ring_e = s->ring_e; a.ring_i = s->ring_i;
     a.scale_e = s->scale_e; a.scale_i = s->scale_i;
+    a.seg_scale = s->seg_scale; a.seg_of = s->seg_of;
     a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
     a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
     return a;
@@ -946,6 +969,7 @@ TiledArgs tiled_args(const mgb_sim* s) {
     a.kind = s->tkind;
     a.scale_e = s->scale_e; a.scale_i = s->scale_i;
     a.inv_e = 1.0f / s->scale_e; a.inv_i = 1.0f / s->scale_i;
+    a.seg_scale = s->seg_scale;
     a.w_t = s->w_t; a.row_scale = s->row_scale;
     a.nseg = s->nseg; a.window = s->window;
     a.tile_lo = s->tile_lo; a.tile_seg = s->tile_seg; a.seg_lo = s->seg_lo;
@@ -1576,7 +1600,7 @@ mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
     out->tile_size = s->B;
     out->tiles = s->G;
     out->block = s->tiled ? s->tblock : s->prop_block;
-    out->segments = s->tiled ? s->nseg : 1;
+    out->segments = std::max<int32_t>(1, (int32_t)s->h_seg_lo.size() - 1);
     out->word_bytes = !s->tiled ? 0 : (s->tkind == kTClass16 ? 2 : (s->tkind == kTF32 || s->tkind == kTStdp ? 8 : 4));
     out->last_run_ms = 0.0;
     if (s->timed) {

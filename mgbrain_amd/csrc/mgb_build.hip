@@ -59,7 +59,7 @@ __global__ void k_stats(StatArgs a) {
                     const uint32_t old = atomicCAS(wc, kWSentinel, wb);
                     if (old != kWSentinel && old != wb) atomicOr(a.flags, kFlagMixed);
                 }
-                atomicMax(a.absmax, __float_as_uint(fabsf(w)));
+                atomicMax(&a.absmax[a.seg ? a.seg[t] : 0], __float_as_uint(fabsf(w)));
                 if (a.plastic && a.plastic[i]) {
                     atomicAdd(a.n_plastic, 1ull);
                     atomicAdd(&a.in_pl[t], 1u);

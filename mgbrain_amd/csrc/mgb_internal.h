@@ -85,6 +85,10 @@ struct NeuArgs {
     int32_t* ring_e;
     int32_t* ring_i;
     float inv_e, inv_i;
+    // batched networks with fixed-point formats: each segment's (scale, 1 / scale)
+    // (a network's own, as if it ran alone), segment of each neuron; null for one
+    const float2* seg_scale;
+    const uint16_t* seg_of;
     Ctl* ctl;
     uint32_t* win_buf;
     uint32_t win_stride;
@@ -105,6 +109,8 @@ struct PropArgs {
     int32_t* ring_e;
     int32_t* ring_i;
     float scale_e, scale_i;
+    const float2* seg_scale;   // per segment (scale, 1 / scale); null: scale_e / scale_i
+    const uint16_t* seg_of;
     Ctl* ctl;
     const uint32_t* win_buf;
     uint32_t win_stride;
@@ -153,6 +159,7 @@ struct TiledArgs {
     int32_t chunk;          // synapses per work item in the propagation phase
     int32_t kind;           // TiledKind
     float scale_e, scale_i, inv_e, inv_i;   // fixed-point scales of the wide kinds
+    const float2* seg_scale;                // per segment (scale, 1 / scale); null: the above
     float* w_t;             // kTF32 / kTStdp: weights in tile order
     const float* row_scale; // kTFP8: per-source scale
     int32_t nseg;

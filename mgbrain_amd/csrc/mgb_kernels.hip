@@ -230,8 +230,9 @@ __global__ __launch_bounds__(kNeuBlock) void k_neurons(NeuArgs a, int32_t off, i
                 const int32_t qe = a.ring_e[loc], qi = a.ring_i[loc];
                 if (qe != 0) a.ring_e[loc] = 0;
                 if (qi != 0) a.ring_i[loc] = 0;
-                be = (float)qe * a.inv_e;
-                bi = (float)qi * a.inv_i;
+                const float inv = a.seg_scale ? a.seg_scale[a.seg_of[i]].y : a.inv_e;   // inv_e == inv_i
+                be = (float)qe * inv;
+                bi = (float)qi * inv;
             }
             fired = integrate_step(type, st, s, be, bi);
         }
@@ -308,6 +309,7 @@ __global__ void k_prop(PropArgs a, int32_t off, int32_t wk) {
             }
         } else {
             const float rs = (WK == kWideF32) ? 1.0f : a.row_scale[i];
+            const float sc = a.seg_scale ? a.seg_scale[a.seg_of[i]].x : a.scale_e;   // scale_e == scale_i
             // four slots per lane in flight: clamped indices keep the loads in the
             // row; slots past its end add nothing
             for (int64_t k0 = beg + lane; k0 < end; k0 += 4 * stride) {
@@ -331,8 +333,8 @@ __global__ void k_prop(PropArgs a, int32_t off, int32_t wk) {
                     slot -= (slot >= D) ? D : 0;
                     const float w = (dl[u] & 128u) ? w32[u] : wn[u];   // plastic weights stay FP32
                     const size_t loc = (size_t)slot * (size_t)a.npad + (size_t)tar[u];
-                    if (w > 0.f) atomicAdd(&a.ring_e[loc], (int32_t)rintf(w * a.scale_e));
-                    else atomicAdd(&a.ring_i[loc], (int32_t)rintf(w * a.scale_i));
+                    if (w > 0.f) atomicAdd(&a.ring_e[loc], (int32_t)rintf(w * sc));
+                    else atomicAdd(&a.ring_i[loc], (int32_t)rintf(w * sc));
                 }
             }
         }
@@ -578,6 +580,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     };
     const bool full = !kPl || a.apply_only == 0;
     Ctl* ctl = a.ctl;
+    // fixed-point (scale, 1 / scale) of the tile's segment (scale_e == scale_i)
+    const float2 fsc = a.seg_scale ? a.seg_scale[a.tile_seg[blockIdx.x]] : make_float2(a.scale_e, a.inv_e);
     const int64_t s0 = ctl->base_step + off;
     const int64_t kwin = ctl->base_window + wk;
     const int bcur = (int)(kwin % 3), bprev = (int)((kwin + 2) % 3), bnext = (int)((kwin + 1) % 3);
@@ -805,7 +809,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                                 }
                             }
                             const bool exc = w > 0.f;
-                            const int32_t v = (int32_t)rintf(w * (exc ? a.scale_e : a.scale_i));
+                            const int32_t v = (int32_t)rintf(w * fsc.x);
                             const int32_t at = idle ? (tid % B) : ((exc ? 0 : D) + slot) * B + loc;
                             if (full) atomicAdd(reinterpret_cast<int32_t*>(ring) + at, valid ? v : 0);
                             if (pl && valid)                           // axon update of t2
@@ -899,8 +903,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                     const int32_t qe = (int32_t)*r, qi = (int32_t)*ri;
                     *r = 0u;
                     *ri = 0u;
-                    be = (float)qe * a.inv_e;
-                    bi = (float)qi * a.inv_i;
+                    be = (float)qe * fsc.y;
+                    bi = (float)qi * fsc.y;
                 }
                 fired = integrate_step(type, st, s, be, bi);
             }

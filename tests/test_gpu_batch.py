@@ -114,3 +114,66 @@ def test_segments_from_streams(cross):
     assert len(ti) > 100
     assert np.array_equal(ts, ds) and np.array_equal(ti, di)
     assert np.array_equal(tc, dc) and tse == dse
+
+
+def test_batch_stdp_fused_equals_separate_runs():
+    """Two networks with E->E STDP batched on the fused path: per-network spikes, fire
+    counts and weights (deferred updates applied at the read) equal separate runs."""
+    print("case: batch of two sweep networks with E->E STDP (fused, FP32)", flush=True)
+    sys.stdout.flush()
+    sc = W.stdp_consts()
+    nets = []
+    # Poisson drive strong enough for the LIF neurons to fire (~0.15 mV per step of input)
+    for n, g, nu, seed in ((3000, 4.0, 500.0, 11), (5000, 5.0, 600.0, 12)):
+        net = W.brunel_sweep_net(n, g, nu, seed)
+        for q in net["projs"]:
+            if q["src"] == 1 and q["tar"] == 1:
+                q["plastic"] = 1
+        net.update(stdp=True, stdpconst=sc)
+        nets.append(net)
+    opts = dict(weight_format=N.WF_F32, propagation=N.PROP_TILED, record_capacity=1_000_000)
+    b = N.Simulator.batch_generated(nets, **opts)
+    assert b.info()["segments"] == 2 and b.info()["window"] == 10
+    b.run(1500)
+    singles = [N.Simulator.generated(**net, **opts) for net in nets]
+    for s in singles:
+        s.run(1500)
+    cb, wb = b.fire_counts(), b.weights()
+    w0 = N.Simulator.batch_generated(nets, **opts)
+    changed = int(np.sum(w0.weights() != wb))
+    w0.close()
+    off, woff = 0, 0
+    for s in singles:
+        c, w = s.fire_counts(), s.weights()
+        assert np.array_equal(cb[off:off + len(c)], c)
+        assert np.array_equal(wb[woff:woff + len(w)].view(np.uint32), w.view(np.uint32))
+        off += len(c)
+        woff += len(w)
+        s.close()
+    print(f"spikes {cb.sum()}, weights changed {changed}", flush=True)
+    assert off == len(cb) and woff == len(wb) and cb.sum() > 20000 and changed > 10000
+    b.close()
+
+
+@pytest.mark.parametrize("wf,prop", [(N.WF_BF16, N.PROP_TILED), (N.WF_F32, N.PROP_DIRECT), (N.WF_FP8, N.PROP_DIRECT)])
+def test_batch_fixed_point_formats_equal_separate_runs(wf, prop):
+    """Fixed-point formats: every network of a batch accumulates with its own scale (the
+    one it has alone), so per-network fire counts equal separate runs bit for bit."""
+    print(f"case: batch of two driven sweep networks, weight_format={wf}, propagation={prop}", flush=True)
+    sys.stdout.flush()
+    nets = [W.brunel_sweep_net(n, g, nu, seed) for n, g, nu, seed in ((3000, 4.0, 500.0, 21), (6000, 5.0, 600.0, 22))]
+    opts = dict(weight_format=wf, propagation=prop)
+    b = N.Simulator.batch_generated(nets, **opts)
+    assert b.info()["segments"] == 2
+    b.run(1200)
+    cb = b.fire_counts()
+    off = 0
+    for net in nets:
+        s = N.Simulator.generated(**net, **opts)
+        s.run(1200)
+        c = s.fire_counts()
+        assert np.array_equal(cb[off:off + len(c)], c)
+        off += len(c)
+        s.close()
+    assert off == len(cb) and cb.sum() > 20000
+    b.close()
