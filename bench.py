@@ -165,6 +165,35 @@ def batched_leg(args, shapes=None, log=None):
         base += sizes[k]
     log(f"# sequential: {seq_se:.4g} events in {seq_wall * 1e3:.1f} ms, {seq_se / seq_wall:.4g} SE/s; "
         f"per-network fire counts identical: {identical}")
+    bf16 = None
+    if not args.no_batch_bf16:
+        # the same batch with BF16 weights (per-source scales, fixed-point sums): rate error
+        # of every network against the exact run above
+        sb = N.Simulator.batch_generated(nets, use_graph=args.graph, weight_format=N.WF_BF16)
+        ib = sb.info()
+        sb.run(args.warmup)
+        sb.sync()
+        a2 = sb.info()
+        t = time.perf_counter()
+        sb.run(args.steps)
+        sb.sync()
+        wall2 = time.perf_counter() - t
+        se2 = sb.info()["syn_events"] - a2["syn_events"]
+        c2 = sb.fire_counts()
+        sb.close()
+        errs, base = [], 0
+        for k in range(len(nets)):
+            r_ex = float(counts[base:base + sizes[k]].sum())
+            r_bf = float(c2[base:base + sizes[k]].sum())
+            if r_ex > 0:
+                errs.append(abs(r_bf - r_ex) / r_ex)
+            base += sizes[k]
+        bf16 = {"value": se2 / wall2, "rtf": wall2 / (args.steps * DT), "tiles": ib["tiles"],
+                "tile_size": ib["tile_size"], "word_bytes": ib["word_bytes"],
+                "max_rate_err_vs_exact": max(errs) if errs else 0.0,
+                "mean_rate_err_vs_exact": float(np.mean(errs)) if errs else 0.0}
+        log(f"# batched BF16: {se2 / wall2:.4g} SE/s, RTF {wall2 / (args.steps * DT):.4f}, rate error vs exact "
+            f"max {bf16['max_rate_err_vs_exact'] * 100:.3f} % mean {bf16['mean_rate_err_vs_exact'] * 100:.3f} %")
     return {
         "workload": f"brunel_sweep{len(nets)}",
         "description": (f"{len(nets)} independent Brunel networks: N log-uniform on [2000, 50000] LIFB "
@@ -178,7 +207,7 @@ def batched_leg(args, shapes=None, log=None):
         "sequential_value": seq_se / seq_wall, "sequential_rtf": seq_wall / (args.steps * DT),
         "speedup_vs_sequential": (se / wall) / (seq_se / seq_wall),
         "fire_counts_identical_to_sequential": identical, "max_rate_err_vs_sequential": max_rate_err,
-        "roofline": roof, "shapes": shapes,
+        "roofline": roof, "shapes": shapes, "bf16": bf16,
     }
 
 
@@ -362,6 +391,7 @@ def main(argv=None):
     ap.add_argument("--stdp-path", default="fused", choices=("fused", "direct"),
                     help="--config 4: fused window kernel (FP32 weights) or the direct kernels")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg of the default run")
+    ap.add_argument("--no-batch-bf16", action="store_true", help="skip the BF16 run of the batched leg")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1, help="replay the step loop as hipGraphs (default 1)")
