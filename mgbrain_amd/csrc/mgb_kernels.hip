@@ -654,6 +654,17 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // their latency overlaps it
     uint32_t pbeg = 0u, plm = 0u, paux = 0u;
     if ((uint32_t)tid < min((uint32_t)bd, E)) meta((uint32_t)tid, pbeg, plm, paux);
+    // a tile of at most one neuron per thread: the neuron's state and out-degree are
+    // loaded now, their latency hidden by the ring copy and the propagation phase
+    const bool pre = B <= bd && full && !(kPl && CH >= 16);   // (not where registers would spill)
+    NState pst;
+    int ptype = T_RAW;
+    int32_t pdeg = 0;
+    if (pre && tid < tn) {
+        ptype = (int)a.type[t0 + tid];
+        neuron_load(a, t0 + tid, ptype, kCls, pst, lastB);
+        if (ptype != T_RAW && ptype != T_LIF0) pdeg = (int32_t)(a.row_ptr[t0 + tid + 1] - a.row_ptr[t0 + tid]);
+    }
     if (full) ring_slice_in<RW>(ring, gring, D, B, rprev, nprev);
     __syncthreads();
 
@@ -854,13 +865,17 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         const int32_t j = j0 + tid;
         const bool active = j < tn;
         const int32_t i = t0 + j;
-        const int type = active ? (int)a.type[i] : T_RAW;
+        const int type = active ? (pre ? ptype : (int)a.type[i]) : T_RAW;
         NState st;
-        if (active) neuron_load(a, i, type, kCls, st, lastB);
-        const bool integ = integrates(type);
         int32_t outdeg = 0;
-        if (active && type != T_RAW && type != T_LIF0)
-            outdeg = (int32_t)(a.row_ptr[i + 1] - a.row_ptr[i]);
+        if (pre) {
+            st = pst;
+            outdeg = pdeg;
+        } else if (active) {
+            neuron_load(a, i, type, kCls, st, lastB);
+            if (type != T_RAW && type != T_LIF0) outdeg = (int32_t)(a.row_ptr[i + 1] - a.row_ptr[i]);
+        }
+        const bool integ = integrates(type);
         // wave-uniform fast path: every active lane an LIFB neuron of a cached class
         int32_t csl = -1;
         if (kCls && active && integ) csl = (st.cls == ccls[0]) ? 0 : ((st.cls == ccls[1]) ? 1 : -1);
