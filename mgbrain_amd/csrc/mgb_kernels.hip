@@ -654,8 +654,10 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // their latency overlaps it
     uint32_t pbeg = 0u, plm = 0u, paux = 0u;
     if ((uint32_t)tid < min((uint32_t)bd, E)) meta((uint32_t)tid, pbeg, plm, paux);
+    if (full) ring_slice_in<RW>(ring, gring, D, B, rprev, nprev);
     // a tile of at most one neuron per thread: the neuron's state and out-degree are
-    // loaded now, their latency hidden by the ring copy and the propagation phase
+    // loaded now (after the ring copy, which they would delay), their latency
+    // hidden by the propagation phase
     const bool pre = B <= bd && full && !(kPl && CH >= 16);   // (not where registers would spill)
     NState pst;
     int ptype = T_RAW;
@@ -665,7 +667,6 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         neuron_load(a, t0 + tid, ptype, kCls, pst, lastB);
         if (ptype != T_RAW && ptype != T_LIF0) pdeg = (int32_t)(a.row_ptr[t0 + tid + 1] - a.row_ptr[t0 + tid]);
     }
-    if (full) ring_slice_in<RW>(ring, gring, D, B, rprev, nprev);
     __syncthreads();
 
     // 2. previous window's spikes into the tile
