@@ -125,3 +125,19 @@ def test_fused_stdp_weights_between_runs(tau_ref, window):
     assert int(np.sum(counts > 31)) > 50        # each of these was flushed at least twice
     g.close()
     o.close()
+
+
+def test_fused_stdp_conductance_neurons():
+    """Fused STDP on E->E of a LIFE (conductance-based) network against the oracle: the
+    window is min(min_delay 8, refractory interval 50) = 8 steps."""
+    print("case: config3 x0.05 LIFE with E->E STDP, fused", flush=True)
+    sys.stdout.flush()
+    net = M.config3(scale=0.05, p=0.1, g_exc=0.4, g_inh=5.1)
+    for q in net.projs:
+        if q["src"] == 0 and q["tar"] == 0:
+            q["plastic"] = 1
+    syn = H.host_synapses(net, seed=4)
+    assert 0 < syn.plastic.sum() < syn.n
+    sc = M.stdp_consts_config4(J=0.4)
+    changed, n = _run(net, syn, sc, 800, 4, N.WF_F32, prop=N.PROP_TILED, window=8)
+    assert changed > 100 and n > 500
