@@ -72,8 +72,10 @@ def fused_roofline(b, info0, steps, se, spikes, n_lif, n_poisson, kname, lif_sta
 
 
 def cpu_baseline(sim, budget_s=12.0, max_steps=4000, seed=0):
-    """The oracle (FP32 restatement of the reference step, one thread) on the same
-    network (CSR read back from the device), timed on this host."""
+    """The oracle (FP32 restatement of the reference step) on the same network (CSR
+    read back from the device), timed on this host: one thread (the checker's step
+    loop), then all cores (its OpenMP step loop: neurons in parallel, fired rows in
+    parallel with atomic ring adds), each on half the time budget from step 0."""
     from oracle import pyoracle as po  # CPU checker / baseline only
 
     inf = sim.info()
@@ -91,17 +93,31 @@ def cpu_baseline(sim, budget_s=12.0, max_steps=4000, seed=0):
     del sim_types
     steps = 0
     t1 = time.perf_counter()
-    while steps < max_steps and (time.perf_counter() - t1) < budget_s:
+    while steps < max_steps and (time.perf_counter() - t1) < budget_s / 2:
         o.run(100, cap=n * 100)
         steps += 100
     el = time.perf_counter() - t1
     se = o.syn_events()
     o.close()
-    return dict(value=se / el, unit="syn_events/s", cores=1, kind="port",
-                sample=f"oracle FP32 step loop, steps 0..{steps} ({steps * DT * 1e3:.0f} ms simulated) of the "
-                       f"same network, 1 thread, {el:.1f} s timed (+{setup:.1f} s read-back/setup); "
-                       f"host {platform.processor() or platform.machine()}, nproc={os.cpu_count()}",
-                steps=steps, seconds=el, syn_events=se, rtf=el / (steps * DT))
+    host = f"host {platform.processor() or platform.machine()}, nproc={os.cpu_count()}"
+    single = dict(value=se / el, cores=1, steps=steps, seconds=el, syn_events=se, rtf=el / (steps * DT))
+    # all cores (the box's CPU share: OMP_NUM_THREADS when set)
+    nt = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    o = po.OracleSim(cs, types, rate, v0, sim._bench_lifconst, seed=seed, acc=po.ACC_F32)
+    steps_mt = 0
+    t2 = time.perf_counter()
+    while steps_mt < 4 * max_steps and (time.perf_counter() - t2) < budget_s / 2:
+        o.run_mt(100, nt)
+        steps_mt += 100
+    el_mt = time.perf_counter() - t2
+    se_mt = o.syn_events()
+    o.close()
+    return dict(value=se_mt / el_mt, unit="syn_events/s", cores=nt, kind="port",
+                sample=f"oracle FP32 step loop with OpenMP ({nt} threads), steps 0..{steps_mt} "
+                       f"({steps_mt * DT * 1e3:.0f} ms simulated) of the same network, {el_mt:.1f} s timed; "
+                       f"one thread: steps 0..{steps}, {el:.1f} s timed (+{setup:.1f} s read-back/setup); {host}",
+                steps=steps_mt, seconds=el_mt, syn_events=se_mt, rtf=el_mt / (steps_mt * DT),
+                single_thread=single)
 
 
 def batched_leg(args, shapes=None, log=None):
@@ -551,8 +567,9 @@ def main(argv=None):
     if not args.no_cpu:
         print("# cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(sim, budget_s=args.cpu_budget, seed=args.seed)
-        print(f"# cpu: {cpu['value']:.4g} SE/s over {cpu['steps']} steps in {cpu['seconds']:.1f} s",
-              file=sys.stderr, flush=True)
+        print(f"# cpu: {cpu['value']:.4g} SE/s with {cpu['cores']} threads over {cpu['steps']} steps in "
+              f"{cpu['seconds']:.1f} s; one thread {cpu['single_thread']['value']:.4g} SE/s", file=sys.stderr,
+              flush=True)
     sim.close()
 
     batched = None

@@ -15,6 +15,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 /* ========================================================================== */
 /* Philox4x32-10 (Salmon et al., SC'11, section 5; constants from the paper)  */
@@ -378,6 +379,9 @@ struct orc_sim {
     double *Vd, *Ied, *Iid, *bexcd, *binhd, *wd;
     int32_t *refr, *last, *cnt, *src_of;
     uint8_t* fired;
+    /* orc_run_mt: every CSR row stably sorted by target (col, delay, weight) */
+    int32_t *mt_col, *mt_dly;
+    float* mt_w;
 };
 
 float orc_poisson_u(uint64_t seed, uint32_t id, uint32_t step) {
@@ -486,15 +490,16 @@ void orc_destroy(orc_sim* S) {
     free(S->qexc); free(S->qinh);
     free(S->Vd); free(S->Ied); free(S->Iid); free(S->bexcd); free(S->binhd); free(S->wd);
     free(S->refr); free(S->last); free(S->cnt); free(S->fired); free(S->src_of);
+    free(S->mt_col); free(S->mt_dly); free(S->mt_w);
     free(S);
 }
 
 /* ---- one step, FP32 state (ORC_ACC_F32 / ORC_ACC_FIXED) -------------------- */
-static void step_f32(orc_sim* S, int32_t s) {
-    const int32_t D = S->D, n = S->n, slot = s % D;
-    const orc_csr* c = S->csr;
-    /* neuron phase: mgcore.cu:116-234 */
-    for (int32_t i = 0; i < n; ++i) {
+/* Neuron i's update at step s (mgcore.cu:116-234): reads and clears its ring
+ * slot, sets fired[i]. */
+static inline void neuron_f32(orc_sim* S, int32_t i, int32_t s) {
+    const int32_t D = S->D, slot = s % D;
+    {
         const float* C = S->lc + 30 * S->cset[i];
         size_t loc = (size_t)i * (size_t)D + (size_t)slot;      /* mgcore.cu:96 */
         float be, bi;
@@ -566,6 +571,13 @@ static void step_f32(orc_sim* S, int32_t s) {
             else { S->bexc[loc] = 0.0f; S->binh[loc] = 0.0f; }
         }
     }
+}
+
+static void step_f32(orc_sim* S, int32_t s) {
+    const int32_t D = S->D, n = S->n;
+    const orc_csr* c = S->csr;
+    /* neuron phase: mgcore.cu:116-234 */
+    for (int32_t i = 0; i < n; ++i) neuron_f32(S, i, s);
     /* propagation phase: mgcore.cu:271-305, ascending source id, CSR order */
     for (int32_t i = 0; i < n; ++i) {
         if (!S->fired[i]) continue;
@@ -715,6 +727,106 @@ int64_t orc_run(orc_sim* S, int32_t nsteps, int32_t* ids, int64_t cap, int64_t* 
     }
     if (step_off) step_off[nsteps] = w;
     return overflow ? -1 : w;
+}
+
+/* Multithreaded step loop of a static FP32 network (timing baseline only, not a
+ * checker).  The neuron phase runs in parallel over neurons.  For propagation
+ * every CSR row is kept sorted by target (a stable sort, so the synapses of one
+ * (source, target) pair keep list order) and each thread owns a contiguous
+ * target range, which it finds in every fired row by binary search: each target
+ * receives its additions in ascending source order, as in the sequential loop,
+ * so the results are the sequential oracle's, without atomics. */
+static void mt_sort_rows(orc_sim* S, int32_t nthreads) {
+    const orc_csr* c = S->csr;
+    const size_t ss = (size_t)(c->s ? c->s : 1);
+    S->mt_col = (int32_t*)malloc(ss * sizeof(int32_t));
+    S->mt_dly = (int32_t*)malloc(ss * sizeof(int32_t));
+    S->mt_w = (float*)malloc(ss * sizeof(float));
+    int64_t maxlen = 1;
+    for (int32_t i = 0; i < S->n; ++i)
+        if (c->row_ptr[i + 1] - c->row_ptr[i] > maxlen) maxlen = c->row_ptr[i + 1] - c->row_ptr[i];
+    int passes = 1;                                  /* 11-bit digits covering ids < n */
+    while (passes < 3 && ((int64_t)1 << (11 * passes)) < (int64_t)S->n) ++passes;
+#pragma omp parallel num_threads(nthreads)
+    {
+        uint32_t* idx = (uint32_t*)malloc((size_t)maxlen * sizeof(uint32_t));
+        uint32_t* tmp = (uint32_t*)malloc((size_t)maxlen * sizeof(uint32_t));
+        uint32_t cnt[2048];
+#pragma omp for schedule(dynamic, 64)
+        for (int32_t i = 0; i < S->n; ++i) {
+            const int64_t b = c->row_ptr[i], len = c->row_ptr[i + 1] - b;
+            for (int64_t j = 0; j < len; ++j) idx[j] = (uint32_t)j;
+            for (int p = 0; p < passes; ++p) {       /* LSD radix sort by target: stable */
+                const int sh = 11 * p;
+                memset(cnt, 0, sizeof(cnt));
+                for (int64_t j = 0; j < len; ++j) cnt[((uint32_t)c->col[b + idx[j]] >> sh) & 2047u]++;
+                uint32_t acc = 0;
+                for (int d = 0; d < 2048; ++d) {
+                    const uint32_t x = cnt[d];
+                    cnt[d] = acc;
+                    acc += x;
+                }
+                for (int64_t j = 0; j < len; ++j) tmp[cnt[((uint32_t)c->col[b + idx[j]] >> sh) & 2047u]++] = idx[j];
+                uint32_t* t = idx;
+                idx = tmp;
+                tmp = t;
+            }
+            for (int64_t j = 0; j < len; ++j) {
+                const int64_t k = b + idx[j];
+                S->mt_col[b + j] = c->col[k];
+                S->mt_dly[b + j] = c->dly[k];
+                S->mt_w[b + j] = S->w[k];
+            }
+        }
+        free(idx);
+        free(tmp);
+    }
+}
+
+static int64_t lower_bound32(const int32_t* a, int64_t lo, int64_t hi, int32_t v) {
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+int64_t orc_run_mt(orc_sim* S, int32_t nsteps, int32_t nthreads) {
+    if (S->acc != ORC_ACC_F32 || S->stdp || nthreads < 1) return -1;
+    const int32_t n = S->n, D = S->D;
+    const orc_csr* c = S->csr;
+    if (!S->mt_col) mt_sort_rows(S, nthreads);
+    int64_t spikes = 0, se = 0;
+    for (int32_t st = 0; st < nsteps; ++st) {
+        const int32_t s = (int32_t)(S->step + st);
+#pragma omp parallel num_threads(nthreads) reduction(+ : spikes, se)
+        {
+#pragma omp for schedule(static, 1024)
+            for (int32_t i = 0; i < n; ++i) neuron_f32(S, i, s);
+            /* the implicit barrier of the loop above: fired[] is complete */
+            const int32_t nt = omp_get_num_threads(), me = omp_get_thread_num();
+            const int32_t t_lo = (int32_t)((int64_t)n * me / nt), t_hi = (int32_t)((int64_t)n * (me + 1) / nt);
+            for (int32_t i = 0; i < n; ++i) {
+                if (!S->fired[i]) continue;
+                if (me == 0) {
+                    ++spikes;
+                    se += c->row_ptr[i + 1] - c->row_ptr[i];
+                }
+                const int64_t b = lower_bound32(S->mt_col, c->row_ptr[i], c->row_ptr[i + 1], t_lo);
+                const int64_t e = lower_bound32(S->mt_col, b, c->row_ptr[i + 1], t_hi);
+                for (int64_t k = b; k < e; ++k) {
+                    const float w = S->mt_w[k];
+                    const size_t loc = (size_t)S->mt_col[k] * (size_t)D + (size_t)((s + S->mt_dly[k] + D) % D);
+                    if (w > 0) S->bexc[loc] += w;
+                    else S->binh[loc] += w;
+                }
+            }
+        }
+    }
+    S->step += nsteps;
+    S->se += se;
+    return spikes;
 }
 
 void orc_get_state(const orc_sim* S, int field, double* out) {

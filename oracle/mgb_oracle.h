@@ -133,6 +133,9 @@ orc_sim* orc_create(const orc_desc* d);
  * ids (capacity cap); step_off[k] is the offset of step k (nsteps+1 entries).
  * Returns the number of spikes written, or -1 if cap was exceeded. */
 int64_t orc_run(orc_sim* s, int32_t nsteps, int32_t* ids, int64_t cap, int64_t* step_off);
+/* Multithreaded FP32 step loop of a static network (OpenMP; timing baseline only,
+ * not a checker): returns the spike count, -1 for other modes. */
+int64_t orc_run_mt(orc_sim* S, int32_t nsteps, int32_t nthreads);
 enum { ORC_F_V = 0, ORC_F_IEXC = 1, ORC_F_IINH = 2, ORC_F_REFR = 3, ORC_F_LAST = 4, ORC_F_CNT = 5 };
 /* Float fields are written as double. */
 void orc_get_state(const orc_sim* s, int field, double* out);

@@ -95,6 +95,8 @@ def lib():
         L.orc_create.restype = C.c_void_p
         L.orc_run.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int64, _i64p]
         L.orc_run.restype = C.c_int64
+        L.orc_run_mt.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.orc_run_mt.restype = C.c_int64
         L.orc_get_state.argtypes = [C.c_void_p, C.c_int, _f64p]
         L.orc_get_weights.argtypes = [C.c_void_p, _f64p]
         L.orc_syn_events.argtypes = [C.c_void_p]
@@ -303,6 +305,13 @@ class OracleSim:
         if w < 0:
             raise RuntimeError("spike capacity exceeded")
         return [ids[off[k]:off[k + 1]].copy() for k in range(nsteps)]
+
+    def run_mt(self, nsteps, nthreads):
+        """Multithreaded FP32 steps (timing baseline; static networks): spike count."""
+        n = lib().orc_run_mt(self.h, nsteps, nthreads)
+        if n < 0:
+            raise RuntimeError("run_mt needs an FP32 oracle without STDP")
+        return int(n)
 
     def state(self, field_):
         out = np.zeros(self.n, dtype=np.float64)

@@ -189,3 +189,22 @@ def test_brunel_rate_matches_reference_benchdata():
     assert abs(cnt[:P].mean() - 20.0) < 0.3              # Poisson inputs at p = 0.002 -> 20 Hz
     rate = cnt.sum() / net.n / 1.0
     assert abs(rate - 27.1845) / 27.1845 < 0.04, rate
+
+
+def test_multithreaded_step_loop_equals_sequential():
+    """The oracle's OpenMP step loop (the all-cores CPU baseline) leaves the sequential
+    loop's state on a Brunel network, where every (target, sign) has one weight."""
+    from tests import harness as H
+    from tests import netmodels as M
+    net = M.config1(scale=0.3, poisson_rate=0.01)
+    syn = H.host_synapses(net, seed=3)
+    _, o1 = H.oracle(net, syn, seed=3)
+    _, o2 = H.oracle(net, syn, seed=3)
+    o1.run(1500)
+    n = o2.run_mt(1500, 4)
+    assert n == int(o1.state(po.F_CNT).sum()) > 1000
+    assert np.array_equal(o1.state(po.F_CNT), o2.state(po.F_CNT))
+    assert np.array_equal(o1.state(po.F_V), o2.state(po.F_V))
+    assert o1.syn_events() == o2.syn_events()
+    o1.close()
+    o2.close()
