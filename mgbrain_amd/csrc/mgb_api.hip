@@ -93,6 +93,7 @@ struct mgb_sim {
     int32_t nseg = 1;
     std::vector<int32_t> h_seg_lo;     // [nseg+1] segment boundaries (independent networks)
     int32_t* tile_lo = nullptr;        // [G+1]
+    uint32_t* tile_of = nullptr;       // [n] tile of each neuron
     uint16_t* tile_seg = nullptr;      // [G]
     int32_t *seg_lo = nullptr, *seg_tile0 = nullptr;   // [nseg+1]
     uint32_t* seg_kbase = nullptr;     // [nseg+1]
@@ -323,53 +324,111 @@ uint64_t host_u64(const void* p) {
     return v;
 }
 
-// Tile size of the fused path: ~256 tiles to fill the chip, bounded by LDS.
-// Tile shape of the fused path.  Tiles never straddle a segment boundary:
-// segment q gets ceil(Nq / B) tiles of B targets (the last one shorter).
-mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D) {
+// Tile shape of the fused path.  Tiles never straddle a segment boundary and
+// never mix neuron types (one update path per tile).  Within a segment they are
+// balanced by weight — a neuron weighs its in-degree plus half the mean
+// in-degree of the neurons that have one (its own update) — so tiles of neurons
+// without input (Poisson) are larger and the others share the synapses evenly.
+// A segment of Nq neurons gets at most max(ceil(Nq / B), its type runs) tiles
+// (B the requested mean tile size, default max(256, n / 256)) of at most the
+// LDS limit; the smallest per-tile weight cap meeting that is found by bisection.
+mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D, const std::vector<uint8_t>& type,
+                        const std::vector<uint32_t>& indeg) {
     int lds_max = 65536;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
     lds_max = std::min(lds_max, 160 * 1024);
     // the kernel's static LDS (block scan sums, event counter) shares the limit
     const int64_t per_b = (int64_t)(tiled_lds_bytes(D, 1, s->tkind) - tiled_lds_bytes(D, 0, s->tkind));
     int32_t bmax = (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0, s->tkind) - kTiledStaticLds) / per_b);
-    // CLASS words hold the global target id; wide words an 11-bit tile-local one
-    bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, s->tkind == kTClass ? (1 << 14) : 2048));
-    int32_t B = (o && o->tile_size > 0) ? o->tile_size : std::max<int32_t>(256, (n + 255) / 256);
-    B = std::max<int32_t>(1, std::min(B, bmax));
+    // wide words hold an 11-bit tile-local target; CLASS tiles of at most 1024
+    // targets take 16-bit words (else 32-bit words with the global id)
+    bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, s->tkind == kTClass ? 1024 : 2048));
+    int32_t Breq = (o && o->tile_size > 0) ? o->tile_size : std::max<int32_t>(256, (n + 255) / 256);
+    Breq = std::max<int32_t>(1, Breq);
     const std::vector<int32_t>& lo = s->h_seg_lo;
     const int32_t nseg = (int32_t)lo.size() - 1;
-    int32_t seg_max = 0;
-    for (int32_t q = 0; q < nseg; ++q) seg_max = std::max(seg_max, lo[q + 1] - lo[q]);
-    B = std::min(B, std::max(seg_max, 1));
+    std::vector<double> wt((size_t)n);
+    double sum_in = 0.0;
+    int64_t cnt_in = 0;
+    for (int32_t t = 0; t < n; ++t) {
+        const uint32_t d = indeg[t] + indeg[(size_t)n + t];
+        if (d) {
+            sum_in += d;
+            ++cnt_in;
+        }
+    }
+    const double base = cnt_in ? 0.5 * sum_in / (double)cnt_in : 1.0;
+    for (int32_t t = 0; t < n; ++t) wt[t] = (double)(indeg[t] + indeg[(size_t)n + t]) + base;
+    // tiles of [a, b) under weight cap T; their starts appended to `cuts`
+    auto greedy = [&](int32_t a, int32_t b, double T, std::vector<int32_t>* cuts) {
+        int32_t count = 0, start = a;
+        double acc = 0.0;
+        for (int32_t i = a; i < b; ++i) {
+            if (i > start && (type[i] != type[i - 1] || i - start >= bmax || acc + wt[i] > T)) {
+                if (cuts) cuts->push_back(start);
+                ++count;
+                start = i;
+                acc = 0.0;
+            }
+            acc += wt[i];
+        }
+        if (b > a) {
+            if (cuts) cuts->push_back(start);
+            ++count;
+        }
+        return count;
+    };
+    std::vector<int32_t> tlo;
+    std::vector<int32_t> tile0(nseg + 1, 0);
+    for (int32_t q = 0; q < nseg; ++q) {
+        const int32_t a = lo[q], b = lo[q + 1];
+        tile0[q] = (int32_t)tlo.size();
+        if (b <= a) continue;
+        const int32_t target = (int32_t)(((int64_t)(b - a) + Breq - 1) / Breq);
+        double t_lo = 0.0, t_hi = 0.0;
+        for (int32_t i = a; i < b; ++i) {
+            t_lo = std::max(t_lo, wt[i]);
+            t_hi += wt[i];
+        }
+        if (greedy(a, b, t_lo, nullptr) > target) {
+            for (int it = 0; it < 40 && t_hi - t_lo > 1e-9 * t_hi; ++it) {
+                const double mid = 0.5 * (t_lo + t_hi);
+                if (greedy(a, b, mid, nullptr) > target) t_lo = mid;
+                else t_hi = mid;
+            }
+        } else {
+            t_hi = t_lo;
+        }
+        greedy(a, b, t_hi, &tlo);
+    }
+    tile0[nseg] = (int32_t)tlo.size();
+    std::vector<uint16_t> tseg(tlo.size());
+    for (int32_t q = 0; q < nseg; ++q)
+        for (int32_t g = tile0[q]; g < tile0[q + 1]; ++g) tseg[g] = (uint16_t)q;
+    tlo.push_back(n);
+    int32_t B = 1;
+    for (size_t g = 0; g + 1 < tlo.size(); ++g) B = std::max(B, tlo[g + 1] - tlo[g]);
     s->B = B;
+    std::vector<uint32_t> tof((size_t)n);
+    for (size_t g = 0; g + 1 < tlo.size(); ++g)
+        for (int32_t t = tlo[g]; t < tlo[g + 1]; ++t) tof[t] = (uint32_t)g;
     s->chunk = (o && (o->chunk == 1 || o->chunk == 2 || o->chunk == 4 || o->chunk == 16)) ? o->chunk : 8;
     s->tblock = (o && o->prop_block >= 64 && o->prop_block <= 1024) ? (o->prop_block / 64) * 64 : 512;
-    std::vector<int32_t> tlo, tile0(nseg + 1, 0);
-    std::vector<uint16_t> tseg;
     std::vector<uint32_t> kbase(nseg + 1, 0);
     std::vector<uint64_t> obase(nseg + 1, 0);
     uint64_t kb = 0, ob = 0;
     for (int32_t q = 0; q < nseg; ++q) {
         const int32_t nq = lo[q + 1] - lo[q];
-        const int32_t gq = (nq + B - 1) / B;
-        tile0[q] = (int32_t)tlo.size();
+        const int32_t gq = tile0[q + 1] - tile0[q];
         kbase[q] = (uint32_t)kb;
         obase[q] = ob;
-        for (int32_t k = 0; k < gq; ++k) {
-            tlo.push_back(lo[q] + k * B);
-            tseg.push_back((uint16_t)q);
-        }
         kb += (uint64_t)nq * (uint64_t)gq;
         ob += (uint64_t)nq * (uint64_t)(gq + 1);
         if (kb >= 0xFFFFFFFFull) {
-            set_error("tiled propagation: %d neurons x %d tiles exceed 32-bit keys", n,
-                      (int32_t)tlo.size());
+            set_error("tiled propagation: %d neurons x %d tiles exceed 32-bit keys", n, tile0[nseg]);
             return MGB_ERR_UNSUPPORTED;
         }
     }
-    tlo.push_back(n);
-    tile0[nseg] = (int32_t)tseg.size();
     kbase[nseg] = (uint32_t)kb;
     obase[nseg] = ob;
     s->G = (int32_t)tseg.size();
@@ -378,6 +437,8 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D) {
     s->n_keys = kb;
     hipStream_t st = s->stream;
     MGB_TRY(dalloc_n(s, &s->tile_lo, tlo.size()));
+    MGB_TRY(dalloc_n(s, &s->tile_of, (size_t)n));
+    MGB_HIP_TRY(hipMemcpyAsync(s->tile_of, tof.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
     MGB_TRY(dalloc_n(s, &s->tile_seg, tseg.size()));
     MGB_TRY(dalloc_n(s, &s->seg_lo, lo.size()));
     MGB_TRY(dalloc_n(s, &s->seg_tile0, tile0.size()));
@@ -401,6 +462,7 @@ TileTabs tile_tabs(const mgb_sim* s, const uint16_t* seg_dev) {
     tt.seg_of = s->nseg > 1 ? seg_dev : nullptr;
     tt.seg_lo = s->seg_lo; tt.seg_tile0 = s->seg_tile0;
     tt.seg_kbase = s->seg_kbase; tt.seg_obase = s->seg_obase;
+    tt.tile_of = s->tile_of; tt.tile_lo = s->tile_lo;
     return tt;
 }
 
@@ -635,7 +697,7 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         s->tiled = (prop == MGB_PROP_AUTO || prop == MGB_PROP_TILED) ? 1 : 0;
         s->tkind = kTClass;
         if (s->tiled) {
-            MGB_TRY(choose_tiles(s, o, n, s->D));
+            MGB_TRY(choose_tiles(s, o, n, s->D, h.type, h_indeg));
             // tile-sorted copy: stable sort by (source, target tile) keeps list order
             // inside every (source, tile) sub-row
             const TileTabs tt = tile_tabs(s, seg_dev);
@@ -729,7 +791,7 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         if (s->tiled) {
             s->tkind = h.stdp ? kTStdp
                               : (s->wide_kind == kWideBF16 ? kTBF16 : (s->wide_kind == kWideFP8 ? kTFP8 : kTF32));
-            MGB_TRY(choose_tiles(s, o, n, s->D));
+            MGB_TRY(choose_tiles(s, o, n, s->D, h.type, h_indeg));
             const TileTabs tt = tile_tabs(s, seg_dev);
             // kout holds each CSR slot's source (the keys of the sort by source)
             MGB_HIP_TRY(make_tile_keys_csr(kout, s->col, tt, kin, vin, S, st));

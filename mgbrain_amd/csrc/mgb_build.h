@@ -40,6 +40,8 @@ struct TileTabs {
     const int32_t* seg_tile0;   // [nseg+1]
     const uint32_t* seg_kbase;  // [nseg+1] first sort key of each segment
     const uint64_t* seg_obase;  // [nseg+1] first offset entry of each segment
+    const uint32_t* tile_of;    // [n] tile of each neuron
+    const int32_t* tile_lo;     // [G+1] first neuron of each tile
 };
 
 struct GatherArgs {

@@ -109,7 +109,7 @@ __global__ void k_tile_keys(const int32_t* src, const int32_t* tar, const float*
         const int32_t q = tt.seg_of ? (int32_t)tt.seg_of[s] : 0;
         const int32_t lo = tt.seg_lo[q];
         const uint32_t gq = (uint32_t)(tt.seg_tile0[q + 1] - tt.seg_tile0[q]);
-        keys[i] = tt.seg_kbase[q] + (uint32_t)(s - lo) * gq + (uint32_t)((t - lo) / tt.B);
+        keys[i] = tt.seg_kbase[q] + (uint32_t)(s - lo) * gq + (tt.tile_of[t] - (uint32_t)tt.seg_tile0[q]);
         const uint32_t inh = (w[i] > 0.f) ? 0u : 1u;
         vals[i] = (uint32_t)t | ((uint32_t)(dq[i] - 1u) << 26) | (inh << 31);
     }
@@ -122,8 +122,7 @@ __global__ void k_class16(const uint32_t* w32, uint64_t S, TileTabs tt, uint16_t
          k += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t w = w32[k];
         const int32_t t = (int32_t)(w & 0x3FFFFFFu);
-        const int32_t q = tt.seg_of ? (int32_t)tt.seg_of[t] : 0;
-        const uint32_t loc = (uint32_t)((t - tt.seg_lo[q]) % tt.B);
+        const uint32_t loc = (uint32_t)(t - tt.tile_lo[tt.tile_of[t]]);
         out[k] = (uint16_t)(loc | (((w >> 26) & 31u) << 10) | ((w >> 31) << 15));
     }
 }
@@ -138,7 +137,7 @@ __global__ void k_tile_keys_csr(const uint32_t* src, const int32_t* col, TileTab
         const int32_t q = tt.seg_of ? (int32_t)tt.seg_of[s] : 0;
         const int32_t lo = tt.seg_lo[q];
         const uint32_t gq = (uint32_t)(tt.seg_tile0[q + 1] - tt.seg_tile0[q]);
-        keys[k] = tt.seg_kbase[q] + (uint32_t)(s - lo) * gq + (uint32_t)((t - lo) / tt.B);
+        keys[k] = tt.seg_kbase[q] + (uint32_t)(s - lo) * gq + (tt.tile_of[t] - (uint32_t)tt.seg_tile0[q]);
         vals[k] = (uint32_t)k;
     }
 }
@@ -153,8 +152,7 @@ __global__ void k_tile_gather_wide(const uint32_t* slot, uint64_t S, const int32
          k += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t p = slot[k];
         const int32_t t = col[p];
-        const int32_t q = tt.seg_of ? (int32_t)tt.seg_of[t] : 0;
-        const uint32_t loc = (uint32_t)((t - tt.seg_lo[q]) % tt.B);
+        const uint32_t loc = (uint32_t)(t - tt.tile_lo[tt.tile_of[t]]);
         const uint32_t d = (uint32_t)(dly[p] & 127u);
         uint32_t payload = 0u;
         if (kind == kTBF16) payload = w16[p];

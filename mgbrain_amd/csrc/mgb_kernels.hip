@@ -654,11 +654,11 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // their latency overlaps it
     uint32_t pbeg = 0u, plm = 0u, paux = 0u;
     if ((uint32_t)tid < min((uint32_t)bd, E)) meta((uint32_t)tid, pbeg, plm, paux);
-    if (full) ring_slice_in<RW>(ring, gring, D, B, rprev, nprev);
+    if (full) ring_slice_in<RW>(ring, gring, D, tn, rprev, nprev);
     // a tile of at most one neuron per thread: the neuron's state and out-degree are
     // loaded now (after the ring copy, which they would delay), their latency
     // hidden by the propagation phase
-    const bool pre = B <= bd && full && !(kPl && CH >= 16);   // (not where registers would spill)
+    const bool pre = tn <= bd && full && !(kPl && CH >= 16);  // (not where registers would spill)
     NState pst;
     int ptype = T_RAW;
     int32_t pdeg = 0;
@@ -793,13 +793,13 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             int32_t slot = sD[h] + (int32_t)((xw >> 26) & 31u) + 1;
                             slot -= (slot >= D) ? D : 0;
                             const int32_t loc = (int32_t)(xw & kIdMask) - t0;
-                            const int32_t at = idle ? (tid % B) : slot * B + loc;
+                            const int32_t at = idle ? (tid % tn) : slot * tn + loc;
                             atomicAdd(&ring[at], valid ? ((xw >> 31) ? 0x10000u : 1u) : 0u);
                         } else if (WK == kTClass16) {
                             int32_t slot = sD[h] + (int32_t)((xw >> 10) & 31u) + 1;
                             slot -= (slot >= D) ? D : 0;
                             const int32_t loc = (int32_t)(xw & 0x3FFu);
-                            const int32_t at = idle ? (tid % B) : slot * B + loc;
+                            const int32_t at = idle ? (tid % tn) : slot * tn + loc;
                             atomicAdd(&ring[at], valid ? ((xw >> 15) ? 0x10000u : 1u) : 0u);
                         } else {
                             int32_t slot = sD[h] + (int32_t)((xw >> 11) & 31u) + 1;
@@ -822,7 +822,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             }
                             const bool exc = w > 0.f;
                             const int32_t v = (int32_t)rintf(w * fsc.x);
-                            const int32_t at = idle ? (tid % B) : ((exc ? 0 : D) + slot) * B + loc;
+                            const int32_t at = idle ? (tid % tn) : ((exc ? 0 : D) + slot) * tn + loc;
                             if (full) atomicAdd(reinterpret_cast<int32_t*>(ring) + at, valid ? v : 0);
                             if (pl && valid)                           // axon update of t2
                                 a.w_t[kb[h] + (uint32_t)u] = clipw(w + ax[h][u], a.w_max, a.w_min);
@@ -862,7 +862,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     uint32_t* wbuf = a.win_buf + (size_t)bcur * a.win_stride + lbase;
     const int32_t slot0 = (int32_t)(s0 % D);
     unsigned long long se_local = 0ull;
-    for (int32_t j0 = 0; j0 < B; j0 += bd) {
+    for (int32_t j0 = 0; j0 < tn; j0 += bd) {
         const int32_t j = j0 + tid;
         const bool active = j < tn;
         const int32_t i = t0 + j;
@@ -897,7 +897,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             };
             if (lean) {
                 if (active) {
-                    uint32_t* r = &ring[slot * B + j];
+                    uint32_t* r = &ring[slot * tn + j];
                     const uint32_t x = *r;
                     *r = 0u;                                          // clearInput
                     float be, bi;
@@ -908,14 +908,14 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                 fired = poisson_step(st, s, w);
             } else if (integ) {
                 float be, bi;
-                uint32_t* r = &ring[slot * B + j];
+                uint32_t* r = &ring[slot * tn + j];
                 if (kCls) {
                     const uint32_t x = *r;
                     *r = 0u;                                          // clearInput
                     if (csl >= 0) cached_input(x, be, bi);
                     else class_input(x, a.seq, st, be, bi);
                 } else {
-                    uint32_t* ri = r + (size_t)D * B;
+                    uint32_t* ri = r + (size_t)D * tn;
                     const int32_t qe = (int32_t)*r, qi = (int32_t)*ri;
                     *r = 0u;
                     *ri = 0u;
@@ -986,7 +986,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     }
     // 4. LDS -> ring slice, except the slots this window consumed (cleared above,
     // or never read for neurons that do not integrate)
-    ring_slice_out<RW>(ring, gring, D, B, (int32_t)(s0 % D), nsteps);
+    ring_slice_out<RW>(ring, gring, D, tn, (int32_t)(s0 % D), nsteps);
 }
 
 // Flush of deferred dendrite updates (kTStdp): for each neuron j the fused
