@@ -49,8 +49,8 @@ def fused_roofline(b, info0, steps, se, spikes, n_lif, n_poisson, kname, lif_sta
     + 4 B list entry; per synaptic event 4 B target + 4 B FP32 weight + 1 B delay.
     layout_bytes_per_launch = what this layout itself must move: word_bytes per event,
     12 B per spike, 4 B per spike and tile of its segment, the ring rows live across
-    windows in and out, 32 B per LIFB (48 B LIFE, lif_layout) / 20 B per Poisson neuron of
-    state in and out."""
+    windows in and out (each tile's slice is as wide as the tile: one column per neuron), 32 B
+    per LIFB (48 B LIFE, lif_layout) / 20 B per Poisson neuron of state in and out."""
     launches = b["prop_launches"]
     if not launches or b["prop_ms"] <= 0:
         return None
@@ -60,7 +60,7 @@ def fused_roofline(b, info0, steps, se, spikes, n_lif, n_poisson, kname, lif_sta
     live = max(info0["ring_len"] - info0["window"], 0)
     g_seg = b["tiles"] / max(1, b["segments"])
     lay = (b["word_bytes"] * se + 12.0 * spikes + 4.0 * spikes * g_seg
-           + launches * (8.0 * rw * live * b["tiles"] * b["tile_size"] + lif_layout * n_lif + 20.0 * n_poisson))
+           + launches * (8.0 * rw * live * info0["n_neurons"] + lif_layout * n_lif + 20.0 * n_poisson))
     ach = alg / launches / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,

@@ -36,6 +36,7 @@ __global__ void k_stats(StatArgs a) {
     for (uint64_t it = 0; it < iters; ++it, i += stride) {
         const bool in = i < total;
         uint32_t key = 0xFFFFFFFFu;    // (target, class) of this lane; ~0 = none
+        uint32_t aseg = 0u, abits = 0u; // segment and |w| bits for the max |w| per segment
         if (in) {
             const int32_t s = a.src[i], t = a.tar[i];
             if ((uint32_t)s >= (uint32_t)a.n || (uint32_t)t >= (uint32_t)a.n) {
@@ -59,13 +60,23 @@ __global__ void k_stats(StatArgs a) {
                     const uint32_t old = atomicCAS(wc, kWSentinel, wb);
                     if (old != kWSentinel && old != wb) atomicOr(a.flags, kFlagMixed);
                 }
-                atomicMax(&a.absmax[a.seg ? a.seg[t] : 0], __float_as_uint(fabsf(w)));
+                aseg = a.seg ? a.seg[t] : 0u;
+                abits = __float_as_uint(fabsf(w));
                 if (a.plastic && a.plastic[i]) {
                     atomicAdd(a.n_plastic, 1ull);
                     atomicAdd(&a.in_pl[t], 1u);
                 }
                 key = ((uint32_t)t << 1) | cls;
             }
+        }
+        // max |w| per segment: one atomic per wave where its lanes share the segment
+        if (wave_all_equal(aseg)) {
+            uint32_t m = abits;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+            if ((threadIdx.x & 63u) == 0u && m) atomicMax(&a.absmax[aseg], m);
+        } else if (abits) {
+            atomicMax(&a.absmax[aseg], abits);
         }
         // in-degree per (target, class); target-major lists hit one address per wave
         if (wave_all_equal(key)) {
