@@ -141,3 +141,26 @@ def test_fused_stdp_conductance_neurons():
     sc = M.stdp_consts_config4(J=0.4)
     changed, n = _run(net, syn, sc, 800, 4, N.WF_F32, prop=N.PROP_TILED, window=8)
     assert changed > 100 and n > 500
+
+
+def test_fused_stdp_graph_replay_identical():
+    """Fused STDP replayed as hipGraphs (fused kernel + flush kernel per window) gives the
+    eager run's spikes and weights bit for bit, over runs that split windows."""
+    print("case: E->E STDP fused, graph replay vs eager", flush=True)
+    sys.stdout.flush()
+    net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
+    syn = H.host_synapses(net, seed=12)
+    sc = M.stdp_consts_config4(J=1e-4)
+    out = []
+    for graph in (0, 1):
+        g = H.gpu_from_list(net, syn, seed=12, stdp=True, stdpconst=sc, weight_format=N.WF_F32,
+                            record_capacity=4_000_000, propagation=N.PROP_TILED, use_graph=graph)
+        for n in (500, 503, 997):
+            g.run(n)
+        st, ids = g.spikes()
+        out.append((st, ids, g.weights(), g.fire_counts()))
+        g.close()
+    (s0, i0, w0, c0), (s1, i1, w1, c1) = out
+    assert np.array_equal(s0, s1) and np.array_equal(i0, i1)
+    assert np.array_equal(w0.view(np.uint32), w1.view(np.uint32)) and np.array_equal(c0, c1)
+    assert int(np.sum(c0 > 31)) > 50                          # flushes happened
