@@ -229,10 +229,10 @@ def batched_leg(args, shapes=None, log=None):
 
 def stdp_leg(args, log):
     """Brunel 100k (K = 10,000) with pair-based STDP on the E->E synapses.  --stdp-path
-    fused (default): k_tiled_stdp, one launch per window of min(min_delay, shortest
-    inter-spike interval of E neurons) = 10 steps, all weights FP32.  direct: a window of
-    one step (k_neurons + k_prop + k_stdp per step), static weights in the format chosen
-    by --precision, plastic weights FP32."""
+    fused (default): k_tiled<CH, kTStdp> ("k_tiled_stdp" below) + k_stdp_flush, one launch
+    each per window of min(min_delay, shortest inter-spike interval of E neurons) = 10
+    steps, all weights FP32.  direct: a window of one step (k_neurons + k_prop + k_stdp per
+    step), static weights in the format chosen by --precision, plastic weights FP32."""
     pops, projs, desc, ntot = W.brunel_fixed_k(args.scale, seed=args.seed, plastic_ee=True)
     sc, lc = W.stdp_consts(), W.lifb_consts()
     fused = args.stdp_path == "fused"

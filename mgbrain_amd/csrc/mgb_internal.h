@@ -222,8 +222,9 @@ struct TiledArgs {
     int32_t lut_n;
     int32_t lut_lds;            // leading entries of each table copied to LDS
     float w_max, w_min;
-    // kTStdp read-back: apply the pending window's STDP to w_t (a copy) and
-    // touch nothing else — no ring, neuron state, lists or counters
+    // kTStdp read-back: the launch applies the pending window's updates to w_t
+    // (a copy) and lists every neuron with deferred dendrite updates for the
+    // flush kernel, touching no ring, neuron state, spike list or tf / cntf
     int32_t apply_only;
 };
 
