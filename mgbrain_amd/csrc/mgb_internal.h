@@ -239,7 +239,7 @@ hipError_t launch_prop(int mode, int wide_kind, const PropArgs& a, int32_t off, 
 hipError_t launch_stdp(const StdpArgs& a, int32_t off, int32_t wk, int grid, hipStream_t st);
 hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
 constexpr int kTiledBatch = 1024;   // spikes staged in LDS per propagation batch
-constexpr int kTiledItems = 8192;   // work items staged in LDS per propagation round
+constexpr int kTiledItems = 16384;  // work items staged in LDS per propagation round
 constexpr int kTiledStaticLds = 1024; // bytes reserved for the fused kernel's static LDS
 constexpr int kClassCache = 32;     // k-fold class sums cached in LDS (tables hold >= this many)
 constexpr int kStdpHist = 16;       // spike steps kept per neuron for deferred dendrite updates
