@@ -587,11 +587,14 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const int bcur = (int)(kwin % 3), bprev = (int)((kwin + 2) % 3), bnext = (int)((kwin + 1) % 3);
     const int32_t t0 = a.tile_lo[g];
     const int32_t tn = a.tile_lo[g + 1] - t0;
-    const int32_t q = (int32_t)a.tile_seg[g];
-    const int32_t lt = g - a.seg_tile0[q];                  // tile index inside the segment
-    const int32_t slo = a.seg_lo[q];
-    const size_t G1 = (size_t)(a.seg_tile0[q + 1] - a.seg_tile0[q] + 1);
-    const size_t obase = (size_t)a.seg_obase[q] + (size_t)lt;
+    // the tile's segment (one segment: all of it known without loads)
+    const bool one = a.nseg == 1;
+    const int32_t q = one ? 0 : (int32_t)a.tile_seg[g];
+    const int32_t lt = one ? g : g - a.seg_tile0[q];        // tile index inside the segment
+    const int32_t slo = one ? 0 : a.seg_lo[q];
+    const int32_t nq = one ? a.n : a.seg_lo[q + 1] - slo;    // >= 1: tiles cover neurons
+    const size_t G1 = one ? (size_t)a.G + 1 : (size_t)(a.seg_tile0[q + 1] - a.seg_tile0[q] + 1);
+    const size_t obase = one ? (size_t)g : (size_t)a.seg_obase[q] + (size_t)lt;
     const size_t lbase = (size_t)slo * (size_t)a.window;    // segment's part of each spike list
     uint32_t* tcnt = a.tcnt + q;                            // tcnt[b * nseg]
     const int32_t nseg = a.nseg;
@@ -636,8 +639,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // metadata of list entry e: first slot of its sub-row in this tile, length |
     // ring slot of its spike step << 26, BF16 / FP8 row scale bits or (kTStdp)
     // the source's last spike before the window
-    auto meta = [&](uint32_t e, uint32_t& beg, uint32_t& lm, uint32_t& aux) {
-        const uint32_t ent = L[e];
+    auto meta = [&](uint32_t ent, uint32_t& beg, uint32_t& lm, uint32_t& aux) {
         const uint32_t src = ent & kIdMask;
         const size_t ob = obase + (size_t)(src - (uint32_t)slo) * G1;
         uint32_t o0, o1;
@@ -650,10 +652,13 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         else if (kPl) aux = (uint32_t)lastA[src];
         else aux = 0u;
     };
-    // the first batch's metadata loads are issued before the ring copy, so that
-    // their latency overlaps it
+    // the first batch's list entries are loaded before their count is known (the
+    // index clamped into the segment's part of the list, which holds nq * window
+    // entries), and their metadata loads before the ring copy, so that these
+    // latencies overlap; entries past the count are never used
     uint32_t pbeg = 0u, plm = 0u, paux = 0u;
-    if ((uint32_t)tid < min((uint32_t)bd, E)) meta((uint32_t)tid, pbeg, plm, paux);
+    const uint32_t ent0 = L[min((uint32_t)tid, (uint32_t)nq * (uint32_t)a.window - 1u)];
+    if ((uint32_t)tid < min((uint32_t)bd, E)) meta(ent0, pbeg, plm, paux);
     if (full) ring_slice_in<RW>(ring, gring, D, tn, rprev, nprev);
     // a tile of at most one neuron per thread: the neuron's state and out-degree are
     // loaded now (after the ring copy, which they would delay), their latency
@@ -675,7 +680,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         uint32_t items = 0u;
         if ((uint32_t)tid < nb) {
             uint32_t beg = pbeg, lm = plm, aux = paux;
-            if (b0 > 0) meta(b0 + (uint32_t)tid, beg, lm, aux);
+            if (b0 > 0) meta(L[b0 + (uint32_t)tid], beg, lm, aux);
             sp_beg[tid] = beg;
             sp_len[tid] = lm;
             if (kAux) sp_aux[tid] = aux;
