@@ -185,7 +185,10 @@ def batched_leg(args, shapes=None, log=None):
     if not args.no_batch_bf16:
         # the same batch with BF16 weights (per-source scales, fixed-point sums): rate error
         # of every network against the exact run above
-        sb = N.Simulator.batch_generated(nets, use_graph=args.graph, weight_format=N.WF_BF16)
+        table = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
+        tb = json.load(open(table)).get(f"brunel_sweep{len(nets)}_bf16") if os.path.exists(table) else None
+        shb = dict(tile_size=tb["tile_size"], prop_block=tb["block"], chunk=tb["chunk"]) if tb else {}
+        sb = N.Simulator.batch_generated(nets, use_graph=args.graph, weight_format=N.WF_BF16, **shb)
         ib = sb.info()
         sb.run(args.warmup)
         sb.sync()
