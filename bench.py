@@ -343,11 +343,14 @@ def coba_leg(args, log):
     pops, projs, desc, ntot = W.vogels_abbott(args.scale, seed=args.seed)
     lc = W.life_const1()
     nE = pops[0]["num"]
+    table = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
+    tc = json.load(open(table)).get("vogels_abbott80k_p0.02") if os.path.exists(table) else None
+    shc = dict(tile_size=tc["tile_size"], prop_block=tc["block"], chunk=tc["chunk"]) if tc and args.scale == 1.0 else {}
     out, ref = {}, None
     for name, wf in (("f32_class", N.WF_CLASS), ("f32_fixed", N.WF_F32), ("bf16", N.WF_BF16), ("fp8", N.WF_FP8)):
         t0 = time.perf_counter()
         sim = N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed, weight_format=wf,
-                                    use_graph=args.graph, profile=0 if args.no_profile else 1)
+                                    use_graph=args.graph, profile=0 if args.no_profile else 1, **shc)
         sim.sync()
         setup_s = time.perf_counter() - t0
         info0 = sim.info()
