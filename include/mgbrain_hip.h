@@ -63,7 +63,8 @@ typedef struct mgb_opts {
     int32_t prop_grid;         /* propagation workgroups; 0 = tuned default */
     int32_t profile;           /* time every kernel with hipEvents (direct launches only) */
     int32_t propagation;       /* MGB_PROP_* */
-    int32_t tile_size;         /* TILED: target neurons per workgroup; 0 = tuned default */
+    int32_t tile_size;         /* TILED: mean target neurons per workgroup (tiles are balanced by
+                                  in-degree and never mix neuron types); 0 = default */
     int32_t chunk;             /* TILED: synapses per propagation work item (1, 2, 4, 8, 16; 0 = 8) */
     int32_t reserved[4];
 } mgb_opts;
@@ -172,7 +173,7 @@ typedef struct mgb_info {
     double neuron_ms, prop_ms, stdp_ms;
     int64_t neuron_launches, prop_launches, stdp_launches;
     int32_t propagation;       /* MGB_PROP_* in use (TILED: the fused kernel's time is prop_ms) */
-    int32_t tile_size;
+    int32_t tile_size;         /* the widest tile */
     int32_t tiles;
     int32_t block;             /* workgroup size of the propagation / fused kernel */
     int32_t segments;          /* independent segments (networks of a batch) */
