@@ -1052,6 +1052,9 @@ TiledArgs tiled_args(const mgb_sim* s) {
         a.ltp = s->ltp; a.ltd = s->ltd; a.lut_n = s->lut_n; a.lut_lds = s->lut_lds;
         a.w_max = s->sc[4]; a.w_min = s->sc[5];
     }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess) cus = 0;
+    a.xcd_map = (int64_t)a.G <= (int64_t)cus * tiled_blocks_per_cu(a, s->tblock) ? 1 : 0;
     return a;
 }
 

@@ -226,6 +226,9 @@ struct TiledArgs {
     // (a copy) and lists every neuron with deferred dendrite updates for the
     // flush kernel, touching no ring, neuron state, spike list or tf / cntf
     int32_t apply_only;
+    // 1: workgroup b runs tile (b % kXcds) * (G / kXcds) + ... (a contiguous run of
+    // tiles per XCD), set when every workgroup of the launch is resident at once
+    int32_t xcd_map;
 };
 
 enum RingMode { kRingClass = 0, kRingFixed = 1 };
@@ -243,8 +246,11 @@ constexpr int kTiledItems = 16384;  // work items staged in LDS per propagation 
 constexpr int kTiledStaticLds = 1024; // bytes reserved for the fused kernel's static LDS
 constexpr int kClassCache = 32;     // k-fold class sums cached in LDS (tables hold >= this many)
 constexpr int kStdpHist = 16;       // spike steps kept per neuron for deferred dendrite updates
+constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 each
 // Dynamic LDS of the fused kernel; kTStdp also caches lut_lds entries of each STDP table.
 size_t tiled_lds_bytes(int D, int B, int kind, int lut_lds = 0);
+// Workgroups of `block` threads of the fused kernel for `a` that one CU holds at once.
+int tiled_blocks_per_cu(const TiledArgs& a, int block);
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st);
 hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);

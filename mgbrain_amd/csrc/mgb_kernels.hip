@@ -551,8 +551,20 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // last and first neurons (one class per tile in population-ordered networks)
     __shared__ float ccache[2][2][kClassCache];
     __shared__ uint32_t ccls[2];
-    const int32_t D = a.D, B = a.B, g = (int32_t)blockIdx.x, bd = (int32_t)blockDim.x;
+    const int32_t D = a.D, B = a.B, bd = (int32_t)blockDim.x;
     const int32_t tid = (int32_t)threadIdx.x;
+    // the workgroup's tile.  Workgroups go round-robin to the kXcds XCDs
+    // (blockIdx.x % kXcds), each with its own L2; neighbouring tiles share the
+    // cache lines at their sub-row boundaries and their row offsets, so each XCD
+    // takes one contiguous run of tiles (a bijection onto [0, G))
+    // (only when the whole grid is resident: a later workgroup waits for a CU of
+    // its own XCD, and tiles of unequal cost are then better spread over XCDs)
+    const int32_t g = [&] {
+        const int32_t b = (int32_t)blockIdx.x, G = (int32_t)gridDim.x;
+        if (!a.xcd_map) return b;
+        const int32_t x = b % kXcds, qx = G / kXcds, rx = G % kXcds;
+        return x * qx + min(x, rx) + b / kXcds;
+    }();
     constexpr bool kCls = WK == kTClass || WK == kTClass16;  // CLASS arrival counts
     constexpr bool kPl = WK == kTStdp;                      // STDP on w_t
     constexpr bool kAux = WK == kTFP8 || WK == kTBF16 || kPl;
@@ -581,7 +593,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const bool full = !kPl || a.apply_only == 0;
     Ctl* ctl = a.ctl;
     // fixed-point (scale, 1 / scale) of the tile's segment (scale_e == scale_i)
-    const float2 fsc = a.seg_scale ? a.seg_scale[a.tile_seg[blockIdx.x]] : make_float2(a.scale_e, a.inv_e);
+    const float2 fsc = a.seg_scale ? a.seg_scale[a.tile_seg[g]] : make_float2(a.scale_e, a.inv_e);
     const int64_t s0 = ctl->base_step + off;
     const int64_t kwin = ctl->base_window + wk;
     const int bcur = (int)(kwin % 3), bprev = (int)((kwin + 2) % 3), bnext = (int)((kwin + 1) % 3);
@@ -1176,6 +1188,36 @@ hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nst
         default: launch_tiled_kind<kTClass>(a, off, wk, nsteps, block, lds, st); break;
     }
     return hipGetLastError();
+}
+
+template <int WK>
+int blocks_per_cu_kind(int chunk, int block, size_t lds) {
+    const void* f;
+    switch (chunk) {
+        case 1: f = (const void*)k_tiled<1, WK>; break;
+        case 2: f = (const void*)k_tiled<2, WK>; break;
+        case 4: f = (const void*)k_tiled<4, WK>; break;
+        case 16: f = (const void*)k_tiled<16, WK>; break;
+        default: f = (const void*)k_tiled<8, WK>; break;
+    }
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return nb;
+}
+
+int tiled_blocks_per_cu(const TiledArgs& a, int block) {
+    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.kind == kTStdp ? a.lut_lds : 0);
+    switch (a.kind) {
+        case kTBF16: return blocks_per_cu_kind<kTBF16>(a.chunk, block, lds);
+        case kTFP8: return blocks_per_cu_kind<kTFP8>(a.chunk, block, lds);
+        case kTF32: return blocks_per_cu_kind<kTF32>(a.chunk, block, lds);
+        case kTClass16: return blocks_per_cu_kind<kTClass16>(a.chunk, block, lds);
+        case kTStdp: return blocks_per_cu_kind<kTStdp>(a.chunk, block, lds);
+        default: return blocks_per_cu_kind<kTClass>(a.chunk, block, lds);
+    }
 }
 
 hipError_t set_tiled_lds_limit(int kind, size_t bytes) {
