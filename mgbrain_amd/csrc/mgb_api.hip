@@ -92,6 +92,8 @@ struct mgb_sim {
     int32_t B = 0, G = 0, chunk = 8, tblock = 512;
     int32_t nseg = 1;
     std::vector<int32_t> h_seg_lo;     // [nseg+1] segment boundaries (independent networks)
+    std::vector<int32_t> h_seg_tile0;  // [nseg+1] first tile of each segment
+    std::vector<uint64_t> h_seg_obase; // [nseg+1] first offset entry of each segment
     int32_t* tile_lo = nullptr;        // [G+1]
     uint32_t* tile_of = nullptr;       // [n] tile of each neuron
     uint16_t* tile_seg = nullptr;      // [G]
@@ -109,7 +111,8 @@ struct mgb_sim {
     uint32_t* fcnt = nullptr;          // [2]
     int32_t stdp_isi = 1;              // kTStdp: shortest inter-spike interval of plastic synapses' ends
     uint32_t* syn_t = nullptr;
-    uint16_t* syn16 = nullptr;         // kTClass16 words in tile order
+    uint16_t* syn16 = nullptr;         // kTClass16 words in tile order (padded sub-rows)
+    int64_t* prow = nullptr;           // kTClass16: first slot of each padded row of syn16
     float* w_t = nullptr;              // kTF32 / kTStdp weights in tile order
     uint16_t* offs16 = nullptr;
     uint32_t* offs32 = nullptr;
@@ -431,6 +434,8 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D, con
     }
     kbase[nseg] = (uint32_t)kb;
     obase[nseg] = ob;
+    s->h_seg_tile0 = tile0;
+    s->h_seg_obase = obase;
     s->G = (int32_t)tseg.size();
     s->nseg = nseg;
     s->n_offs = ob;
@@ -484,6 +489,97 @@ mgb_status build_tile_offsets(mgb_sim* s, int32_t n, const uint32_t* sorted_keys
     if (max_row <= 0xFFFFu) MGB_TRY(dalloc_n(s, &s->offs16, no));
     else MGB_TRY(dalloc_n(s, &s->offs32, no));
     MGB_HIP_TRY(tile_offsets(sorted_keys, s->row_ptr, tt, s->n_offs, s->offs16, s->offs32, st));
+    return MGB_OK;
+}
+
+// The tile-order CLASS16 copy with every (source, tile) sub-row starting at a
+// multiple of kSubAlign words, so that a work item of the fused kernel is one or
+// two aligned vector loads.  Offset entry t of row m becomes the sub-row's start
+// inside the padded row, which begins at prow[m]; its low bits hold the padding
+// after sub-row t - 1, so that sub-row t is (e[t + 1] & ~7) - (e[t] & ~7) -
+// (e[t + 1] & 7) words long.  Padding words are zero; kSubAlign * 2 words of
+// slack follow the last row.
+mgb_status pad_subrows16(mgb_sim* s, int32_t n) {
+    hipStream_t st = s->stream;
+    const size_t no = (size_t)s->n_offs;
+    std::vector<uint32_t> eo(no), en(no);
+    if (s->offs16) {
+        std::vector<uint16_t> t(no);
+        MGB_HIP_TRY(hipMemcpyAsync(t.data(), s->offs16, 2 * no, hipMemcpyDeviceToHost, st));
+        MGB_HIP_TRY(hipStreamSynchronize(st));
+        for (size_t e = 0; e < no; ++e) eo[e] = t[e];
+    } else {
+        MGB_HIP_TRY(hipMemcpyAsync(eo.data(), s->offs32, 4 * no, hipMemcpyDeviceToHost, st));
+        MGB_HIP_TRY(hipStreamSynchronize(st));
+    }
+    const std::vector<int32_t>& lo = s->h_seg_lo;
+    const std::vector<int32_t>& t0 = s->h_seg_tile0;
+    const std::vector<uint64_t>& ob = s->h_seg_obase;
+    std::vector<int64_t> prow((size_t)n + 1, 0);
+    constexpr uint32_t A = (uint32_t)kSubAlign;
+    uint64_t p = 0, s_old = 0;
+    uint32_t emax = 0;
+    for (size_t q = 0; q + 1 < lo.size(); ++q) {
+        const uint32_t gq = (uint32_t)(t0[q + 1] - t0[q]);
+        for (int32_t m = lo[q]; m < lo[q + 1]; ++m) {
+            prow[m] = (int64_t)p;
+            const size_t e0 = ob[q] + (size_t)(m - lo[q]) * (gq + 1);
+            uint32_t r = 0, pad = 0;
+            for (uint32_t t = 0; t < gq; ++t) {
+                const uint32_t len = eo[e0 + t + 1] - eo[e0 + t];
+                en[e0 + t] = r | pad;
+                pad = (A - len % A) % A;
+                r += len + pad;
+                s_old += len;
+            }
+            en[e0 + gq] = r | pad;
+            emax = std::max(emax, r | pad);
+            p += r;
+        }
+    }
+    prow[n] = (int64_t)p;
+    const uint64_t total = p + 2 * A;
+    if (total > 0xFFFFFFFFull) {
+        set_error("tiled propagation: %llu padded synapse slots exceed 32-bit positions",
+                  (unsigned long long)total);
+        return MGB_ERR_UNSUPPORTED;
+    }
+    uint16_t* syn = nullptr;
+    uint32_t *deo = nullptr, *den = nullptr;
+    MGB_TRY(dalloc_n(s, &syn, (size_t)total));
+    MGB_TRY(dalloc_n(s, &s->prow, (size_t)n + 1));
+    MGB_TRY(dalloc_n(s, &deo, no));
+    MGB_TRY(dalloc_n(s, &den, no));
+    MGB_HIP_TRY(hipMemsetAsync(syn, 0, 2 * (size_t)total, st));
+    MGB_HIP_TRY(hipMemcpyAsync(s->prow, prow.data(), 8 * ((size_t)n + 1), hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(deo, eo.data(), 4 * no, hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(hipMemcpyAsync(den, en.data(), 4 * no, hipMemcpyHostToDevice, st));
+    MGB_HIP_TRY(pad_copy16(s->syn16, s->row_ptr, s->prow, deo, den, tile_tabs(s, nullptr), no, syn, st));
+    MGB_HIP_TRY(hipStreamSynchronize(st));
+    dfree(s, s->syn16);
+    dfree(s, deo);
+    s->bytes -= (int64_t)(2 * s_old + 4 * no);
+    s->syn16 = syn;
+    if (emax <= 0xFFFFu) {
+        std::vector<uint16_t> t(no);
+        for (size_t e = 0; e < no; ++e) t[e] = (uint16_t)en[e];
+        if (!s->offs16) {
+            dfree(s, s->offs32);
+            s->bytes -= (int64_t)(4 * no);
+            s->offs32 = nullptr;
+            MGB_TRY(dalloc_n(s, &s->offs16, no));
+        }
+        MGB_HIP_TRY(hipMemcpyAsync(s->offs16, t.data(), 2 * no, hipMemcpyHostToDevice, st));
+        MGB_HIP_TRY(hipStreamSynchronize(st));
+        dfree(s, den);
+        s->bytes -= (int64_t)(4 * no);
+    } else {
+        s->bytes -= (int64_t)((s->offs16 ? 2 : 4) * no);
+        dfree(s, s->offs16);
+        dfree(s, s->offs32);
+        s->offs16 = nullptr;
+        s->offs32 = den;
+    }
     return MGB_OK;
 }
 
@@ -723,6 +819,7 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
                 if (S) MGB_HIP_TRY(hipMemcpyAsync(s->syn_t, vout, S * 4, hipMemcpyDeviceToDevice, st));
             }
             MGB_TRY(build_tile_offsets(s, n, kout, tt));
+            if (s->tkind == kTClass16) MGB_TRY(pad_subrows16(s, n));
         }
         MGB_HIP_TRY(hipStreamSynchronize(st));
     } else {
@@ -1042,6 +1139,7 @@ TiledArgs tiled_args(const mgb_sim* s) {
     a.csets = s->csets; a.stream_base = s->stream_base; a.stream_seed = s->stream_seed;
     a.tbl_off = s->tbl_off; a.seq = s->seq; a.tbl_w = s->tbl_w;
     a.row_ptr = s->row_ptr; a.syn_t = s->syn_t; a.syn16 = s->syn16; a.offs16 = s->offs16; a.offs32 = s->offs32;
+    a.prow = s->prow;
     a.ring_t = s->ring_t; a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
     a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
     if (s->tkind == kTStdp) {

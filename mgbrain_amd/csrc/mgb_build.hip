@@ -240,6 +240,32 @@ __global__ void k_tile_offsets(const uint32_t* keys, const int64_t* row_ptr, Til
     }
 }
 
+// Copy of the tile-order CLASS16 words into the padded layout: offset entry e
+// (source m, local tile t < Gq) moves its sub-row from row_ptr[m] + eo[e] (the
+// unpadded offsets) to prow[m] + (en[e] & ~(kSubAlign - 1)).
+__global__ void k_pad_copy16(const uint16_t* src16, const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo,
+                             const uint32_t* en, TileTabs tt, uint64_t total, uint16_t* dst16) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        int32_t lo = 0, hi = tt.nseg;          // seg_obase[lo] <= e < seg_obase[hi]
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (tt.seg_obase[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        const int32_t q = lo;
+        const uint64_t r = e - tt.seg_obase[q];
+        const uint64_t gq = (uint64_t)(tt.seg_tile0[q + 1] - tt.seg_tile0[q]);
+        const uint64_t rel = r / (gq + 1);
+        if (r - rel * (gq + 1) == gq) continue;   // the row's end entry
+        const int32_t m = tt.seg_lo[q] + (int32_t)rel;
+        const uint32_t len = eo[e + 1] - eo[e];
+        const uint16_t* from = src16 + row_ptr[m] + eo[e];
+        uint16_t* to = dst16 + prow[m] + (en[e] & ~(uint32_t)(kSubAlign - 1));
+        for (uint32_t j = 0; j < len; ++j) to[j] = from[j];
+    }
+}
+
 __global__ void k_bounds(const uint32_t* keys, uint64_t S, int32_t n, int64_t* out) {
     const int32_t m = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (m > n) return;
@@ -475,6 +501,14 @@ hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, con
     else
         hipLaunchKernelGGL(k_tile_offsets<uint32_t>, dim3(blocks_for(total, 256)), dim3(256), 0, st,
                            sorted_keys, row_ptr, tt, total, offs32);
+    return hipGetLastError();
+}
+
+hipError_t pad_copy16(const uint16_t* src16, const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo,
+                      const uint32_t* en, const TileTabs& tt, uint64_t total, uint16_t* dst16, hipStream_t st) {
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pad_copy16, dim3(blocks_for(total, 256)), dim3(256), 0, st, src16, row_ptr, prow, eo, en,
+                       tt, total, dst16);
     return hipGetLastError();
 }
 

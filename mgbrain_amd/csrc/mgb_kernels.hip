@@ -24,6 +24,9 @@
 namespace mgb {
 namespace {
 
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
 __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
@@ -658,8 +661,16 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         if (a.offs16) { o0 = a.offs16[ob]; o1 = a.offs16[ob + 1]; }
         else { o0 = a.offs32[ob]; o1 = a.offs32[ob + 1]; }
         const int32_t sD = (int32_t)((sprev + (int64_t)(ent >> 26)) % D);
-        beg = (uint32_t)a.row_ptr[src] + o0;
-        lm = (o1 - o0) | ((uint32_t)sD << 26);
+        if (WK == kTClass16) {
+            // padded sub-rows: starts are multiples of kSubAlign, the low bits of the
+            // next entry hold the padding
+            constexpr uint32_t am = (uint32_t)kSubAlign - 1u;
+            beg = (uint32_t)a.prow[src] + (o0 & ~am);
+            lm = ((o1 & ~am) - (o0 & ~am) - (o1 & am)) | ((uint32_t)sD << 26);
+        } else {
+            beg = (uint32_t)a.row_ptr[src] + o0;
+            lm = (o1 - o0) | ((uint32_t)sD << 26);
+        }
         if (WK == kTFP8 || WK == kTBF16) aux = __float_as_uint(a.row_scale[src]);
         else if (kPl) aux = (uint32_t)lastA[src];
         else aux = 0u;
@@ -744,6 +755,37 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                 float wv[U][CH];
 #pragma unroll
                 for (int h = 0; h < U; ++h) {
+                    if constexpr (WK == kTClass16 && CH >= 2) {
+                        // the item's CH words as aligned vector loads (sub-rows start at
+                        // multiples of kSubAlign words; the words past the sub-row's end
+                        // are padding, the next sub-row or the slack after the last one)
+                        const uint16_t* p16 = a.syn16 + kb[h];
+                        if constexpr (CH == 2) {
+                            const uint32_t v = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p16));
+                            x[h][0] = v & 0xFFFFu;
+                            x[h][1] = v >> 16;
+                        } else if constexpr (CH == 4) {
+                            const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p16));
+                            x[h][0] = v.x & 0xFFFFu;
+                            x[h][1] = v.x >> 16;
+                            x[h][2] = v.y & 0xFFFFu;
+                            x[h][3] = v.y >> 16;
+                        } else {
+#pragma unroll
+                            for (int v8 = 0; v8 < CH / 8; ++v8) {
+                                const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p16) + v8);
+                                x[h][8 * v8 + 0] = v.x & 0xFFFFu;
+                                x[h][8 * v8 + 1] = v.x >> 16;
+                                x[h][8 * v8 + 2] = v.y & 0xFFFFu;
+                                x[h][8 * v8 + 3] = v.y >> 16;
+                                x[h][8 * v8 + 4] = v.z & 0xFFFFu;
+                                x[h][8 * v8 + 5] = v.z >> 16;
+                                x[h][8 * v8 + 6] = v.w & 0xFFFFu;
+                                x[h][8 * v8 + 7] = v.w >> 16;
+                            }
+                        }
+                        continue;
+                    }
                     const uint32_t last = max(cnt[h], 1u) - 1u;
 #pragma unroll
                     for (int u = 0; u < CH; ++u) {
@@ -816,7 +858,9 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             int32_t slot = sD[h] + (int32_t)((xw >> 10) & 31u) + 1;
                             slot -= (slot >= D) ? D : 0;
                             const int32_t loc = (int32_t)(xw & 0x3FFu);
-                            const int32_t at = idle ? (tid % tn) : slot * tn + loc;
+                            // words past the item's end are not this tile's: a distinct
+                            // in-slice entry per lane, adding 0
+                            const int32_t at = valid ? slot * tn + loc : (tid % tn);
                             atomicAdd(&ring[at], valid ? ((xw >> 15) ? 0x10000u : 1u) : 0u);
                         } else {
                             int32_t slot = sD[h] + (int32_t)((xw >> 11) & 31u) + 1;

@@ -1,6 +1,6 @@
 """Edge cases of the HIP path against the oracle (bit-exact spike lists and state):
 no synapses, a silent network, delays beyond the 5-bit field (direct path),
-one-neuron tiles, a one-neuron network with a self-synapse, a batch with an
+one-neuron tiles, rows of about 2^16 synapses, a one-neuron network with a self-synapse, a batch with an
 empty and a Poisson-only network, and ragged tile shapes."""
 import sys
 
@@ -103,6 +103,24 @@ def test_ragged_tiles(tile):
     assert sum(len(x) for x in want) > 500
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("n_tar", [65_530, 70_000])
+def test_long_rows(n_tar):
+    """Rows of about 2^16 synapses: the padded 16-bit tile-order copy needs 32-bit
+    row offsets (70,000: already unpadded; 65,530: only once padded)."""
+    _say(f"case: 4 Poisson sources x {n_tar} LIFB targets, tiled CLASS16")
+    n_src = 4
+    types = [M.POISSON] * n_src + [M.LIFB] * n_tar
+    rate = [0.2] * n_src + [0.0] * n_tar
+    rng = np.random.default_rng(5)
+    src = np.repeat(np.arange(n_src, dtype=np.int32), n_tar)
+    tar = np.tile(np.arange(n_src, n_src + n_tar, dtype=np.int32), n_src)
+    w = np.full(src.size, 4e-3, np.float32)
+    d = (rng.integers(10, 21, src.size) * 1e-4).astype(np.float32)
+    cnt, info = _arrays_vs_oracle(types, rate, W.lifb_consts(), src, tar, w, d, 300, propagation=N.PROP_TILED,
+                                  tile_size=1000)
+    assert info["word_bytes"] == 2 and cnt > n_tar
 
 
 def test_single_neuron_self_synapse():
