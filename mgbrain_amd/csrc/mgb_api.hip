@@ -112,7 +112,7 @@ struct mgb_sim {
     int32_t stdp_isi = 1;              // kTStdp: shortest inter-spike interval of plastic synapses' ends
     uint32_t* syn_t = nullptr;
     uint16_t* syn16 = nullptr;         // kTClass16 words in tile order (padded sub-rows)
-    int64_t* prow = nullptr;           // kTClass16: first slot of each padded row of syn16
+    int64_t* prow = nullptr;           // first slot of each padded row of syn16 / syn_t / w_t (not kTStdp)
     float* w_t = nullptr;              // kTF32 / kTStdp weights in tile order
     uint16_t* offs16 = nullptr;
     uint32_t* offs32 = nullptr;
@@ -492,14 +492,30 @@ mgb_status build_tile_offsets(mgb_sim* s, int32_t n, const uint32_t* sorted_keys
     return MGB_OK;
 }
 
-// The tile-order CLASS16 copy with every (source, tile) sub-row starting at a
-// multiple of kSubAlign words, so that a work item of the fused kernel is one or
-// two aligned vector loads.  Offset entry t of row m becomes the sub-row's start
-// inside the padded row, which begins at prow[m]; its low bits hold the padding
-// after sub-row t - 1, so that sub-row t is (e[t + 1] & ~7) - (e[t] & ~7) -
-// (e[t + 1] & 7) words long.  Padding words are zero; kSubAlign * 2 words of
-// slack follow the last row.
-mgb_status pad_subrows16(mgb_sim* s, int32_t n) {
+// Moves one tile-order array into the padded layout (`total` elements, zero
+// padding) and frees the unpadded one (`s_old` elements).
+template <class T>
+mgb_status repad(mgb_sim* s, T** arr, uint64_t total, uint64_t s_old, const uint32_t* deo, const uint32_t* den) {
+    hipStream_t st = s->stream;
+    T* to = nullptr;
+    MGB_TRY(dalloc_n(s, &to, (size_t)total));
+    MGB_HIP_TRY(hipMemsetAsync(to, 0, sizeof(T) * (size_t)total, st));
+    MGB_HIP_TRY(pad_copy<T>(*arr, s->row_ptr, s->prow, deo, den, tile_tabs(s, nullptr), s->n_offs, to, st));
+    MGB_HIP_TRY(hipStreamSynchronize(st));
+    dfree(s, *arr);
+    s->bytes -= (int64_t)(sizeof(T) * s_old);
+    *arr = to;
+    return MGB_OK;
+}
+
+// The tile-order copy (syn16, or syn_t and w_t) with every (source, tile)
+// sub-row starting at a multiple of kSubAlign words, so that a work item of the
+// fused kernel is aligned vector loads.  Offset entry t of row m becomes the
+// sub-row's start inside the padded row, which begins at prow[m]; its low bits
+// hold the padding after sub-row t - 1, so that sub-row t is (e[t + 1] & ~7) -
+// (e[t] & ~7) - (e[t + 1] & 7) words long.  Padding words are zero;
+// 2 * kSubAlign words of slack follow the last row.
+mgb_status pad_subrows(mgb_sim* s, int32_t n) {
     hipStream_t st = s->stream;
     const size_t no = (size_t)s->n_offs;
     std::vector<uint32_t> eo(no), en(no);
@@ -544,22 +560,18 @@ mgb_status pad_subrows16(mgb_sim* s, int32_t n) {
                   (unsigned long long)total);
         return MGB_ERR_UNSUPPORTED;
     }
-    uint16_t* syn = nullptr;
     uint32_t *deo = nullptr, *den = nullptr;
-    MGB_TRY(dalloc_n(s, &syn, (size_t)total));
     MGB_TRY(dalloc_n(s, &s->prow, (size_t)n + 1));
     MGB_TRY(dalloc_n(s, &deo, no));
     MGB_TRY(dalloc_n(s, &den, no));
-    MGB_HIP_TRY(hipMemsetAsync(syn, 0, 2 * (size_t)total, st));
     MGB_HIP_TRY(hipMemcpyAsync(s->prow, prow.data(), 8 * ((size_t)n + 1), hipMemcpyHostToDevice, st));
     MGB_HIP_TRY(hipMemcpyAsync(deo, eo.data(), 4 * no, hipMemcpyHostToDevice, st));
     MGB_HIP_TRY(hipMemcpyAsync(den, en.data(), 4 * no, hipMemcpyHostToDevice, st));
-    MGB_HIP_TRY(pad_copy16(s->syn16, s->row_ptr, s->prow, deo, den, tile_tabs(s, nullptr), no, syn, st));
-    MGB_HIP_TRY(hipStreamSynchronize(st));
-    dfree(s, s->syn16);
+    if (s->syn16) MGB_TRY(repad(s, &s->syn16, total, s_old, deo, den));
+    if (s->syn_t) MGB_TRY(repad(s, &s->syn_t, total, s_old, deo, den));
+    if (s->w_t) MGB_TRY(repad(s, reinterpret_cast<uint32_t**>(&s->w_t), total, s_old, deo, den));
     dfree(s, deo);
-    s->bytes -= (int64_t)(2 * s_old + 4 * no);
-    s->syn16 = syn;
+    s->bytes -= (int64_t)(4 * no);
     if (emax <= 0xFFFFu) {
         std::vector<uint16_t> t(no);
         for (size_t e = 0; e < no; ++e) t[e] = (uint16_t)en[e];
@@ -819,7 +831,7 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
                 if (S) MGB_HIP_TRY(hipMemcpyAsync(s->syn_t, vout, S * 4, hipMemcpyDeviceToDevice, st));
             }
             MGB_TRY(build_tile_offsets(s, n, kout, tt));
-            if (s->tkind == kTClass16) MGB_TRY(pad_subrows16(s, n));
+            MGB_TRY(pad_subrows(s, n));
         }
         MGB_HIP_TRY(hipStreamSynchronize(st));
     } else {
@@ -908,6 +920,9 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
             MGB_HIP_TRY(tile_gather_wide(vout, S, s->col, s->dly, s->w16, s->w8, s->w32, tt, s->tkind, s->syn_t,
                                          s->w_t, st));
             MGB_TRY(build_tile_offsets(s, n, kout, tt));
+            // (plastic weights keep their unpadded tile order: the in-synapse lists and
+            // the read-back permutation index it)
+            if (s->tkind != kTStdp) MGB_TRY(pad_subrows(s, n));
             if (s->tkind == kTStdp) {
                 // plastic weights live in tile order: keep the slot of each position (read-back)
                 // and map the CSR slots of the in-synapse lists below to tile positions

@@ -96,8 +96,10 @@ hipError_t plastic_isi(const int64_t* in_ptr, const int32_t* in_src, const int32
                        int32_t* out, hipStream_t st);
 hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, const TileTabs& tt,
                         uint64_t total, uint16_t* offs16, uint32_t* offs32, hipStream_t st);
-hipError_t pad_copy16(const uint16_t* src16, const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo,
-                      const uint32_t* en, const TileTabs& tt, uint64_t total, uint16_t* dst16, hipStream_t st);
+// Tile-order array `from` (unpadded offsets eo) into `to` (padded rows prow, offsets en).
+template <class T>
+hipError_t pad_copy(const T* from, const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo, const uint32_t* en,
+                    const TileTabs& tt, uint64_t total, T* to, hipStream_t st);
 hipError_t row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n, float top,
                       uint32_t* rmax_tmp, float* scale, hipStream_t st);
 hipError_t make_in_keys(const int32_t* tar, const uint8_t* plastic, int32_t n, uint32_t* keys,

@@ -240,11 +240,12 @@ __global__ void k_tile_offsets(const uint32_t* keys, const int64_t* row_ptr, Til
     }
 }
 
-// Copy of the tile-order CLASS16 words into the padded layout: offset entry e
-// (source m, local tile t < Gq) moves its sub-row from row_ptr[m] + eo[e] (the
-// unpadded offsets) to prow[m] + (en[e] & ~(kSubAlign - 1)).
-__global__ void k_pad_copy16(const uint16_t* src16, const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo,
-                             const uint32_t* en, TileTabs tt, uint64_t total, uint16_t* dst16) {
+// Copy of a tile-order array into the padded layout: offset entry e (source m,
+// local tile t < Gq) moves its sub-row from row_ptr[m] + eo[e] (the unpadded
+// offsets) to prow[m] + (en[e] & ~(kSubAlign - 1)).
+template <class T>
+__global__ void k_pad_copy(const T* from_arr, const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo,
+                           const uint32_t* en, TileTabs tt, uint64_t total, T* to_arr) {
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
         int32_t lo = 0, hi = tt.nseg;          // seg_obase[lo] <= e < seg_obase[hi]
@@ -260,8 +261,8 @@ __global__ void k_pad_copy16(const uint16_t* src16, const int64_t* row_ptr, cons
         if (r - rel * (gq + 1) == gq) continue;   // the row's end entry
         const int32_t m = tt.seg_lo[q] + (int32_t)rel;
         const uint32_t len = eo[e + 1] - eo[e];
-        const uint16_t* from = src16 + row_ptr[m] + eo[e];
-        uint16_t* to = dst16 + prow[m] + (en[e] & ~(uint32_t)(kSubAlign - 1));
+        const T* from = from_arr + row_ptr[m] + eo[e];
+        T* to = to_arr + prow[m] + (en[e] & ~(uint32_t)(kSubAlign - 1));
         for (uint32_t j = 0; j < len; ++j) to[j] = from[j];
     }
 }
@@ -504,13 +505,18 @@ hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, con
     return hipGetLastError();
 }
 
-hipError_t pad_copy16(const uint16_t* src16, const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo,
-                      const uint32_t* en, const TileTabs& tt, uint64_t total, uint16_t* dst16, hipStream_t st) {
+template <class T>
+hipError_t pad_copy(const T* from, const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo, const uint32_t* en,
+                    const TileTabs& tt, uint64_t total, T* to, hipStream_t st) {
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pad_copy16, dim3(blocks_for(total, 256)), dim3(256), 0, st, src16, row_ptr, prow, eo, en,
-                       tt, total, dst16);
+    hipLaunchKernelGGL(k_pad_copy<T>, dim3(blocks_for(total, 256)), dim3(256), 0, st, from, row_ptr, prow, eo, en,
+                       tt, total, to);
     return hipGetLastError();
 }
+template hipError_t pad_copy<uint16_t>(const uint16_t*, const int64_t*, const int64_t*, const uint32_t*,
+                                       const uint32_t*, const TileTabs&, uint64_t, uint16_t*, hipStream_t);
+template hipError_t pad_copy<uint32_t>(const uint32_t*, const int64_t*, const int64_t*, const uint32_t*,
+                                       const uint32_t*, const TileTabs&, uint64_t, uint32_t*, hipStream_t);
 
 hipError_t row_scales(const int32_t* src, const float* w, uint64_t S, int32_t n, float top,
                       uint32_t* rmax_tmp, float* scale, hipStream_t st) {

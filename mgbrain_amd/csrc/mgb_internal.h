@@ -192,7 +192,7 @@ struct TiledArgs {
     const uint16_t* syn16;      // kTClass16 words
     const uint16_t* offs16;
     const uint32_t* offs32;
-    const int64_t* prow;    // kTClass16: first slot of each padded row of syn16
+    const int64_t* prow;    // first slot of each padded row of syn16 / syn_t / w_t (all kinds but kTStdp)
     uint32_t* ring_t;
     Ctl* ctl;
     uint32_t* win_buf;
@@ -247,9 +247,9 @@ constexpr int kTiledItems = 16384;  // work items staged in LDS per propagation 
 constexpr int kTiledStaticLds = 1024; // bytes reserved for the fused kernel's static LDS
 constexpr int kClassCache = 32;     // k-fold class sums cached in LDS (tables hold >= this many)
 constexpr int kStdpHist = 16;       // spike steps kept per neuron for deferred dendrite updates
-// kTClass16: every (source, tile) sub-row of the tile-order copy starts at a
-// multiple of kSubAlign words (16 B), so that a work item of 2-16 words is
-// aligned vector loads
+// Every (source, tile) sub-row of the tile-order copy starts at a multiple of
+// kSubAlign words (16 B of CLASS16, 32 B of 32-bit words; not kTStdp), so that a
+// work item of 2-16 words is aligned vector loads
 constexpr int kSubAlign = 8;
 constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 each
 // Dynamic LDS of the fused kernel; kTStdp also caches lut_lds entries of each STDP table.

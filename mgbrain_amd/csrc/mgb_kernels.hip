@@ -661,7 +661,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         if (a.offs16) { o0 = a.offs16[ob]; o1 = a.offs16[ob + 1]; }
         else { o0 = a.offs32[ob]; o1 = a.offs32[ob + 1]; }
         const int32_t sD = (int32_t)((sprev + (int64_t)(ent >> 26)) % D);
-        if (WK == kTClass16) {
+        if (!kPl) {
             // padded sub-rows: starts are multiples of kSubAlign, the low bits of the
             // next entry hold the padding
             constexpr uint32_t am = (uint32_t)kSubAlign - 1u;
@@ -786,6 +786,38 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                         }
                         continue;
                     }
+                    if constexpr (!kPl && WK != kTClass16 && CH >= 2) {
+                        // 32-bit words (and FP32 weights): aligned vector loads likewise
+                        const uint32_t* p32 = a.syn_t + kb[h];
+                        if constexpr (CH == 2) {
+                            const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p32));
+                            x[h][0] = v.x;
+                            x[h][1] = v.y;
+                            if (WK == kTF32) {
+                                const u32x2 f = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(a.w_t + kb[h]));
+                                wv[h][0] = __uint_as_float(f.x);
+                                wv[h][1] = __uint_as_float(f.y);
+                            }
+                        } else {
+#pragma unroll
+                            for (int v4 = 0; v4 < CH / 4; ++v4) {
+                                const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p32) + v4);
+                                x[h][4 * v4 + 0] = v.x;
+                                x[h][4 * v4 + 1] = v.y;
+                                x[h][4 * v4 + 2] = v.z;
+                                x[h][4 * v4 + 3] = v.w;
+                                if (WK == kTF32) {
+                                    const u32x4 f = __builtin_nontemporal_load(
+                                        reinterpret_cast<const u32x4*>(a.w_t + kb[h]) + v4);
+                                    wv[h][4 * v4 + 0] = __uint_as_float(f.x);
+                                    wv[h][4 * v4 + 1] = __uint_as_float(f.y);
+                                    wv[h][4 * v4 + 2] = __uint_as_float(f.z);
+                                    wv[h][4 * v4 + 3] = __uint_as_float(f.w);
+                                }
+                            }
+                        }
+                        continue;
+                    }
                     const uint32_t last = max(cnt[h], 1u) - 1u;
 #pragma unroll
                     for (int u = 0; u < CH; ++u) {
@@ -844,22 +876,21 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                 for (int h = 0; h < U; ++h)
 #pragma unroll
                     for (int u = 0; u < CH; ++u) {
+                        // words past the item's end (padding, another tile's words) and
+                        // lanes past the round's items: a distinct in-slice entry per
+                        // lane, adding 0
                         const bool valid = (uint32_t)u < cnt[h];
                         const uint32_t xw = x[h][u];
-                        // past the round's items: a distinct entry per lane, adding 0
-                        const bool idle = cnt[h] == 0u;
                         if (WK == kTClass) {
                             int32_t slot = sD[h] + (int32_t)((xw >> 26) & 31u) + 1;
                             slot -= (slot >= D) ? D : 0;
                             const int32_t loc = (int32_t)(xw & kIdMask) - t0;
-                            const int32_t at = idle ? (tid % tn) : slot * tn + loc;
+                            const int32_t at = valid ? slot * tn + loc : (tid % tn);
                             atomicAdd(&ring[at], valid ? ((xw >> 31) ? 0x10000u : 1u) : 0u);
                         } else if (WK == kTClass16) {
                             int32_t slot = sD[h] + (int32_t)((xw >> 10) & 31u) + 1;
                             slot -= (slot >= D) ? D : 0;
                             const int32_t loc = (int32_t)(xw & 0x3FFu);
-                            // words past the item's end are not this tile's: a distinct
-                            // in-slice entry per lane, adding 0
                             const int32_t at = valid ? slot * tn + loc : (tid % tn);
                             atomicAdd(&ring[at], valid ? ((xw >> 15) ? 0x10000u : 1u) : 0u);
                         } else {
@@ -883,7 +914,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             }
                             const bool exc = w > 0.f;
                             const int32_t v = (int32_t)rintf(w * fsc.x);
-                            const int32_t at = idle ? (tid % tn) : ((exc ? 0 : D) + slot) * tn + loc;
+                            const int32_t at = valid ? ((exc ? 0 : D) + slot) * tn + loc : (tid % tn);
                             if (full) atomicAdd(reinterpret_cast<int32_t*>(ring) + at, valid ? v : 0);
                             if (pl && valid)                           // axon update of t2
                                 a.w_t[kb[h] + (uint32_t)u] = clipw(w + ax[h][u], a.w_max, a.w_min);
