@@ -113,10 +113,13 @@ struct mgb_sim {
     uint32_t* syn_t = nullptr;
     uint16_t* syn16 = nullptr;         // kTClass16 words in tile order (padded sub-rows)
     int64_t* prow = nullptr;           // first slot of each padded row of syn16 / syn_t / w_t (not kTStdp)
+    float* tile_cc = nullptr;          // CLASS kinds: class cache of each tile
+    uint32_t* tile_ccls = nullptr;
     float* w_t = nullptr;              // kTF32 / kTStdp weights in tile order
     uint16_t* offs16 = nullptr;
     uint32_t* offs32 = nullptr;
     uint32_t* ring_t = nullptr;
+    size_t ring_slab = 0;              // words per tile in ring_t (RW * D * B rounded up to 16 bytes)
     // host mirrors for read-back
     std::vector<float> class_w;        // [n_tbl][2]
     std::vector<uint16_t> h_tbl;       // [n]
@@ -832,6 +835,10 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
             }
             MGB_TRY(build_tile_offsets(s, n, kout, tt));
             MGB_TRY(pad_subrows(s, n));
+            MGB_TRY(dalloc_n(s, &s->tile_cc, (size_t)s->G * 4 * kClassCache));
+            MGB_TRY(dalloc_n(s, &s->tile_ccls, (size_t)s->G * 2));
+            MGB_HIP_TRY(launch_tile_ccache(s->tile_lo, s->G, s->tbl, s->tbl_off, s->seq, s->tile_cc, s->tile_ccls,
+                                           st));
         }
         MGB_HIP_TRY(hipStreamSynchronize(st));
     } else {
@@ -1041,7 +1048,8 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     // ring
     if (s->tiled) {
         const size_t rw = (s->tkind == kTClass || s->tkind == kTClass16) ? 1u : 2u;
-        const size_t rt = (size_t)s->G * (size_t)s->D * (size_t)s->B * rw;
+        s->ring_slab = ((size_t)s->D * (size_t)s->B * rw + 3u) & ~(size_t)3u;
+        const size_t rt = (size_t)s->G * s->ring_slab;
         MGB_TRY(dalloc_n(s, &s->ring_t, rt));
         MGB_HIP_TRY(hipMemsetAsync(s->ring_t, 0, 4 * rt, st));
     } else if (s->mode == kRingClass) {
@@ -1155,7 +1163,8 @@ TiledArgs tiled_args(const mgb_sim* s) {
     a.tbl_off = s->tbl_off; a.seq = s->seq; a.tbl_w = s->tbl_w;
     a.row_ptr = s->row_ptr; a.syn_t = s->syn_t; a.syn16 = s->syn16; a.offs16 = s->offs16; a.offs32 = s->offs32;
     a.prow = s->prow;
-    a.ring_t = s->ring_t; a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
+    a.tile_cc = s->tile_cc; a.tile_ccls = s->tile_ccls;
+    a.ring_t = s->ring_t; a.ring_slab = s->ring_slab; a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
     a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
     if (s->tkind == kTStdp) {
         a.last3 = s->last; a.last_stride = (size_t)s->npad;
@@ -1900,8 +1909,7 @@ std::vector<CkptSeg> ckpt_segments(const mgb_sim* s, uint64_t n_rec) {
                               {s->ctl, sizeof(Ctl)},
                               {s->win_buf, 4 * (size_t)(s->tiled ? 3 : 2) * (size_t)s->win_stride}};
     if (s->tiled) {
-        const size_t rw = (s->tkind == kTClass || s->tkind == kTClass16) ? 1 : 2;
-        v.push_back({s->ring_t, 4 * (size_t)s->G * (size_t)s->D * (size_t)s->B * rw});
+        v.push_back({s->ring_t, 4 * (size_t)s->G * s->ring_slab});
         v.push_back({s->tcnt, 12 * (size_t)s->nseg});
     } else if (s->mode == kRingClass) {
         v.push_back({s->ring, 4 * (size_t)s->D * np});

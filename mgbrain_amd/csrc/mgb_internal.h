@@ -193,7 +193,10 @@ struct TiledArgs {
     const uint16_t* offs16;
     const uint32_t* offs32;
     const int64_t* prow;    // first slot of each padded row of syn16 / syn_t / w_t (all kinds but kTStdp)
+    const float* tile_cc;   // CLASS kinds: [G][2][2][kClassCache] class cache of each tile (k_tile_ccache)
+    const uint32_t* tile_ccls;   // [G][2] its classes
     uint32_t* ring_t;
+    size_t ring_slab;       // words per tile in ring_t: RW * D * B rounded up to 16 bytes
     Ctl* ctl;
     uint32_t* win_buf;
     uint32_t win_stride;
@@ -254,6 +257,9 @@ constexpr int kSubAlign = 8;
 constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 each
 // Dynamic LDS of the fused kernel; kTStdp also caches lut_lds entries of each STDP table.
 size_t tiled_lds_bytes(int D, int B, int kind, int lut_lds = 0);
+// Class cache of every tile of the CLASS kinds (cc [G][2][2][kClassCache], cls [G][2]).
+hipError_t launch_tile_ccache(const int32_t* tile_lo, int32_t G, const uint16_t* tbl, const uint32_t* tbl_off,
+                              const float* seq, float* cc, uint32_t* cls, hipStream_t st);
 // Workgroups of `block` threads of the fused kernel for `a` that one CU holds at once.
 int tiled_blocks_per_cu(const TiledArgs& a, int block);
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,

@@ -77,15 +77,21 @@ __device__ __forceinline__ void neuron_load(const A& a, int i, int type, bool cl
     st.rnd = U32x4{{0u, 0u, 0u, 0u}};
     st.last = last[i];
     st.cnt = a.cnt[i];
+    // the per-neuron words that index tables, and V / refr, are loaded before the
+    // branch on the type (one round trip with the type's), the table entries after it
+    const uint32_t cs = a.cset[i];
+    const uint32_t tb = class_tables ? (uint32_t)a.tbl[i] : 0u;
+    const uint32_t sidx = a.stream ? a.stream[i] : 0u;
+    const float v0 = a.V[i];
+    const int32_t r0 = a.refr[i];
     if (type == T_POISSON) {
         st.rate = a.rate[i];
-        const uint32_t sidx = a.stream ? a.stream[i] : 0u;
         const uint64_t seed = a.stream_seed[sidx];
         st.key0 = (uint32_t)seed;
         st.key1 = (uint32_t)(seed >> 32);
         st.lid = (uint32_t)(i - a.stream_base[sidx]);
     } else if (integrates(type)) {
-        const CSet& C = a.csets[a.cset[i]];
+        const CSet& C = a.csets[cs];
         st.c3 = C.c[3];
         st.c4 = C.c[4];
         st.refr_steps = __float_as_int(C.c[30]);
@@ -93,14 +99,14 @@ __device__ __forceinline__ void neuron_load(const A& a, int i, int type, bool cl
             st.c12 = C.c[12]; st.c13 = C.c[13]; st.c14 = C.c[14];
             st.c15 = C.c[15]; st.c16 = C.c[16]; st.c17 = C.c[17];
         }
-        st.V = a.V[i];
-        st.refr = a.refr[i];
+        st.V = v0;
+        st.refr = r0;
         if (type != T_LIFB) {
             st.Ie = a.Ie[i];
             st.Ii = a.Ii[i];
         }
         if (class_tables) {
-            const uint32_t t = a.tbl[i];
+            const uint32_t t = tb;
             st.cls = t;
             st.off_e = a.tbl_off[2 * t];
             st.off_i = a.tbl_off[2 * t + 1];
@@ -546,7 +552,7 @@ __device__ __forceinline__ void ring_slice_out(const uint32_t* ring, uint32_t* g
 // the reference holds after the last step run.
 template <int CH, int WK>
 __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_t wk, int32_t nsteps) {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];
     __shared__ uint32_t wsum[16];
     __shared__ unsigned long long se_blk, lsp_r;
     __shared__ uint32_t lsp_n, lsp_g;
@@ -636,10 +642,10 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         }
     }
     if (kCls && tid < 4 * kClassCache) {
-        const int32_t c = tid / (2 * kClassCache), side = (tid / kClassCache) & 1, k = tid % kClassCache;
-        const uint32_t cl = a.tbl[c == 0 ? t0 + tn - 1 : t0];
-        ccache[c][side][k] = a.seq[a.tbl_off[2 * cl + side] + k];
-        if (side == 0 && k == 0) ccls[c] = cl;
+        // (precomputed per tile by k_tile_ccache: one load instead of a chain of three)
+        ccache[tid / (2 * kClassCache)][(tid / kClassCache) & 1][tid % kClassCache] =
+            a.tile_cc[(size_t)g * 4 * kClassCache + tid];
+        if (tid < 2) ccls[tid] = a.tile_ccls[2 * g + tid];
     }
     // 1. ring slice -> LDS.  The slots the previous window consumed ([sprev, s0)
     // mod D) were cleared by it and not stored back (step 4): they are zero here.
@@ -648,7 +654,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // and windows of up to kMaxWindow steps)
     const int32_t nprev = (int32_t)min(s0 - sprev, (int64_t)D);
     const int32_t rprev = (int32_t)(sprev % D);
-    uint32_t* gring = a.ring_t + (size_t)g * (size_t)RW * (size_t)D * (size_t)B;
+    uint32_t* gring = a.ring_t + (size_t)g * a.ring_slab;
     const uint32_t E = tcnt[bprev * nseg];
     const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride + lbase;
     // metadata of list entry e: first slot of its sub-row in this tile, length |
@@ -1250,6 +1256,26 @@ hipError_t set_lds_kind(size_t bytes) {
     return e;
 }
 }  // namespace
+
+// The fused kernel's class cache of every tile (CLASS kinds): cc[g][c][side][k] =
+// the k-fold sum of side (excitatory, inhibitory) of class cls[g][c], where c = 0
+// is the class of the tile's last neuron and c = 1 that of its first.
+__global__ void k_tile_ccache(const int32_t* tile_lo, const uint16_t* tbl, const uint32_t* tbl_off,
+                              const float* seq, float* cc, uint32_t* cls) {
+    const int32_t g = (int32_t)blockIdx.x, tid = (int32_t)threadIdx.x;
+    const int32_t c = tid / (2 * kClassCache), side = (tid / kClassCache) & 1, k = tid % kClassCache;
+    const int32_t t0 = tile_lo[g], t1 = tile_lo[g + 1];
+    const uint32_t cl = tbl[c == 0 ? t1 - 1 : t0];
+    cc[(size_t)g * 4 * kClassCache + tid] = seq[tbl_off[2 * cl + side] + k];
+    if (side == 0 && k == 0) cls[2 * g + c] = cl;
+}
+
+hipError_t launch_tile_ccache(const int32_t* tile_lo, int32_t G, const uint16_t* tbl, const uint32_t* tbl_off,
+                              const float* seq, float* cc, uint32_t* cls, hipStream_t st) {
+    if (G <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_ccache, dim3(G), dim3(4 * kClassCache), 0, st, tile_lo, tbl, tbl_off, seq, cc, cls);
+    return hipGetLastError();
+}
 
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st) {
