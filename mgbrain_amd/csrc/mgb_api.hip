@@ -103,7 +103,8 @@ struct mgb_sim {
     uint64_t n_offs = 0, n_keys = 0;
     uint32_t* tcnt = nullptr;          // [3][nseg]
     int tkind = kTClass;               // synapse word kind of the fused kernel
-    uint32_t* tperm = nullptr;         // kTStdp: CSR slot of each tile-order position
+    uint32_t* tperm = nullptr;         // kTStdp: CSR slot of each tile-order position (~0u: padding)
+    uint64_t n_tslots = 0;             // length of the (padded) tile-order arrays
     float2* seg_scale = nullptr;       // fixed-point formats, several segments: (scale, 1 / scale)
     uint16_t* seg_of = nullptr;        // segment of each neuron (with seg_scale)
     int32_t *hist = nullptr, *tf = nullptr, *cntf = nullptr;   // kTStdp: deferred dendrite updates
@@ -518,7 +519,7 @@ mgb_status repad(mgb_sim* s, T** arr, uint64_t total, uint64_t s_old, const uint
 // hold the padding after sub-row t - 1, so that sub-row t is (e[t + 1] & ~7) -
 // (e[t] & ~7) - (e[t + 1] & 7) words long.  Padding words are zero;
 // 2 * kSubAlign words of slack follow the last row.
-mgb_status pad_subrows(mgb_sim* s, int32_t n) {
+mgb_status pad_subrows(mgb_sim* s, int32_t n, uint32_t* pmap = nullptr) {
     hipStream_t st = s->stream;
     const size_t no = (size_t)s->n_offs;
     std::vector<uint32_t> eo(no), en(no);
@@ -570,6 +571,8 @@ mgb_status pad_subrows(mgb_sim* s, int32_t n) {
     MGB_HIP_TRY(hipMemcpyAsync(s->prow, prow.data(), 8 * ((size_t)n + 1), hipMemcpyHostToDevice, st));
     MGB_HIP_TRY(hipMemcpyAsync(deo, eo.data(), 4 * no, hipMemcpyHostToDevice, st));
     MGB_HIP_TRY(hipMemcpyAsync(den, en.data(), 4 * no, hipMemcpyHostToDevice, st));
+    s->n_tslots = total;
+    if (pmap) MGB_HIP_TRY(pad_map(s->row_ptr, s->prow, deo, den, tile_tabs(s, nullptr), no, pmap, st));
     if (s->syn16) MGB_TRY(repad(s, &s->syn16, total, s_old, deo, den));
     if (s->syn_t) MGB_TRY(repad(s, &s->syn_t, total, s_old, deo, den));
     if (s->w_t) MGB_TRY(repad(s, reinterpret_cast<uint32_t**>(&s->w_t), total, s_old, deo, den));
@@ -927,15 +930,18 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
             MGB_HIP_TRY(tile_gather_wide(vout, S, s->col, s->dly, s->w16, s->w8, s->w32, tt, s->tkind, s->syn_t,
                                          s->w_t, st));
             MGB_TRY(build_tile_offsets(s, n, kout, tt));
-            // (plastic weights keep their unpadded tile order: the in-synapse lists and
-            // the read-back permutation index it)
-            if (s->tkind != kTStdp) MGB_TRY(pad_subrows(s, n));
+            uint32_t* pmap = nullptr;          // kTStdp: padded position of each tile-order position
+            if (s->tkind == kTStdp) MGB_TRY(X.get(&pmap, std::max<size_t>(S, 1)));
+            MGB_TRY(pad_subrows(s, n, pmap));
             if (s->tkind == kTStdp) {
-                // plastic weights live in tile order: keep the slot of each position (read-back)
-                // and map the CSR slots of the in-synapse lists below to tile positions
-                MGB_TRY(dalloc_n(s, &s->tperm, S));
-                if (S) MGB_HIP_TRY(hipMemcpyAsync(s->tperm, vout, S * 4, hipMemcpyDeviceToDevice, st));
+                // plastic weights live in (padded) tile order: keep the CSR slot of each
+                // position (read-back; ~0u at padding) and map the CSR slots of the
+                // in-synapse lists below to padded tile positions
+                MGB_TRY(dalloc_n(s, &s->tperm, (size_t)s->n_tslots));
+                MGB_HIP_TRY(hipMemsetAsync(s->tperm, 0xFF, 4 * (size_t)s->n_tslots, st));
+                MGB_HIP_TRY(scatter_u32(pmap, vout, S, s->tperm, st));
                 MGB_HIP_TRY(invert_slots(vout, S, vin, st));
+                MGB_HIP_TRY(remap_u32(vin, pmap, S, st));
                 MGB_HIP_TRY(remap_u32(pos_of, vin, S, st));
             }
         }
@@ -1917,7 +1923,10 @@ std::vector<CkptSeg> ckpt_segments(const mgb_sim* s, uint64_t n_rec) {
         v.push_back({s->ring_e, 4 * (size_t)s->D * np});
         v.push_back({s->ring_i, 4 * (size_t)s->D * np});
     }
-    if (s->stdp) v.push_back({s->tkind == kTStdp && s->tiled ? s->w_t : s->w32, 4 * (size_t)s->S});
+    if (s->stdp) {
+        if (s->tkind == kTStdp && s->tiled) v.push_back({s->w_t, 4 * (size_t)s->n_tslots});
+        else v.push_back({s->w32, 4 * (size_t)s->S});
+    }
     if (s->hist) {
         v.push_back({s->hist, 4 * np * kStdpHist});
         v.push_back({s->tf, 4 * np});
@@ -2031,7 +2040,7 @@ namespace {
 // Writes to w32 (CSR order) the weights the reference holds after the last
 // step run: a copy of w_t with the pending window's updates applied.
 mgb_status current_weights(mgb_sim* s) {
-    const size_t S = (size_t)s->S;
+    const size_t S = (size_t)s->n_tslots;
     float* tmp = nullptr;
     hipError_t e = hipMalloc(&tmp, std::max<size_t>(S, 1) * 4);
     if (e != hipSuccess) {

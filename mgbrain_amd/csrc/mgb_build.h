@@ -96,6 +96,11 @@ hipError_t plastic_isi(const int64_t* in_ptr, const int32_t* in_src, const int32
                        int32_t* out, hipStream_t st);
 hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, const TileTabs& tt,
                         uint64_t total, uint16_t* offs16, uint32_t* offs32, hipStream_t st);
+// pmap[k]: the padded position of unpadded tile-order position k.
+hipError_t pad_map(const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo, const uint32_t* en,
+                   const TileTabs& tt, uint64_t total, uint32_t* pmap, hipStream_t st);
+// out[idx[k]] = val[k], k < S.
+hipError_t scatter_u32(const uint32_t* idx, const uint32_t* val, uint64_t S, uint32_t* out, hipStream_t st);
 // Tile-order array `from` (unpadded offsets eo) into `to` (padded rows prow, offsets en).
 template <class T>
 hipError_t pad_copy(const T* from, const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo, const uint32_t* en,

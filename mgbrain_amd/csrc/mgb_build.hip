@@ -188,11 +188,21 @@ __global__ void k_remap(uint32_t* a, const uint32_t* map, uint64_t np) {
         a[p] = map[a[p]];
 }
 
-// w32[slot[k]] = w_t[k]: the CSR-order copy of tile-order weights.
+// w32[slot[k]] = w_t[k]: the CSR-order copy of tile-order weights (slot
+// 0xFFFFFFFF: a padding position).
 __global__ void k_scatter_w(const uint32_t* slot, const float* w_t, uint64_t S, float* w32) {
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = slot[k];
+        if (p != 0xFFFFFFFFu) w32[p] = w_t[k];
+    }
+}
+
+// out[idx[k]] = val[k].
+__global__ void k_scatter_u32(const uint32_t* idx, const uint32_t* val, uint64_t S, uint32_t* out) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
          k += (uint64_t)gridDim.x * blockDim.x)
-        w32[slot[k]] = w_t[k];
+        out[idx[k]] = val[k];
 }
 
 // Smallest isi[] over the ends of plastic synapses (in-CSR in_ptr / in_src):
@@ -502,6 +512,45 @@ hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, con
     else
         hipLaunchKernelGGL(k_tile_offsets<uint32_t>, dim3(blocks_for(total, 256)), dim3(256), 0, st,
                            sorted_keys, row_ptr, tt, total, offs32);
+    return hipGetLastError();
+}
+
+// pmap[row_ptr[m] + eo[e] + j] = prow[m] + (en[e] & ~(kSubAlign - 1)) + j: the
+// padded position of every unpadded tile-order position.
+__global__ void k_pad_map(const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo, const uint32_t* en,
+                          TileTabs tt, uint64_t total, uint32_t* pmap) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        int32_t lo = 0, hi = tt.nseg;
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (tt.seg_obase[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        const int32_t q = lo;
+        const uint64_t r = e - tt.seg_obase[q];
+        const uint64_t gq = (uint64_t)(tt.seg_tile0[q + 1] - tt.seg_tile0[q]);
+        const uint64_t rel = r / (gq + 1);
+        if (r - rel * (gq + 1) == gq) continue;
+        const int32_t m = tt.seg_lo[q] + (int32_t)rel;
+        const uint32_t len = eo[e + 1] - eo[e];
+        const uint64_t from = (uint64_t)row_ptr[m] + eo[e];
+        const uint64_t to = (uint64_t)prow[m] + (en[e] & ~(uint32_t)(kSubAlign - 1));
+        for (uint32_t j = 0; j < len; ++j) pmap[from + j] = (uint32_t)(to + j);
+    }
+}
+
+hipError_t pad_map(const int64_t* row_ptr, const int64_t* prow, const uint32_t* eo, const uint32_t* en,
+                   const TileTabs& tt, uint64_t total, uint32_t* pmap, hipStream_t st) {
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pad_map, dim3(blocks_for(total, 256)), dim3(256), 0, st, row_ptr, prow, eo, en, tt, total,
+                       pmap);
+    return hipGetLastError();
+}
+
+hipError_t scatter_u32(const uint32_t* idx, const uint32_t* val, uint64_t S, uint32_t* out, hipStream_t st) {
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_u32, dim3(blocks_for(S, 256)), dim3(256), 0, st, idx, val, S, out);
     return hipGetLastError();
 }
 

@@ -667,16 +667,11 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         if (a.offs16) { o0 = a.offs16[ob]; o1 = a.offs16[ob + 1]; }
         else { o0 = a.offs32[ob]; o1 = a.offs32[ob + 1]; }
         const int32_t sD = (int32_t)((sprev + (int64_t)(ent >> 26)) % D);
-        if (!kPl) {
-            // padded sub-rows: starts are multiples of kSubAlign, the low bits of the
-            // next entry hold the padding
-            constexpr uint32_t am = (uint32_t)kSubAlign - 1u;
-            beg = (uint32_t)a.prow[src] + (o0 & ~am);
-            lm = ((o1 & ~am) - (o0 & ~am) - (o1 & am)) | ((uint32_t)sD << 26);
-        } else {
-            beg = (uint32_t)a.row_ptr[src] + o0;
-            lm = (o1 - o0) | ((uint32_t)sD << 26);
-        }
+        // padded sub-rows: starts are multiples of kSubAlign, the low bits of the next
+        // entry hold the padding
+        constexpr uint32_t am = (uint32_t)kSubAlign - 1u;
+        beg = (uint32_t)a.prow[src] + (o0 & ~am);
+        lm = ((o1 & ~am) - (o0 & ~am) - (o1 & am)) | ((uint32_t)sD << 26);
         if (WK == kTFP8 || WK == kTBF16) aux = __float_as_uint(a.row_scale[src]);
         else if (kPl) aux = (uint32_t)lastA[src];
         else aux = 0u;
@@ -792,15 +787,18 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                         }
                         continue;
                     }
-                    if constexpr (!kPl && WK != kTClass16 && CH >= 2) {
-                        // 32-bit words (and FP32 weights): aligned vector loads likewise
+                    if constexpr (WK != kTClass16 && CH >= 2) {
+                        // 32-bit words and FP32 weights: aligned vector loads likewise
+                        // (kTStdp rewrites its weights below: cached loads)
                         const uint32_t* p32 = a.syn_t + kb[h];
+                        const u32x2* f2 = reinterpret_cast<const u32x2*>(a.w_t + kb[h]);
+                        const u32x4* f4 = reinterpret_cast<const u32x4*>(a.w_t + kb[h]);
                         if constexpr (CH == 2) {
                             const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p32));
                             x[h][0] = v.x;
                             x[h][1] = v.y;
-                            if (WK == kTF32) {
-                                const u32x2 f = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(a.w_t + kb[h]));
+                            if (WK == kTF32 || kPl) {
+                                const u32x2 f = kPl ? *f2 : __builtin_nontemporal_load(f2);
                                 wv[h][0] = __uint_as_float(f.x);
                                 wv[h][1] = __uint_as_float(f.y);
                             }
@@ -812,9 +810,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                                 x[h][4 * v4 + 1] = v.y;
                                 x[h][4 * v4 + 2] = v.z;
                                 x[h][4 * v4 + 3] = v.w;
-                                if (WK == kTF32) {
-                                    const u32x4 f = __builtin_nontemporal_load(
-                                        reinterpret_cast<const u32x4*>(a.w_t + kb[h]) + v4);
+                                if (WK == kTF32 || kPl) {
+                                    const u32x4 f = kPl ? f4[v4] : __builtin_nontemporal_load(f4 + v4);
                                     wv[h][4 * v4 + 0] = __uint_as_float(f.x);
                                     wv[h][4 * v4 + 1] = __uint_as_float(f.y);
                                     wv[h][4 * v4 + 2] = __uint_as_float(f.z);
@@ -850,7 +847,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             const int32_t loc = (int32_t)(xw & 0x7FFu);
                             int32_t lj = 0, mlo = kStdpHist, mhi = 0;
                             rng[h][u] = -1;
-                            if ((xw >> 16) & 1u) {
+                            // (words past the item's end are padding or another tile's)
+                            if ((uint32_t)u < cnt[h] && ((xw >> 16) & 1u)) {
                                 const int32_t* hj = hL + loc * kStdpHist;
                                 const int32_t c = cL[loc];
                                 const int32_t lo = max(la[h] == 0 ? -1 : la[h], tfL[loc]);
