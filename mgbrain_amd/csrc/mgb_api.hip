@@ -113,7 +113,7 @@ struct mgb_sim {
     int32_t stdp_isi = 1;              // kTStdp: shortest inter-spike interval of plastic synapses' ends
     uint32_t* syn_t = nullptr;
     uint16_t* syn16 = nullptr;         // kTClass16 words in tile order (padded sub-rows)
-    int64_t* prow = nullptr;           // first slot of each padded row of syn16 / syn_t / w_t (not kTStdp)
+    int64_t* prow = nullptr;           // first slot of each padded row of syn16 / syn_t / w_t
     float* tile_cc = nullptr;          // CLASS kinds: class cache of each tile
     uint32_t* tile_ccls = nullptr;
     float* w_t = nullptr;              // kTF32 / kTStdp weights in tile order

@@ -192,7 +192,7 @@ struct TiledArgs {
     const uint16_t* syn16;      // kTClass16 words
     const uint16_t* offs16;
     const uint32_t* offs32;
-    const int64_t* prow;    // first slot of each padded row of syn16 / syn_t / w_t (all kinds but kTStdp)
+    const int64_t* prow;    // first slot of each padded row of syn16 / syn_t / w_t
     const float* tile_cc;   // CLASS kinds: [G][2][2][kClassCache] class cache of each tile (k_tile_ccache)
     const uint32_t* tile_ccls;   // [G][2] its classes
     uint32_t* ring_t;
@@ -251,7 +251,7 @@ constexpr int kTiledStaticLds = 1024; // bytes reserved for the fused kernel's s
 constexpr int kClassCache = 32;     // k-fold class sums cached in LDS (tables hold >= this many)
 constexpr int kStdpHist = 16;       // spike steps kept per neuron for deferred dendrite updates
 // Every (source, tile) sub-row of the tile-order copy starts at a multiple of
-// kSubAlign words (16 B of CLASS16, 32 B of 32-bit words; not kTStdp), so that a
+// kSubAlign words (16 B of CLASS16, 32 B of 32-bit words), so that a
 // work item of 2-16 words is aligned vector loads
 constexpr int kSubAlign = 8;
 constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 each
