@@ -27,9 +27,9 @@ from tests import netmodels as M
 pytestmark = pytest.mark.gpu
 
 
-def _run(net, syn, sc, nsteps, seed, wf, prop=N.PROP_DIRECT, window=1):
+def _run(net, syn, sc, nsteps, seed, wf, prop=N.PROP_DIRECT, window=1, **kw):
     g = H.gpu_from_list(net, syn, seed=seed, stdp=True, stdpconst=sc, weight_format=wf, record_capacity=4_000_000,
-                        propagation=prop)
+                        propagation=prop, **kw)
     info = g.info()
     assert info["window"] == window
     assert info["propagation"] == (N.PROP_DIRECT if prop == N.PROP_DIRECT else N.PROP_TILED)
@@ -84,6 +84,20 @@ def test_ee_stdp_benchmark_constants(wf, prop):
         syn.weight[stat] = (r * sc_row).astype(np.float32)[stat]
     window = 1 if prop == N.PROP_DIRECT else 10     # min_delay 10, refractory interval 20
     changed, n = _run(net, syn, sc, 800, 6, wf, prop=prop, window=window)
+    assert changed > 100 and n > 1000
+
+
+@pytest.mark.parametrize("chunk", [1, 2, 16])
+def test_fused_stdp_chunks(chunk):
+    """Fused path with work items of 1, 2 and 16 synapses (the default is 8): items
+    are aligned vector loads of the padded tile order; a 16-synapse item reads past
+    its sub-row's padding into the next sub-row, whose words must be ignored."""
+    print(f"case: fixed in-degree net, E->E STDP, fused, chunk={chunk}", flush=True)
+    sys.stdout.flush()
+    net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
+    syn = H.host_synapses(net, seed=6)
+    sc = M.stdp_consts_config4(J=1e-4)
+    changed, n = _run(net, syn, sc, 800, 6, N.WF_F32, prop=N.PROP_TILED, window=10, chunk=chunk)
     assert changed > 100 and n > 1000
 
 
