@@ -346,6 +346,8 @@ def coba_leg(args, log):
     table = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
     tc = json.load(open(table)).get("vogels_abbott80k_p0.02") if os.path.exists(table) else None
     shc = dict(tile_size=tc["tile_size"], prop_block=tc["block"], chunk=tc["chunk"]) if tc and args.scale == 1.0 else {}
+    if args.tile or args.block or args.chunk:          # explicit shapes win over the table
+        shc = dict(tile_size=args.tile, prop_block=args.block, chunk=args.chunk)
     out, ref = {}, None
     for name, wf in (("f32_class", N.WF_CLASS), ("f32_fixed", N.WF_F32), ("bf16", N.WF_BF16), ("fp8", N.WF_FP8)):
         t0 = time.perf_counter()
@@ -464,6 +466,8 @@ def main(argv=None):
     table = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
     tuned_table = json.load(open(table)) if os.path.exists(table) else {}
     batch_shapes = None if args.tune else tuned_table.get(f"brunel_sweep{args.nets}")
+    if args.config == 5 and (args.tile or args.block or args.chunk):   # explicit shapes win over the table
+        batch_shapes = {"tile_size": args.tile, "block": args.block, "chunk": args.chunk}
     if args.config == 5:
         if args.tune:
             batch_shapes = tune_batched(args, log)

@@ -575,7 +575,11 @@ mgb_status pad_subrows(mgb_sim* s, int32_t n, uint32_t* pmap = nullptr) {
     if (pmap) MGB_HIP_TRY(pad_map(s->row_ptr, s->prow, deo, den, tile_tabs(s, nullptr), no, pmap, st));
     if (s->syn16) MGB_TRY(repad(s, &s->syn16, total, s_old, deo, den));
     if (s->syn_t) MGB_TRY(repad(s, &s->syn_t, total, s_old, deo, den));
-    if (s->w_t) MGB_TRY(repad(s, reinterpret_cast<uint32_t**>(&s->w_t), total, s_old, deo, den));
+    if (s->w_t) {
+        uint32_t* w = reinterpret_cast<uint32_t*>(s->w_t);    // moved as 32-bit words
+        MGB_TRY(repad(s, &w, total, s_old, deo, den));
+        s->w_t = reinterpret_cast<float*>(w);
+    }
     dfree(s, deo);
     s->bytes -= (int64_t)(4 * no);
     if (emax <= 0xFFFFu) {
