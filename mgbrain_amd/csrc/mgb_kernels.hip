@@ -603,9 +603,6 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     Ctl* ctl = a.ctl;
     // fixed-point (scale, 1 / scale) of the tile's segment (scale_e == scale_i)
     const float2 fsc = a.seg_scale ? a.seg_scale[a.tile_seg[g]] : make_float2(a.scale_e, a.inv_e);
-    const int64_t s0 = ctl->base_step + off;
-    const int64_t kwin = ctl->base_window + wk;
-    const int bcur = (int)(kwin % 3), bprev = (int)((kwin + 2) % 3), bnext = (int)((kwin + 1) % 3);
     const int32_t t0 = a.tile_lo[g];
     const int32_t tn = a.tile_lo[g + 1] - t0;
     // the tile's segment (one segment: all of it known without loads)
@@ -619,6 +616,18 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const size_t lbase = (size_t)slo * (size_t)a.window;    // segment's part of each spike list
     uint32_t* tcnt = a.tcnt + q;                            // tcnt[b * nseg]
     const int32_t nseg = a.nseg;
+    // the window's index comes from ctl; the start step, spike count and first
+    // entries of all three spike lists load with it (one round trip instead of two,
+    // the index then picks one; the other lists' words, which other workgroups may
+    // be writing, are not used)
+    const uint32_t li = min((uint32_t)tid, (uint32_t)nq * (uint32_t)a.window - 1u);
+    const int64_t tb0 = ctl->tbase[0], tb1 = ctl->tbase[1], tb2 = ctl->tbase[2];
+    const uint32_t n0 = tcnt[0], n1 = tcnt[nseg], n2 = tcnt[2 * nseg];
+    const size_t ws = a.win_stride;
+    const uint32_t l0 = a.win_buf[lbase + li], l1 = a.win_buf[ws + lbase + li], l2 = a.win_buf[2 * ws + lbase + li];
+    const int64_t s0 = ctl->base_step + off;
+    const int64_t kwin = ctl->base_window + wk;
+    const int bcur = (int)(kwin % 3), bprev = (int)((kwin + 2) % 3), bnext = (int)((kwin + 1) % 3);
     if (full && lt == 0 && tid == 0) tcnt[bnext * nseg] = 0u;
     if (kPl && full && g == 0 && tid == 0) a.fcnt[(kwin + 1) & 1] = 0u;
     if (full && g == 0 && tid == 0) ctl->tbase[bcur] = s0;
@@ -649,13 +658,13 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     }
     // 1. ring slice -> LDS.  The slots the previous window consumed ([sprev, s0)
     // mod D) were cleared by it and not stored back (step 4): they are zero here.
-    const int64_t sprev = ctl->tbase[bprev];
+    const int64_t sprev = bprev == 0 ? tb0 : (bprev == 1 ? tb1 : tb2);
     // 0 before the first window; at most D (a network without synapses has D = 1
     // and windows of up to kMaxWindow steps)
     const int32_t nprev = (int32_t)min(s0 - sprev, (int64_t)D);
     const int32_t rprev = (int32_t)(sprev % D);
     uint32_t* gring = a.ring_t + (size_t)g * a.ring_slab;
-    const uint32_t E = tcnt[bprev * nseg];
+    const uint32_t E = bprev == 0 ? n0 : (bprev == 1 ? n1 : n2);
     const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride + lbase;
     // metadata of list entry e: first slot of its sub-row in this tile, length |
     // ring slot of its spike step << 26, BF16 / FP8 row scale bits or (kTStdp)
@@ -681,7 +690,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // entries), and their metadata loads before the ring copy, so that these
     // latencies overlap; entries past the count are never used
     uint32_t pbeg = 0u, plm = 0u, paux = 0u;
-    const uint32_t ent0 = L[min((uint32_t)tid, (uint32_t)nq * (uint32_t)a.window - 1u)];
+    const uint32_t ent0 = bprev == 0 ? l0 : (bprev == 1 ? l1 : l2);
     if ((uint32_t)tid < min((uint32_t)bd, E)) meta(ent0, pbeg, plm, paux);
     if (full) ring_slice_in<RW>(ring, gring, D, tn, rprev, nprev);
     // a tile of at most one neuron per thread: the neuron's state and out-degree are
