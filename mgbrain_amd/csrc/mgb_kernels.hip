@@ -1078,6 +1078,10 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             lsp_r = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)nl);
         }
     }
+    // 4. LDS -> ring slice, except the slots this window consumed (cleared above,
+    // or never read for neurons that do not integrate); meanwhile the list's
+    // offset comes back from the atomic above
+    ring_slice_out<RW>(ring, gring, D, tn, (int32_t)(s0 % D), nsteps);
     __syncthreads();
     for (uint32_t k = (uint32_t)tid; k < nl; k += (uint32_t)bd) {
         const uint32_t ent = lsp[k];
@@ -1089,9 +1093,6 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             else ctl->rec_overflow = 1;
         }
     }
-    // 4. LDS -> ring slice, except the slots this window consumed (cleared above,
-    // or never read for neurons that do not integrate)
-    ring_slice_out<RW>(ring, gring, D, tn, (int32_t)(s0 % D), nsteps);
 }
 
 // Flush of deferred dendrite updates (kTStdp): for each neuron j the fused
