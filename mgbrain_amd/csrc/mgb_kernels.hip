@@ -1106,9 +1106,18 @@ __global__ __launch_bounds__(256) void k_stdp_flush(TiledArgs a, int32_t off, in
     __shared__ int64_t epb[256], epe[256];
     __shared__ uint32_t eui[256], wsum[16];                // inclusive prefix of units
     const Ctl* ctl = a.ctl;
+    const uint32_t tid0 = threadIdx.x;
+    // both lists' counts and first entries load with the window index (one round
+    // trip instead of three); the index's parity then picks one
+    const uint32_t nf0 = a.fcnt[0], nf1 = a.fcnt[1];
+    const uint32_t i0 = min(tid0, a.flist_cap - 1u);
+    const int32_t* fl1 = a.flist + (size_t)a.flist_cap * 3;
+    const int32_t j0 = a.flist[3 * i0], c0 = a.flist[3 * i0 + 1], tf0 = a.flist[3 * i0 + 2];
+    const int32_t j1 = fl1[3 * i0], c1 = fl1[3 * i0 + 1], tf1 = fl1[3 * i0 + 2];
     const int64_t kwin = ctl->base_window + wk;
-    const uint32_t nf = a.fcnt[kwin & 1];
-    const int32_t* fli = a.flist + (size_t)(kwin & 1) * a.flist_cap * 3;
+    const bool par = (kwin & 1) != 0;
+    const uint32_t nf = par ? nf1 : nf0;
+    const int32_t* fli = par ? fl1 : a.flist;
     const int32_t hz = (int32_t)(ctl->base_step + off - 1);
     const int32_t* lastB = a.last3 + (size_t)(kwin % 3) * a.last_stride;   // end of window k-1
     const uint32_t bd = blockDim.x, G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
@@ -1117,10 +1126,11 @@ __global__ __launch_bounds__(256) void k_stdp_flush(TiledArgs a, int32_t off, in
         const uint32_t nc = min(bd, nf - f0);
         uint32_t nu = 0;
         if (tid < nc) {
-            const int32_t j = fli[3 * (f0 + tid)];
+            const bool first = f0 == 0;
+            const int32_t j = first ? (par ? j1 : j0) : fli[3 * (f0 + tid)];
             ej[tid] = j;
-            ec[tid] = fli[3 * (f0 + tid) + 1];
-            etf[tid] = fli[3 * (f0 + tid) + 2];
+            ec[tid] = first ? (par ? c1 : c0) : fli[3 * (f0 + tid) + 1];
+            etf[tid] = first ? (par ? tf1 : tf0) : fli[3 * (f0 + tid) + 2];
             const int64_t pb = a.din_ptr[j], pe = a.din_ptr[j + 1];
             epb[tid] = pb;
             epe[tid] = pe;
@@ -1143,6 +1153,7 @@ __global__ __launch_bounds__(256) void k_stdp_flush(TiledArgs a, int32_t off, in
             if (p >= epe[f]) continue;
             const int32_t j = ej[f], c = ec[f];
             const uint32_t kk = a.din_pos[p];
+            float w = a.w_t[kk];                            // (with lk: one round trip)
             const int32_t lk = lastB[a.din_src[p]];
             const int32_t lo = max(lk == 0 ? -1 : lk, etf[f]);
             const int32_t* hj = a.hist + (size_t)j * kStdpHist;
@@ -1160,7 +1171,6 @@ __global__ __launch_bounds__(256) void k_stdp_flush(TiledArgs a, int32_t off, in
 #pragma unroll
             for (int m = 0; m < kStdpHist; ++m)
                 dv[m] = (m < nr && hv[m] <= hz) ? stdp_dw(a.ltp, a.ltd, a.lut_n, lk - hv[m]) : 0.0f;
-            float w = a.w_t[kk];
 #pragma unroll
             for (int m = kStdpHist - 1; m >= 0; --m)
                 if (m < nr && hv[m] <= hz) w = clipw(w + dv[m], a.w_max, a.w_min);
