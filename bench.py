@@ -396,7 +396,7 @@ def tune_batched(args, log):
     cands = sorted(set(T.tile_candidates(max(W.net_size(x) for x in nets), pinfo["ring_len"])[:2]
                        + T.tile_candidates(pinfo["n_neurons"], pinfo["ring_len"])[:3]))
     best, res = T.autotune(lambda b: N.Simulator.batch_generated(nets, tile_size=b), cands,
-                           blocks=(512, 1024), chunks=(2, 4), log=lambda m: log("# " + m))
+                           blocks=(512, 1024), chunks=(4, 8, 16), log=lambda m: log("# " + m))
     return dict(best, candidates=len(res)) if best else None
 
 

@@ -27,10 +27,12 @@ def tile_candidates(n, ring_len, lds_bytes=160 * 1024, batch_words=3 * 1024 + 40
     return out
 
 
-def autotune(make_sim, tiles, blocks=(256, 512, 1024), chunks=(1, 2, 4, 8), steps=400, warmup=200,
+def autotune(make_sim, tiles, blocks=(256, 512, 1024), chunks=(1, 2, 4, 8, 16), steps=400, warmup=200,
              budget_s=90.0, log=None):
     """make_sim(tile_size) -> Simulator.  Returns (best, results) where each
-    result is a dict(tile_size, block, chunk, ms_per_step)."""
+    result is a dict(tile_size, block, chunk, ms_per_step, widest): tile_size is
+    the mean tile width asked for (what to pass back to build the same tiling),
+    widest the simulator's widest tile."""
     t_start = time.perf_counter()
     results = []
     for tile in tiles:
@@ -45,10 +47,12 @@ def autotune(make_sim, tiles, blocks=(256, 512, 1024), chunks=(1, 2, 4, 8), step
                     sim.run(steps // 4)
                     sim.run(steps)
                     ms = sim.info()["last_run_ms"]
-                    r = dict(tile_size=sim.info()["tile_size"], block=block, chunk=chunk, ms_per_step=ms / steps)
+                    r = dict(tile_size=tile, block=block, chunk=chunk, ms_per_step=ms / steps,
+                             widest=sim.info()["tile_size"])
                     results.append(r)
                     if log:
-                        log(f"tune tile={r['tile_size']} block={block} chunk={chunk}: {r['ms_per_step'] * 1e3:.3f} us/step")
+                        log(f"tune tile={tile} (widest {r['widest']}) block={block} chunk={chunk}: "
+                            f"{r['ms_per_step'] * 1e3:.3f} us/step")
         finally:
             sim.close()
     best = min(results, key=lambda r: r["ms_per_step"]) if results else None
