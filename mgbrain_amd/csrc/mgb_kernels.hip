@@ -144,6 +144,18 @@ __device__ __forceinline__ bool poisson_step(NState& st, int32_t s, int32_t w) {
 }
 
 // One step of an integrating neuron given this step's ring input (be, bi).
+// Inputs of a step from the fixed-point ring sums (scale 1 / inv on both sides):
+// LIFB integrates be + bi only, so it gets their integer sum converted once.
+__device__ __forceinline__ void fixed_input(int type, int32_t qe, int32_t qi, float inv, float& be, float& bi) {
+    if (type == T_LIFB) {
+        be = (float)(qe + qi) * inv;
+        bi = 0.0f;
+    } else {
+        be = (float)qe * inv;
+        bi = (float)qi * inv;
+    }
+}
+
 __device__ __forceinline__ bool integrate_step(int type, NState& st, int32_t s, float be, float bi) {
     bool fired = false;
     if (type == T_LIFB) {                                        // mgcore.cu:164-196
@@ -240,8 +252,7 @@ __global__ __launch_bounds__(kNeuBlock) void k_neurons(NeuArgs a, int32_t off, i
                 if (qe != 0) a.ring_e[loc] = 0;
                 if (qi != 0) a.ring_i[loc] = 0;
                 const float inv = a.seg_scale ? a.seg_scale[a.seg_of[i]].y : a.inv_e;   // inv_e == inv_i
-                be = (float)qe * inv;
-                bi = (float)qi * inv;
+                fixed_input(type, qe, qi, inv, be, bi);
             }
             fired = integrate_step(type, st, s, be, bi);
         }
@@ -1024,8 +1035,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                     const int32_t qe = (int32_t)*r, qi = (int32_t)*ri;
                     *r = 0u;
                     *ri = 0u;
-                    be = (float)qe * fsc.y;
-                    bi = (float)qi * fsc.y;
+                    fixed_input(type, qe, qi, fsc.y, be, bi);
                 }
                 fired = integrate_step(type, st, s, be, bi);
             }

@@ -504,8 +504,15 @@ static inline void neuron_f32(orc_sim* S, int32_t i, int32_t s) {
         size_t loc = (size_t)i * (size_t)D + (size_t)slot;      /* mgcore.cu:96 */
         float be, bi;
         if (S->acc == ORC_ACC_FIXED) {
-            be = (float)S->qexc[loc] * S->inv_exc;
-            bi = (float)S->qinh[loc] * S->inv_inh;
+            if (S->type[i] == T_LIFB) {
+                /* LIFB integrates be + bi only: one fixed-point sum, converted once
+                 * (the two scales are equal in this mode) */
+                be = (float)(S->qexc[loc] + S->qinh[loc]) * S->inv_exc;
+                bi = 0.0f;
+            } else {
+                be = (float)S->qexc[loc] * S->inv_exc;
+                bi = (float)S->qinh[loc] * S->inv_inh;
+            }
         } else {
             be = S->bexc[loc];
             bi = S->binh[loc];

@@ -83,6 +83,33 @@ def test_fan_in_sequential_sum_of_equal_addends(k):
     assert fx.state(po.F_V)[k] == float(f32(f32(k * q) * f32(1.0 / scale)))
 
 
+def test_fixed_point_lifb_input_is_one_sum():
+    """Fixed-point mode: a LIFB neuron integrates be + bi only, so its input is the
+    integer sum of both sides converted once, (qe + qi) * 2^-k, not qe * 2^-k +
+    qi * 2^-k.  Three excitatory and two inhibitory arrivals whose sums exceed
+    2^24 tell the two apart (at scale 2^40)."""
+    we = [f32(0.0003559108), f32(0.00057523185), f32(0.0001720798)]
+    wi = [f32(-0.00057432475), f32(-0.00025591571)]
+    syn = [(i, 5, w, 5) for i, w in enumerate(we + wi)]
+    n = 6
+    L = po.SynapseList(np.array([s[0] for s in syn], np.int32), np.array([s[1] for s in syn], np.int32),
+                       np.array([s[2] for s in syn], np.float32),
+                       np.array([f32(s[3] * 1e-4) for s in syn], np.float32), np.ones(len(syn), np.uint8))
+    csr = po.build_csr(n, L, step=1e-4)
+    v0 = np.array([0.5] * 5 + [0.0], np.float32)
+    sim = po.OracleSim(csr, [po.LIFB] * n, np.zeros(n, np.float32), v0, M.life_const2(), acc=po.ACC_FIXED,
+                       scale_exc=2.0 ** 40, scale_inh=2.0 ** 40)
+    sim.run(6)
+    sc, inv = f32(2.0 ** 40), f32(2.0 ** -40)
+    qe = sum(int(np.rint(f32(w * sc))) for w in we)
+    qi = sum(int(np.rint(f32(w * sc))) for w in wi)
+    one_sum = f32(f32(qe + qi) * inv)
+    two_sums = f32(f32(f32(qe) * inv) + f32(f32(qi) * inv))
+    assert one_sum != two_sums
+    assert sim.state(po.F_V)[5] == float(one_sum)
+    sim.close()
+
+
 def test_inhibitory_and_zero_weights_go_to_the_inhibitory_buffer():
     # w <= 0 is added to I_buffer_inh (mgcore.cu:292-295); LIF keeps Ie / Ii separately
     lc = M.gen_life_const(0, 0, 0, 0, 20e-3, 0.002, 5e-3, 10e-3, 1.0, 0)
