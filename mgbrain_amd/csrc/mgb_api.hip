@@ -90,6 +90,7 @@ struct mgb_sim {
     // fused tiled path
     int tiled = 0;
     int32_t B = 0, G = 0, chunk = 8, tblock = 512;
+    int32_t trw = 1;                   // fused path: ring words per (slot, target)
     int32_t nseg = 1;
     std::vector<int32_t> h_seg_lo;     // [nseg+1] segment boundaries (independent networks)
     std::vector<int32_t> h_seg_tile0;  // [nseg+1] first tile of each segment
@@ -345,8 +346,10 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D, con
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
     lds_max = std::min(lds_max, 160 * 1024);
     // the kernel's static LDS (block scan sums, event counter) shares the limit
-    const int64_t per_b = (int64_t)(tiled_lds_bytes(D, 1, s->tkind) - tiled_lds_bytes(D, 0, s->tkind));
-    int32_t bmax = (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0, s->tkind) - kTiledStaticLds) / per_b);
+    const int64_t per_b =
+        (int64_t)(tiled_lds_bytes(D, 1, s->tkind, s->trw) - tiled_lds_bytes(D, 0, s->tkind, s->trw));
+    int32_t bmax =
+        (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0, s->tkind, s->trw) - kTiledStaticLds) / per_b);
     // wide words hold an 11-bit tile-local target; CLASS tiles of at most 1024
     // targets take 16-bit words (else 32-bit words with the global id)
     bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, s->tkind == kTClass ? 1024 : 2048));
@@ -460,7 +463,7 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D, con
     MGB_HIP_TRY(hipMemcpyAsync(s->seg_kbase, kbase.data(), 4 * kbase.size(), hipMemcpyHostToDevice, st));
     MGB_HIP_TRY(hipMemcpyAsync(s->seg_obase, obase.data(), 8 * obase.size(), hipMemcpyHostToDevice, st));
     MGB_HIP_TRY(hipStreamSynchronize(st));   // host vectors go out of scope
-    const size_t lds = tiled_lds_bytes(D, B, s->tkind);
+    const size_t lds = tiled_lds_bytes(D, B, s->tkind, s->trw);
     if (lds > 65536) MGB_HIP_TRY(set_tiled_lds_limit(s->tkind, lds));
     return MGB_OK;
 }
@@ -914,6 +917,12 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         if (s->tiled) {
             s->tkind = h.stdp ? kTStdp
                               : (s->wide_kind == kWideBF16 ? kTBF16 : (s->wide_kind == kWideFP8 ? kTFP8 : kTF32));
+            // one fixed-point ring when every integrating neuron is LIFB (it integrates
+            // the sum of both sides); STDP keeps two
+            bool lifb_only = true;
+            for (int32_t i = 0; i < n && lifb_only; ++i)
+                lifb_only = h.type[i] != MGB_LIF && h.type[i] != MGB_LIFE;
+            s->trw = (s->tkind != kTStdp && lifb_only) ? 1 : 2;
             MGB_TRY(choose_tiles(s, o, n, s->D, h.type, h_indeg));
             const TileTabs tt = tile_tabs(s, seg_dev);
             // kout holds each CSR slot's source (the keys of the sort by source)
@@ -1057,8 +1066,7 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
                                hipMemcpyHostToDevice, st));
     // ring
     if (s->tiled) {
-        const size_t rw = (s->tkind == kTClass || s->tkind == kTClass16) ? 1u : 2u;
-        s->ring_slab = ((size_t)s->D * (size_t)s->B * rw + 3u) & ~(size_t)3u;
+        s->ring_slab = ((size_t)s->D * (size_t)s->B * (size_t)s->trw + 3u) & ~(size_t)3u;
         const size_t rt = (size_t)s->G * s->ring_slab;
         MGB_TRY(dalloc_n(s, &s->ring_t, rt));
         MGB_HIP_TRY(hipMemsetAsync(s->ring_t, 0, 4 * rt, st));
@@ -1102,9 +1110,10 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
             int lds_max = 65536;
             (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
             lds_max = std::min(lds_max, 160 * 1024);
-            const int64_t room = (int64_t)lds_max - kTiledStaticLds - (int64_t)tiled_lds_bytes(s->D, s->B, s->tkind);
+            const int64_t room =
+                (int64_t)lds_max - kTiledStaticLds - (int64_t)tiled_lds_bytes(s->D, s->B, s->tkind, s->trw);
             s->lut_lds = (int32_t)std::max<int64_t>(0, std::min<int64_t>(s->lut_n, room / 8));
-            const size_t lds = tiled_lds_bytes(s->D, s->B, s->tkind, s->lut_lds);
+            const size_t lds = tiled_lds_bytes(s->D, s->B, s->tkind, s->trw, s->lut_lds);
             if (lds > 65536) MGB_HIP_TRY(set_tiled_lds_limit(s->tkind, lds));
         }
     }
@@ -1174,7 +1183,8 @@ TiledArgs tiled_args(const mgb_sim* s) {
     a.row_ptr = s->row_ptr; a.syn_t = s->syn_t; a.syn16 = s->syn16; a.offs16 = s->offs16; a.offs32 = s->offs32;
     a.prow = s->prow;
     a.tile_cc = s->tile_cc; a.tile_ccls = s->tile_ccls;
-    a.ring_t = s->ring_t; a.ring_slab = s->ring_slab; a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
+    a.ring_t = s->ring_t; a.ring_slab = s->ring_slab; a.rw = s->trw;
+    a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
     a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
     if (s->tkind == kTStdp) {
         a.last3 = s->last; a.last_stride = (size_t)s->npad;

@@ -230,6 +230,9 @@ struct TiledArgs {
     // (a copy) and lists every neuron with deferred dendrite updates for the
     // flush kernel, touching no ring, neuron state, spike list or tf / cntf
     int32_t apply_only;
+    // ring words per (slot, target): 1 for CLASS counts and for the wide kinds when
+    // every integrating neuron is LIFB (one fixed-point sum), else 2 (exc, inh)
+    int32_t rw;
     // 1: workgroup b runs tile (b % kXcds) * (G / kXcds) + ... (a contiguous run of
     // tiles per XCD), set when every workgroup of the launch is resident at once
     int32_t xcd_map;
@@ -255,8 +258,9 @@ constexpr int kStdpHist = 16;       // spike steps kept per neuron for deferred 
 // work item of 2-16 words is aligned vector loads
 constexpr int kSubAlign = 8;
 constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 each
-// Dynamic LDS of the fused kernel; kTStdp also caches lut_lds entries of each STDP table.
-size_t tiled_lds_bytes(int D, int B, int kind, int lut_lds = 0);
+// Dynamic LDS of the fused kernel (rw ring words per slot and target); kTStdp also
+// caches lut_lds entries of each STDP table.
+size_t tiled_lds_bytes(int D, int B, int kind, int rw, int lut_lds = 0);
 // Class cache of every tile of the CLASS kinds (cc [G][2][2][kClassCache], cls [G][2]).
 hipError_t launch_tile_ccache(const int32_t* tile_lo, int32_t G, const uint16_t* tbl, const uint32_t* tbl_off,
                               const float* seq, float* cc, uint32_t* cls, hipStream_t st);

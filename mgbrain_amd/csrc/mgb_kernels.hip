@@ -465,13 +465,12 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* wsum) 
     return v;
 }
 
-// One tile's ring slice (RW halves of D x B words) from HBM into LDS.  Rows
+// One tile's ring slice (rw halves of D x B words) from HBM into LDS.  Rows
 // [r0, r0 + nz) mod D were consumed by the previous window and not stored
 // back: they are cleared instead of loaded.  8 loads per thread in flight
 // (clamped indices keep every load in bounds; the extra values are not stored).
-template <int RW>
 __device__ __forceinline__ void ring_slice_in(uint32_t* ring, const uint32_t* gring, int32_t D, int32_t B,
-                                              int32_t r0, int32_t nz) {
+                                              int32_t r0, int32_t nz, int RW) {
     const int32_t tid = (int32_t)threadIdx.x, bd = (int32_t)blockDim.x;
     auto copy_in = [&](uint32_t* rr, const uint32_t* gr, int32_t k0, int32_t k1) {
         for (int32_t kb = k0 + tid; kb < k1; kb += 8 * bd) {
@@ -503,9 +502,8 @@ __device__ __forceinline__ void ring_slice_in(uint32_t* ring, const uint32_t* gr
 
 // LDS -> the tile's ring slice, except the min(nsteps, D) rows from rcur that
 // this window consumed (cleared, or never read for neurons that do not integrate).
-template <int RW>
 __device__ __forceinline__ void ring_slice_out(const uint32_t* ring, uint32_t* gring, int32_t D, int32_t B,
-                                               int32_t rcur, int32_t nsteps) {
+                                               int32_t rcur, int32_t nsteps, int RW) {
     const int32_t tid = (int32_t)threadIdx.x, bd = (int32_t)blockDim.x;
     const int32_t e = rcur + min(nsteps, D);
 #pragma unroll
@@ -588,7 +586,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     constexpr bool kCls = WK == kTClass || WK == kTClass16;  // CLASS arrival counts
     constexpr bool kPl = WK == kTStdp;                      // STDP on w_t
     constexpr bool kAux = WK == kTFP8 || WK == kTBF16 || kPl;
-    constexpr int RW = kCls ? 1 : 2;                        // ring words per (slot, target)
+    const int RW = kCls ? 1 : a.rw;                         // ring words per (slot, target)
     uint32_t* ring = lds;                                   // [RW][D][B]
     uint32_t* sp_beg = ring + (size_t)RW * D * B;           // [kTiledBatch] first CSR slot
     uint32_t* sp_len = sp_beg + kTiledBatch;                // len | slot-of-spike << 26
@@ -703,7 +701,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     uint32_t pbeg = 0u, plm = 0u, paux = 0u;
     const uint32_t ent0 = bprev == 0 ? l0 : (bprev == 1 ? l1 : l2);
     if ((uint32_t)tid < min((uint32_t)bd, E)) meta(ent0, pbeg, plm, paux);
-    if (full) ring_slice_in<RW>(ring, gring, D, tn, rprev, nprev);
+    if (full) ring_slice_in(ring, gring, D, tn, rprev, nprev, RW);
     // a tile of at most one neuron per thread: the neuron's state and out-degree are
     // loaded now (after the ring copy, which they would delay), their latency
     // hidden by the propagation phase
@@ -938,7 +936,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             }
                             const bool exc = w > 0.f;
                             const int32_t v = (int32_t)rintf(w * fsc.x);
-                            const int32_t at = valid ? ((exc ? 0 : D) + slot) * tn + loc : (tid % tn);
+                            // (one ring: both signs into one fixed-point sum)
+                            const int32_t at = valid ? ((exc || RW == 1 ? 0 : D) + slot) * tn + loc : (tid % tn);
                             if (full) atomicAdd(reinterpret_cast<int32_t*>(ring) + at, valid ? v : 0);
                             if (pl && valid)                           // axon update of t2
                                 a.w_t[kb[h] + (uint32_t)u] = clipw(w + ax[h][u], a.w_max, a.w_min);
@@ -1032,9 +1031,9 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                     else class_input(x, a.seq, st, be, bi);
                 } else {
                     uint32_t* ri = r + (size_t)D * tn;
-                    const int32_t qe = (int32_t)*r, qi = (int32_t)*ri;
+                    const int32_t qe = (int32_t)*r, qi = RW == 2 ? (int32_t)*ri : 0;
                     *r = 0u;
-                    *ri = 0u;
+                    if (RW == 2) *ri = 0u;
                     fixed_input(type, qe, qi, fsc.y, be, bi);
                 }
                 fired = integrate_step(type, st, s, be, bi);
@@ -1091,7 +1090,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // 4. LDS -> ring slice, except the slots this window consumed (cleared above,
     // or never read for neurons that do not integrate); meanwhile the list's
     // offset comes back from the atomic above
-    ring_slice_out<RW>(ring, gring, D, tn, (int32_t)(s0 % D), nsteps);
+    ring_slice_out(ring, gring, D, tn, (int32_t)(s0 % D), nsteps, RW);
     __syncthreads();
     for (uint32_t k = (uint32_t)tid; k < nl; k += (uint32_t)bd) {
         const uint32_t ent = lsp[k];
@@ -1237,10 +1236,9 @@ hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t
     return hipGetLastError();
 }
 
-size_t tiled_lds_bytes(int D, int B, int kind, int lut_lds) {
-    const size_t rw = (kind == kTClass || kind == kTClass16) ? 1 : 2;
+size_t tiled_lds_bytes(int D, int B, int kind, int rw, int lut_lds) {
     const bool pl = kind == kTStdp;
-    return sizeof(uint32_t) * (rw * (size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
+    return sizeof(uint32_t) * ((size_t)rw * (size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
            sizeof(uint16_t) * (size_t)kTiledItems +
            ((kind == kTFP8 || kind == kTBF16 || pl) ? sizeof(uint32_t) * kTiledBatch : 0) +
            (pl ? (kStdpHist + 2) * sizeof(int32_t) * (size_t)B + 2 * sizeof(float) * (size_t)lut_lds : 0);
@@ -1307,7 +1305,7 @@ hipError_t launch_tile_ccache(const int32_t* tile_lo, int32_t G, const uint16_t*
 
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st) {
-    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.kind == kTStdp ? a.lut_lds : 0);
+    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, a.kind == kTStdp ? a.lut_lds : 0);
     switch (a.kind) {
         case kTBF16: launch_tiled_kind<kTBF16>(a, off, wk, nsteps, block, lds, st); break;
         case kTFP8: launch_tiled_kind<kTFP8>(a, off, wk, nsteps, block, lds, st); break;
@@ -1338,7 +1336,7 @@ int blocks_per_cu_kind(int chunk, int block, size_t lds) {
 }
 
 int tiled_blocks_per_cu(const TiledArgs& a, int block) {
-    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.kind == kTStdp ? a.lut_lds : 0);
+    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, a.kind == kTStdp ? a.lut_lds : 0);
     switch (a.kind) {
         case kTBF16: return blocks_per_cu_kind<kTBF16>(a.chunk, block, lds);
         case kTFP8: return blocks_per_cu_kind<kTFP8>(a.chunk, block, lds);
