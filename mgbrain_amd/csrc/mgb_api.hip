@@ -918,11 +918,11 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
             s->tkind = h.stdp ? kTStdp
                               : (s->wide_kind == kWideBF16 ? kTBF16 : (s->wide_kind == kWideFP8 ? kTFP8 : kTF32));
             // one fixed-point ring when every integrating neuron is LIFB (it integrates
-            // the sum of both sides); STDP keeps two
+            // the sum of both sides)
             bool lifb_only = true;
             for (int32_t i = 0; i < n && lifb_only; ++i)
                 lifb_only = h.type[i] != MGB_LIF && h.type[i] != MGB_LIFE;
-            s->trw = (s->tkind != kTStdp && lifb_only) ? 1 : 2;
+            s->trw = lifb_only ? 1 : 2;
             MGB_TRY(choose_tiles(s, o, n, s->D, h.type, h_indeg));
             const TileTabs tt = tile_tabs(s, seg_dev);
             // kout holds each CSR slot's source (the keys of the sort by source)
