@@ -106,7 +106,7 @@ void orc_free_csr(orc_csr* c);
 /* ---- simulation (simulator/mgsim/adj/mgcore.cu:116-412, 535-554) -------- */
 enum {
     ORC_ACC_F32 = 0,     /* FP32 state, sequential float ring adds in canonical order */
-    ORC_ACC_FIXED = 1,   /* FP32 state, int32 fixed-point ring (LSB 1/scale) */
+    ORC_ACC_FIXED = 1,   /* FP32 state, int32 fixed-point ring (LSB 1/scale; LIFB reads exc + inh as one sum) */
     ORC_ACC_F64 = 2      /* FP64 state and accumulation (accuracy oracle) */
 };
 
