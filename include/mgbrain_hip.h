@@ -46,7 +46,8 @@ enum {
                           share one weight value, else F32 */
     MGB_WF_CLASS = 1,  /* 4 B/synapse: target|delay|sign; per-class weight, exact integer
                           counts reproduce the reference's FP32 sum of equal addends */
-    MGB_WF_F32 = 2,    /* per-synapse FP32 weight, int32 fixed-point accumulation */
+    MGB_WF_F32 = 2,    /* per-synapse FP32 weight, int32 fixed-point accumulation (a LIFB
+                          neuron reads its excitatory + inhibitory sums as one) */
     MGB_WF_BF16 = 3,   /* per-synapse BF16 of w / s, s = the source row's largest |w|
                           (plastic synapses stay FP32) */
     MGB_WF_FP8 = 4     /* per-synapse FP8-E4M3 of w / s, s = the source row's largest |w|
