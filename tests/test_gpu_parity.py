@@ -138,6 +138,18 @@ def test_lif_exponential_currents(prop, wf):
     assert n > 3000
 
 
+@pytest.mark.parametrize("prop", [1, 2])
+def test_mixed_lif_lifb_fixed_point(prop):
+    """LIF and LIFB neurons in one fixed-point network: two rings, the LIF neurons
+    read both sums, the LIFB neurons their sum converted once."""
+    _say(f"case: LIF + LIFB network, F32 fixed point, prop={prop}")
+    net = M.lif_net()
+    net.pops[2] = (400, M.LIFB, 0)
+    syn = H.host_synapses(net, seed=4)
+    n = _compare_run(net, syn, 1500, seed=4, gpu_kw=dict(propagation=prop, weight_format=2), acc=po.ACC_FIXED)
+    assert n > 3000
+
+
 def test_graph_replay_identical():
     _say("case: config1 x0.2, hipGraph replay vs oracle")
     net = M.config1(scale=0.2, poisson_rate=0.2)
