@@ -1,0 +1,33 @@
+"""The launch-shape autotuner (mgbrain_amd/tune.py) on a stand-in simulator (no GPU):
+the best shape names the tile size that was asked for, which rebuilds the same
+tiling, not the simulator's widest tile."""
+from mgbrain_amd import tune as T
+
+
+class FakeSim:
+    """Widest tile 1024 whatever is asked; time per step set by (tile, block, chunk)."""
+
+    def __init__(self, tile):
+        self.tile, self.block, self.chunk = tile, 0, 0
+        self.steps = 0
+
+    def run(self, n):
+        self.steps = n
+
+    def set_launch(self, block, chunk):
+        self.block, self.chunk = block, chunk
+
+    def info(self):
+        cost = abs(self.tile - 430) + abs(self.block - 1024) / 64 + abs(self.chunk - 8)
+        return {"tile_size": 1024, "last_run_ms": self.steps * (1.0 + cost) * 1e-3}
+
+    def close(self):
+        pass
+
+
+def test_autotune_reports_the_requested_tile_size():
+    best, res = T.autotune(FakeSim, [200, 430, 900], blocks=(512, 1024), chunks=(4, 8, 16), steps=40, warmup=4)
+    assert len(res) == 3 * 2 * 3
+    assert (best["tile_size"], best["block"], best["chunk"]) == (430, 1024, 8)
+    assert best["widest"] == 1024
+    assert sorted({r["tile_size"] for r in res}) == [200, 430, 900]
