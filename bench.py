@@ -400,6 +400,20 @@ def tune_batched(args, log):
     return dict(best, candidates=len(res)) if best else None
 
 
+def copy_probe(nbytes=1 << 30):
+    """A device-to-device copy of exactly nbytes read + nbytes written per launch (torch's
+    copy kernel), run under rocprofv3 --pmc beside the workload to scale its counters."""
+    import torch  # plumbing only: device buffers and a copy kernel
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {nbytes} bytes x 3", flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -427,7 +441,12 @@ def main(argv=None):
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--propagation", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device three times: the known-size reference that "
+                         "converts rocprofv3 FETCH_SIZE / WRITE_SIZE counters into HBM bytes")
     args = ap.parse_args(argv)
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
                                    "not an implemented path", "n_gpus": args.gpus}))

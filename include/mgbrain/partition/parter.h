@@ -1,11 +1,16 @@
 // parter.h -- partitioning entry point (reference src/partition/parter.h:35,
-// parter.cpp:100-123).  One GPU: every neuron is in partition 0, which is what
-// the reference returns for nparts <= 1 (parter.cpp:103-104).  Graph
-// partitioning (METIS, MODEL, BSIM, SIMPLE) for several GPUs is not part of
-// this build; nparts > 1 throws std::invalid_argument.
+// parter.cpp:100-123).  nparts <= 1: every neuron is in partition 0, as the
+// reference returns (parter.cpp:103-104).  nparts > 1 is accepted only with
+// Config::SINGLE_GPU, the mode in which the reference places every partition on
+// device 0 (mgsim.cpp:341-352) and MGSimulator here simulates them as one
+// network: neurons are then split into nparts contiguous id ranges (the
+// partition does not change a single-device run).  Graph partitioning (METIS,
+// MODEL, BSIM) for several GPUs is not part of this build; nparts > 1 without
+// SINGLE_GPU throws std::invalid_argument, as MGSimulator's constructor does.
 #pragma once
 #include <stdexcept>
 
+#include "../macro.h"
 #include "../model/network.h"
 
 namespace MGBrain {
@@ -14,9 +19,14 @@ class Partitioner {
    public:
     static std::vector<int> part_network(Network& network, int nparts, PartType type) {
         (void)type;
-        if (nparts > 1)
-            throw std::invalid_argument("Partitioner: only one partition (one GPU) is supported");
-        return std::vector<int>(network.neurons.size(), 0);
+        const size_t n = network.neurons.size();
+        if (nparts <= 1) return std::vector<int>(n, 0);
+        if (!Config::SINGLE_GPU)
+            throw std::invalid_argument(
+                "Partitioner: nparts > 1 needs Config::enable_single_gpu() (one GPU is supported)");
+        std::vector<int> part(n);
+        for (size_t i = 0; i < n; ++i) part[i] = (int)((i * (size_t)nparts) / n);
+        return part;
     }
 };
 

@@ -20,7 +20,7 @@ INC = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["mgb_kernels.hip", "mgb_build.hip", "mgb_api.hip"]
 HOST_SOURCES = ["host/model.cpp", "host/netgen.cpp", "host/mgsim.cpp"]
-PROGRAMS = {"mgb_single_brunel": "host/single_brunel.cpp", "mgb_run_model": "host/run_model.cpp"}
+PROGRAMS = {"mgb_run_model": "host/run_model.cpp"}
 
 ARCH = os.environ.get("MGB_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

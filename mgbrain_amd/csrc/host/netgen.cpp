@@ -156,7 +156,9 @@ void generate(Network& net, Model& model, std::mt19937& rng, uint64_t philox_see
                     for (size_t n = 0; n < (size_t)tn; n++)
                         if (count < gen_num && ((rng() % max_num) < lim)) genlist[count++] = m * tn + n;
                 net.synapses.reserve(net.synapses.size() + gen_num);
-                for (size_t t = 0; t < count; t++) {
+                // all gen_num entries (:210): unfilled ones are zero, i.e. the projection's
+                // first source to its first target, when the 1.1p acceptance ran out early
+                for (size_t t = 0; t < genlist.size(); t++) {
                     const int m = (int)(genlist[t] / tn);
                     const int n = (int)(genlist[t] % tn);
                     const real w = NetGen::rand_weight(minw, maxw);

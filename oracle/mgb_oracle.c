@@ -228,13 +228,18 @@ int orc_gen_net(const orc_pop* pops, int32_t n_pops, const orc_proj* pros, int32
                 uint64_t gen_num = (uint64_t)roundf((float)max_num * pr->ctype);
                 double lim = (double)gen_num * 1.1;
                 uint64_t count = 0;
-                uint64_t* genlist = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(gen_num ? gen_num : 1));
+                /* genlist is zero-initialised (std::vector<size_t>(gen_num), :172) and the
+                 * emission loop walks all gen_num entries (:210), so when the 1.1p acceptance
+                 * runs out of pairs before gen_num, the gen_num - count unfilled entries emit
+                 * synapses from the projection's first source to its first target, each
+                 * drawing its weight and delay in turn. */
+                uint64_t* genlist = (uint64_t*)calloc((size_t)(gen_num ? gen_num : 1), sizeof(uint64_t));
                 if (!genlist) { rc = -1; break; }
                 for (uint64_t m = 0; m < (uint64_t)sn; ++m)
                     for (uint64_t n = 0; n < (uint64_t)tn; ++n)
                         if (count < gen_num && (double)((uint64_t)orc_mt_next(&mt) % max_num) < lim)
                             genlist[count++] = m * (uint64_t)tn + n;
-                for (uint64_t t = 0; t < count && rc == 0; ++t) {
+                for (uint64_t t = 0; t < gen_num && rc == 0; ++t) {
                     int32_t m = (int32_t)(genlist[t] / (uint64_t)tn);
                     int32_t n = (int32_t)(genlist[t] % (uint64_t)tn);
                     float w = ref_rand_weight(&gr, pr->wmin, pr->wmax, dw);

@@ -32,6 +32,16 @@ SPECS = {
                                    "proj 0 1 0.001 0.001 0.001 0.001 0.5", "proj 0 1 0.002 0.002 0.001 0.001 0.5",
                                    "proj 1 1 0.003 0.003 0.001 0.001 0.2", "proj 1 1 0.004 0.004 0.002 0.002 0.2"],
     "timestep_1ms": ["timestep 0.001", "clock 31", "srand 1", "brunel 5000"],
+    # small projections where the 1.1p acceptance runs out before gen_num: the reference still
+    # emits gen_num synapses (the unfilled genlist entries are first source -> first target)
+    "quota_tail_single": ["timestep 0.0001", "dw 0.01", "clock 12", "srand 1", "pop 10 2 1", "pop 10 5 0",
+                          "proj 0 1 0.001 0.001 0.001 0.001 0.2"],
+    "quota_tail_chain_clock27": ["timestep 0.0001", "dw 0.002", "clock 27", "srand 5", "pop 10 2 1", "pop 12 5 0",
+                                 "pop 7 5 0", "proj 0 1 0.001 0.009 0.001 0.002 0.2",
+                                 "proj 1 2 -0.02 -0.01 0.0005 0.0015 0.15", "proj 2 1 0.003 0.011 0.001 0.003 0.3"],
+    "quota_tail_chain_clock33": ["timestep 0.0001", "dw 0.002", "clock 33", "srand 5", "pop 10 2 1", "pop 12 5 0",
+                                 "pop 7 5 0", "proj 0 1 0.001 0.009 0.001 0.002 0.2",
+                                 "proj 1 2 -0.02 -0.01 0.0005 0.0015 0.15", "proj 2 1 0.003 0.011 0.001 0.003 0.3"],
 }
 
 
@@ -60,7 +70,8 @@ def main():
                 "head": [[int(a), int(b), float(c), float(e)] for a, b, c, e in
                          zip(r["src"][:8], r["tar"][:8], r["weight"][:8], r["delay"][:8])],
                 "nlifconst": r["nlifconst"],
-                "lifconst": [float(x) for x in r["lifconst"]],
+                # the reference leaves Network::lifconst uninitialised unless a model sets it
+                "lifconst": [float(x) for x in r["lifconst"]] if r["nlifconst"] else None,
                 "life1": [float(x) for x in r["life1"]],
                 "life2": [float(x) for x in r["life2"]],
                 "life3": [float(x) for x in r["life3"]],
