@@ -12,11 +12,20 @@
  *                       legacy MT19937 seeding
  *   orc_glibc_*         glibc rand()/srand() called from the test
  *   orc_gen_net         reference NetGen::gen_net compiled from its own sources
- *                       (oracle/_ref/ref_netgen), plus closed-form counts
+ *                       (oracle/_ref/ref_netgen) on fixtures and live specs, incl.
+ *                       quota projections that run short of gen_num; closed-form
+ *                       counts (every quota projection emits exactly gen_num)
  *   orc_build_csr       brute-force invariants + delay known answers
- *   orc_sim_* (F32)     closed forms (free decay, single-synapse jump, refractory
+ *   orc_sim_* LIFB      closed forms (free decay, single-synapse jump, refractory
  *                       ISI, seq-sum of equal addends), brute force on tiny nets
- *   orc_sim_* (F64)     agreement with F32 to FP32 rounding on sub-threshold runs
+ *   orc_sim_* LIFE      closed forms in F64 (~1e-12) and F32 (few ulps): free
+ *                       relaxation to E_L + I_off and its first spike step,
+ *                       conductance decay 0.98^k / 0.99^k (also while
+ *                       refractory), the steady-drive fixed point
+ *                       V = (E_L + I_off + E_inh I_i) / (1 + I_e + I_i)
+ *   orc_sim_* LIF       closed forms in F64 and F32: free decay
+ *                       C17^n V0 + C14 (1 - C17^n) / (1 - C17), the response to
+ *                       one arrival C12 w sum C17^(k-1-j) C15^j (C13, C16 for w < 0)
  *   STDP                two-neuron scripted pairs against A*expf(-k/tau)
  */
 #ifndef MGB_ORACLE_H

@@ -235,3 +235,129 @@ def test_multithreaded_step_loop_equals_sequential():
     assert o1.syn_events() == o2.syn_events()
     o1.close()
     o2.close()
+
+
+# ---- LIFE (mgcore.cu:197-233) and LIF (mgcore.cu:132-163) pinned by closed forms --------
+#
+# The FP64 mode is checked against each closed form to ~1e-12 and the FP32 mode to a few
+# float ulps of the accumulated trajectory, so a swapped decay constant, a wrong reversal
+# potential or an input added at the wrong point of the update fails one of them.
+
+K005, K098, K099 = float(f32(0.005)), float(f32(0.98)), float(f32(0.99))
+E_L, E_I, I_OFF = float(f32(-0.06)), float(f32(-0.08)), float(f32(0.02))
+
+
+def _life_lc(v_th=-0.05):
+    lc = M.life_const1().copy()
+    lc[4] = f32(v_th)
+    return lc
+
+
+@pytest.mark.parametrize("acc", [po.ACC_F64, po.ACC_F32])
+def test_life_free_relaxation_and_first_spike(acc):
+    """No input: V_{k} = V* + (V0 - V*) (1 - 0.005)^k with V* = E_L + I_off = -0.04 V,
+    and the neuron fires at the first step whose V is at or above V_th = -0.05 V."""
+    v0 = -0.06
+    vstar = E_L + I_OFF
+    sim = tiny([po.LIFE], [], v0=[v0], lc=_life_lc(), acc=acc)
+    tol = 1e-12 if acc == po.ACC_F64 else 2e-6
+    done = 0
+    for k in (1, 10, 60, 120):
+        sim.run(k - done)
+        done = k
+        want = vstar + (float(f32(v0)) - vstar) * (1 - K005) ** k
+        assert math.isclose(sim.state(po.F_V)[0], want, rel_tol=tol), k
+    # first spike: the step s whose pre-update V_s first reaches V_th
+    vth = float(f32(-0.05))
+    s_fire = next(k for k in range(1000) if vstar + (float(f32(v0)) - vstar) * (1 - K005) ** k >= vth)
+    fresh = tiny([po.LIFE], [], v0=[v0], lc=_life_lc(), acc=acc)
+    spikes = fresh.run(s_fire + 2)
+    assert [k for k, s in enumerate(spikes) if len(s)] == [s_fire]
+    assert fresh.state(po.F_V)[0] == float(f32(-0.06))            # reset to C[3]
+
+
+@pytest.mark.parametrize("acc", [po.ACC_F64, po.ACC_F32])
+@pytest.mark.parametrize("target_refractory", [False, True])
+def test_life_conductances_decay_geometrically(acc, target_refractory):
+    """One excitatory arrival w_e and one inhibitory arrival w_i (< 0) at step d:
+    I_exc = w_e 0.98^k and I_inh = -w_i 0.99^k, k steps later -- also while the
+    target is refractory (the conductance update precedes the refractory test)."""
+    we, wi, d = f32(0.03), f32(-0.2), 3
+    # neurons 0 and 1 start above threshold: they fire at step 0 only within 100 steps
+    v0 = [0.5, 0.5, 0.5 if target_refractory else -0.07]
+    sim = tiny([po.LIFE] * 3, [(0, 2, we, d), (1, 2, wi, d)], v0=v0, lc=_life_lc(), acc=acc)
+    sim.run(d)
+    tol = 1e-12 if acc == po.ACC_F64 else 4e-6
+    for k in range(0, 60):
+        sim.run(1)
+        assert math.isclose(sim.state(po.F_IEXC)[2], float(we) * K098 ** k, rel_tol=tol), k
+        assert math.isclose(sim.state(po.F_IINH)[2], -float(wi) * K099 ** k, rel_tol=tol), k
+        if target_refractory and d + k <= 49:       # refractory for 49 steps after its step-0 spike
+            assert sim.state(po.F_V)[2] == float(f32(-0.06))
+    assert sim.state(po.F_CNT)[2] == (1 if target_refractory else 0)
+
+
+@pytest.mark.parametrize("acc", [po.ACC_F64, po.ACC_F32])
+def test_life_steady_drive_fixed_point(acc):
+    """Two always-firing Poisson sources (rate 2 > every u) deliver w_e and w_i each step:
+    I_exc -> w_e / (1 - 0.98), I_inh -> -w_i / (1 - 0.99), and V settles where the update
+    vanishes: V_inf = (E_L + I_off + E_inh I_inh) / (1 + I_exc + I_inh)."""
+    we, wi = f32(0.001), f32(-0.01)
+    sim = tiny([po.POISSON, po.POISSON, po.LIFE], [(0, 2, we, 1), (1, 2, wi, 1)], rate=[2.0, 2.0, 0.0],
+               v0=[0.0, 0.0, -0.06], lc=_life_lc(), acc=acc)
+    sim.run(20000)
+    ie, ii = float(we) / (1 - K098), -float(wi) / (1 - K099)
+    vinf = (E_L + I_OFF + E_I * ii) / (1 + ie + ii)
+    assert sim.state(po.F_CNT)[2] == 0
+    tol = 1e-9 if acc == po.ACC_F64 else 2e-4
+    assert math.isclose(sim.state(po.F_IEXC)[2], ie, rel_tol=tol)
+    assert math.isclose(sim.state(po.F_IINH)[2], ii, rel_tol=tol)
+    assert math.isclose(sim.state(po.F_V)[2], vinf, rel_tol=tol)
+
+
+def _lif_lc():
+    # v_init, v_rest, v_reset, cm, tau_m, t_ref, tau_e, tau_i, v_th, i_off: C14 != 0 (rest and offset)
+    return M.gen_life_const(-0.07, -0.06, -0.065, 0.25, 0.02, 0.002, 0.005, 0.01, -0.05, 0.1, dt=1e-4)
+
+
+@pytest.mark.parametrize("acc", [po.ACC_F64, po.ACC_F32])
+def test_lif_free_decay_closed_form(acc):
+    """No input: V_n = C17^n V0 + C14 (1 - C17^n) / (1 - C17)."""
+    lc = _lif_lc()
+    c14, c17 = float(lc[14]), float(lc[17])
+    assert c14 != 0.0 and 0.0 < c17 < 1.0
+    v0 = float(f32(-0.07))
+    sim = tiny([po.LIF], [], v0=[v0], lc=lc, acc=acc)
+    done = 0
+    for n in (1, 5, 50, 400):
+        sim.run(n - done)
+        done = n
+        want = c17 ** n * v0 + c14 * (1 - c17 ** n) / (1 - c17)
+        assert math.isclose(sim.state(po.F_V)[0], want, rel_tol=1e-12 if acc == po.ACC_F64 else 2e-6), n
+    assert sim.state(po.F_CNT)[0] == 0
+
+
+@pytest.mark.parametrize("acc", [po.ACC_F64, po.ACC_F32])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_lif_single_input_response(acc, sign):
+    """A single arrival w at step s enters I after that step's V update; k steps later
+    V has moved by C12 w sum_j C17^(k-1-j) C15^j (C13 and C16 for w < 0), and the
+    current has decayed to w C15^k (C16^k)."""
+    lc = M.gen_life_const(0.0, 0.0, 0.0, 0.25, 0.02, 0.002, 0.005, 0.01, 0.02, 0.0, dt=1e-4)
+    assert lc[14] == 0.0
+    w = f32(sign * 0.4)
+    cg, cd, c17 = (float(lc[12]), float(lc[15]), float(lc[17])) if sign > 0 else \
+        (float(lc[13]), float(lc[16]), float(lc[17]))
+    d = 4
+    # neuron 0 (LIFB) starts above threshold and fires at step 0 only
+    sim = tiny([po.LIFB, po.LIF], [(0, 1, w, d)], v0=[0.5, 0.0], lc=lc, acc=acc)
+    sim.run(d + 1)
+    field = po.F_IEXC if sign > 0 else po.F_IINH
+    assert sim.state(po.F_V)[1] == 0.0 and sim.state(field)[1] == float(w)
+    tol = 1e-12 if acc == po.ACC_F64 else 4e-6
+    for k in range(1, 80):
+        sim.run(1)
+        want_v = cg * float(w) * sum(c17 ** (k - 1 - j) * cd ** j for j in range(k))
+        assert math.isclose(sim.state(po.F_V)[1], want_v, rel_tol=tol), k
+        assert math.isclose(sim.state(field)[1], float(w) * cd ** k, rel_tol=tol), k
+    assert sim.state(po.F_CNT)[1] == 0
