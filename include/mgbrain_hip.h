@@ -248,6 +248,16 @@ mgb_status mgb_read_csr(mgb_sim* sim, int64_t* row_ptr, int32_t* col, int32_t* d
 /* Current weights in CSR order (changed by STDP). */
 mgb_status mgb_read_weights(mgb_sim* sim, float* out);
 
+/* As mgb_read_weights, allocating no device memory (no hipMalloc / hipFree):
+ * no counterpart in the reference; for callers that cannot allocate (code under
+ * graph capture, tight serving loops, harnesses that forbid it).  Its working
+ * memory is one fixed device array per GPU (2^24 floats, 64 MiB, held until the
+ * process ends): a network needing more -- S slots for the CLASS, BF16 and FP8
+ * formats, the padded tile-order length with fused STDP -- gets
+ * MGB_ERR_UNSUPPORTED.  Calls on one GPU share that array: calls that overlap on
+ * one GPU (from several threads) race. */
+mgb_status mgb_read_weights_noalloc(mgb_sim* sim, float* out);
+
 /* Checkpoint of everything a run changes: neuron state, ring buffers, the
  * pending spike lists and their counters, step counters, plastic weights and
  * the recorded spikes (the reference keeps none; its stale network writer is

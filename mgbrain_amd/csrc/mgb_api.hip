@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -1355,6 +1356,13 @@ mgb_status create_common(mgb_sim* s, const HostNeurons& h, const DevList& L, con
 
 }  // namespace
 
+// Device memory of mgb_read_weights_noalloc, one copy per GPU, held until the
+// process ends.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns on each device (scratch_on).
+constexpr uint64_t kNoAllocSlots = 1ull << 24;   // floats: 64 MiB
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) float scratch[kNoAllocSlots];
+
 extern "C" {
 
 int32_t mgb_abi_version(void) { return MGB_ABI_VERSION; }
@@ -2052,7 +2060,23 @@ namespace {
 
 // kTStdp: the fused kernel applies a window's STDP in the next window's launch.
 // Writes to w32 (CSR order) the weights the reference holds after the last
-// step run: a copy of w_t with the pending window's updates applied.
+// step run: a copy of w_t (in tmp, n_tslots floats) with the pending window's
+// updates applied.
+hipError_t current_weights_into(mgb_sim* s, float* tmp) {
+    const size_t S = (size_t)s->n_tslots;
+    TiledArgs ta = tiled_args(s);
+    ta.w_t = tmp;
+    ta.apply_only = 1;
+    hipError_t e = hipMemcpyAsync(tmp, s->w_t, S * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = launch_tiled(ta, 0, 0, 1, s->tblock, s->stream);
+    if (e == hipSuccess) e = launch_stdp_flush(ta, 0, 0, s->stream);
+    // the list this used is empty before the next window appends to it
+    if (e == hipSuccess) e = hipMemsetAsync(s->fcnt, 0, 8, s->stream);
+    if (e == hipSuccess) e = scatter_weights(s->tperm, tmp, (uint64_t)S, s->w32, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return e;
+}
+
 mgb_status current_weights(mgb_sim* s) {
     const size_t S = (size_t)s->n_tslots;
     float* tmp = nullptr;
@@ -2061,22 +2085,37 @@ mgb_status current_weights(mgb_sim* s) {
         set_error("hipMalloc(%zu) for the weight read-back failed: %s", S * 4, hipGetErrorString(e));
         return e == hipErrorOutOfMemory ? MGB_ERR_OOM : MGB_ERR_HIP;
     }
-    TiledArgs ta = tiled_args(s);
-    ta.w_t = tmp;
-    ta.apply_only = 1;
-    e = hipMemcpyAsync(tmp, s->w_t, S * 4, hipMemcpyDeviceToDevice, s->stream);
-    if (e == hipSuccess) e = launch_tiled(ta, 0, 0, 1, s->tblock, s->stream);
-    if (e == hipSuccess) e = launch_stdp_flush(ta, 0, 0, s->stream);
-    // the list this used is empty before the next window appends to it
-    if (e == hipSuccess) e = hipMemsetAsync(s->fcnt, 0, 8, s->stream);
-    if (e == hipSuccess) e = scatter_weights(s->tperm, tmp, (uint64_t)S, s->w32, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    e = current_weights_into(s, tmp);
     const hipError_t f = hipFree(tmp);
     if (e == hipSuccess) e = f;
     if (e != hipSuccess) {
         set_error("weight read-back: %s", hipGetErrorString(e));
         return MGB_ERR_HIP;
     }
+    return MGB_OK;
+}
+
+std::mutex g_scratch_mu;
+float* g_scratch_at[kNoAllocDevices] = {};
+
+// `device` must be current.
+mgb_status scratch_on(int device, float** out) {
+    if (device < 0 || device >= kNoAllocDevices) {
+        set_error("device %d outside the %d devices the no-allocation read-back supports", device,
+                  kNoAllocDevices);
+        return MGB_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_at[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch) on device %d failed: %s", device, hipGetErrorString(e));
+            return MGB_ERR_HIP;
+        }
+        g_scratch_at[device] = static_cast<float*>(p);
+    }
+    *out = g_scratch_at[device];
     return MGB_OK;
 }
 
@@ -2165,6 +2204,47 @@ mgb_status mgb_read_weights(mgb_sim* s, float* out) {
         return MGB_ERR_INVALID;
     }
     return mgb_read_csr(s, nullptr, nullptr, nullptr, out);
+}
+
+mgb_status mgb_read_weights_noalloc(mgb_sim* s, float* out) {
+    if (!s || !out) {
+        set_error("null argument");
+        return MGB_ERR_INVALID;
+    }
+    MGB_TRY(mgb_sync(s));
+    const uint64_t S = (uint64_t)s->S;
+    if (S == 0) return MGB_OK;
+    const bool pending = s->tiled && s->tkind == kTStdp;   // fused STDP: updates still to apply
+    const bool decode = s->mode == kRingClass || s->wide_kind != kWideF32;
+    uint64_t need = 0;
+    if (pending) need = s->n_tslots;
+    if (decode) need = std::max(need, S);
+    if (need > kNoAllocSlots) {
+        set_error("mgb_read_weights_noalloc: the network needs %llu scratch floats, the device scratch holds %llu "
+                  "(use mgb_read_weights)", (unsigned long long)need, (unsigned long long)kNoAllocSlots);
+        return MGB_ERR_UNSUPPORTED;
+    }
+    float* scr = nullptr;
+    if (need) MGB_TRY(scratch_on(s->device, &scr));
+    if (pending) {
+        const hipError_t e = current_weights_into(s, scr);
+        if (e != hipSuccess) {
+            set_error("weight read-back: %s", hipGetErrorString(e));
+            return MGB_ERR_HIP;
+        }
+    }
+    const float* from = s->w32;
+    if (s->mode == kRingClass) {
+        MGB_HIP_TRY(class_weights(s->syn, s->tbl, s->tbl_w, S, scr, s->stream));
+        from = scr;
+    } else if (decode) {
+        MGB_HIP_TRY(wide_weights(s->row_ptr, s->n, s->dly, s->w32, s->wide_kind == kWideBF16 ? s->w16 : nullptr,
+                                 s->wide_kind == kWideFP8 ? s->w8 : nullptr, s->row_scale, scr, s->stream));
+        from = scr;
+    }
+    MGB_HIP_TRY(hipMemcpyAsync(out, from, 4 * S, hipMemcpyDeviceToHost, s->stream));
+    MGB_HIP_TRY(hipStreamSynchronize(s->stream));
+    return MGB_OK;
 }
 
 mgb_status mgb_destroy(mgb_sim* s) {

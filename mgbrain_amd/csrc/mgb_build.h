@@ -92,6 +92,14 @@ hipError_t tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col
 hipError_t invert_slots(const uint32_t* slot, uint64_t S, uint32_t* inv, hipStream_t st);
 hipError_t remap_u32(uint32_t* a, const uint32_t* map, uint64_t np, hipStream_t st);
 hipError_t scatter_weights(const uint32_t* slot, const float* w_t, uint64_t S, float* w32, hipStream_t st);
+// Weights of every CSR slot as mgb_read_csr decodes them: CLASS words through the
+// targets' class tables; BF16 (w16) or FP8 (w8) codes times the row scales, plastic
+// slots from w32.
+hipError_t class_weights(const uint32_t* syn, const uint16_t* tbl, const float* tbl_w, uint64_t S, float* out,
+                         hipStream_t st);
+hipError_t wide_weights(const int64_t* row_ptr, int32_t n, const uint8_t* dly, const float* w32,
+                        const uint16_t* w16, const uint8_t* w8, const float* row_scale, float* out,
+                        hipStream_t st);
 hipError_t plastic_isi(const int64_t* in_ptr, const int32_t* in_src, const int32_t* isi, int32_t n, uint64_t np,
                        int32_t* out, hipStream_t st);
 hipError_t tile_offsets(const uint32_t* sorted_keys, const int64_t* row_ptr, const TileTabs& tt,

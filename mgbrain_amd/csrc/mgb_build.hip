@@ -198,6 +198,44 @@ __global__ void k_scatter_w(const uint32_t* slot, const float* w_t, uint64_t S, 
     }
 }
 
+// CLASS format: out[k] = the class weight of synapse word k (target's table,
+// sign bit), as the host read-back decodes it.
+__global__ void k_class_weights(const uint32_t* syn, const uint16_t* tbl, const float* tbl_w, uint64_t S,
+                                float* out) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = syn[k];
+        out[k] = tbl_w[2u * (uint32_t)tbl[x & kIdMask] + (x >> 31)];
+    }
+}
+
+// OCP FP8 E4M3 (bias 7, no infinities) to FP32, as the kernels decode it.
+__device__ __forceinline__ float fp8_value(uint8_t b) {
+    const uint32_t s = (uint32_t)(b >> 7) << 31;
+    const uint32_t e = (b >> 3) & 15u, m = b & 7u;
+    if (e == 0) return __uint_as_float(__float_as_uint((float)m * 0x1p-9f) | s);
+    return __uint_as_float(s | ((e + 120u) << 23) | (m << 20));
+}
+
+// BF16 / FP8 formats: the weight of every CSR slot (plastic slots: their FP32
+// weight; others: the stored code times the source row's scale).  One wave per
+// source row.
+__global__ void k_wide_weights(const int64_t* row_ptr, int32_t n, const uint8_t* dly, const float* w32,
+                               const uint16_t* w16, const uint8_t* w8, const float* row_scale, float* out) {
+    const int32_t lane = (int32_t)(threadIdx.x & 63u);
+    const int32_t waves = (int32_t)(gridDim.x * (blockDim.x >> 6));
+    for (int32_t i = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); i < n; i += waves) {
+        const float rs = row_scale[i];
+        for (int64_t k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += 64) {
+            float v;
+            if (dly[k] & 128u) v = w32[k];
+            else if (w16) v = __uint_as_float((uint32_t)w16[k] << 16) * rs;
+            else v = fp8_value(w8[k]) * rs;
+            out[k] = v;
+        }
+    }
+}
+
 // out[idx[k]] = val[k].
 __global__ void k_scatter_u32(const uint32_t* idx, const uint32_t* val, uint64_t S, uint32_t* out) {
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
@@ -492,6 +530,22 @@ hipError_t remap_u32(uint32_t* a, const uint32_t* map, uint64_t np, hipStream_t 
 hipError_t scatter_weights(const uint32_t* slot, const float* w_t, uint64_t S, float* w32, hipStream_t st) {
     if (S == 0) return hipSuccess;
     hipLaunchKernelGGL(k_scatter_w, dim3(blocks_for(S, 256)), dim3(256), 0, st, slot, w_t, S, w32);
+    return hipGetLastError();
+}
+
+hipError_t class_weights(const uint32_t* syn, const uint16_t* tbl, const float* tbl_w, uint64_t S, float* out,
+                         hipStream_t st) {
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_class_weights, dim3(blocks_for(S, 256)), dim3(256), 0, st, syn, tbl, tbl_w, S, out);
+    return hipGetLastError();
+}
+
+hipError_t wide_weights(const int64_t* row_ptr, int32_t n, const uint8_t* dly, const float* w32,
+                        const uint16_t* w16, const uint8_t* w8, const float* row_scale, float* out,
+                        hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_wide_weights, dim3(blocks_for((uint64_t)n * 64u, 256)), dim3(256), 0, st, row_ptr, n,
+                       dly, w32, w16, w8, row_scale, out);
     return hipGetLastError();
 }
 

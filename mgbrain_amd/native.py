@@ -25,7 +25,8 @@ F_V, F_IEXC, F_IINH, F_REFR, F_LAST, F_CNT = range(6)
 EXPORTS = ["mgb_abi_version", "mgb_last_error", "mgb_create", "mgb_create_generated",
            "mgb_create_batch_generated", "mgb_run",
            "mgb_sync", "mgb_set_launch", "mgb_get_info", "mgb_read_spikes", "mgb_clear_spikes",
-           "mgb_read_state", "mgb_fire_counts", "mgb_read_csr", "mgb_read_weights", "mgb_checkpoint_size",
+           "mgb_read_state", "mgb_fire_counts", "mgb_read_csr", "mgb_read_weights", "mgb_read_weights_noalloc",
+           "mgb_checkpoint_size",
            "mgb_checkpoint_save", "mgb_checkpoint_load", "mgb_destroy"]
 
 _p = C.c_void_p
@@ -111,6 +112,7 @@ def lib():
         L.mgb_fire_counts.argtypes = [_p, _p]
         L.mgb_read_csr.argtypes = [_p, _p, _p, _p, _p]
         L.mgb_read_weights.argtypes = [_p, _p]
+        L.mgb_read_weights_noalloc.argtypes = [_p, _p]
         L.mgb_checkpoint_size.argtypes = [_p, C.POINTER(C.c_int64)]
         L.mgb_checkpoint_save.argtypes = [_p, _p, C.c_int64]
         L.mgb_checkpoint_load.argtypes = [_p, _p, C.c_int64]
@@ -325,6 +327,14 @@ class Simulator:
         s = self.info()["n_synapses"]
         w = np.zeros(max(s, 1), dtype=np.float32)
         _check(lib().mgb_read_weights(self.h, _addr(w)))
+        return w[:s]
+
+    def weights_noalloc(self):
+        """mgb_read_weights_noalloc: the same weights, read through the library's fixed
+        device scratch (no device allocation)."""
+        s = self.info()["n_synapses"]
+        w = np.zeros(max(s, 1), dtype=np.float32)
+        _check(lib().mgb_read_weights_noalloc(self.h, _addr(w)))
         return w[:s]
 
     # -- checkpoint -------------------------------------------------------------

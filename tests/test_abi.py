@@ -124,6 +124,7 @@ def test_null_arguments():
     h = C.c_void_p()
     assert N.lib().mgb_create(None, None, C.byref(h)) == N.ERR_INVALID
     assert N.lib().mgb_run(None, 10) == N.ERR_INVALID
+    assert N.lib().mgb_read_weights_noalloc(None, None) == N.ERR_INVALID
     assert N.lib().mgb_destroy(None) == N.OK
 
 
