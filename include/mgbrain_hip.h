@@ -67,8 +67,18 @@ typedef struct mgb_opts {
     int32_t tile_size;         /* TILED: mean target neurons per workgroup (tiles are balanced by
                                   in-degree and never mix neuron types); 0 = default */
     int32_t chunk;             /* TILED: synapses per propagation work item (1, 2, 4, 8, 16; 0 = 8) */
-    int32_t reserved[4];
+    int32_t persistent;        /* TILED without STDP: MGB_PERSIST_* */
+    int32_t spin_limit_us;     /* persistent launches: bound on a workgroup's wait between two
+                                  windows before the run fails with MGB_ERR_HIP; 0 = 2 s */
+    int32_t reserved[2];
 } mgb_opts;
+
+/* Launch form of the fused kernel. */
+enum {
+    MGB_PERSIST_AUTO = 0,  /* the whole windows of a run (or of a graph block) in one launch when
+                              every workgroup fits on the device at once, else one launch per window */
+    MGB_PERSIST_OFF = 1    /* one launch per window */
+};
 
 /* Spike propagation strategy. */
 enum {
@@ -179,6 +189,7 @@ typedef struct mgb_info {
     int32_t block;             /* workgroup size of the propagation / fused kernel */
     int32_t segments;          /* independent segments (networks of a batch) */
     int32_t word_bytes;        /* bytes per synapse the fused kernel reads (0: direct path) */
+    int32_t persistent;        /* 1: runs use the persistent launch (MGB_PERSIST_AUTO and resident) */
 } mgb_info;
 
 typedef struct mgb_sim mgb_sim;

@@ -118,11 +118,19 @@ struct mgb_sim {
     int64_t* prow = nullptr;           // first slot of each padded row of syn16 / syn_t / w_t
     float* tile_cc = nullptr;          // CLASS kinds: class cache of each tile
     uint32_t* tile_ccls = nullptr;
+    uint8_t* tile_lean = nullptr;      // every neuron of the tile an LIFB neuron of its two classes
     float* w_t = nullptr;              // kTF32 / kTStdp weights in tile order
     uint16_t* offs16 = nullptr;
     uint32_t* offs32 = nullptr;
     uint32_t* ring_t = nullptr;
     size_t ring_slab = 0;              // words per tile in ring_t (RW * D * B rounded up to 16 bytes)
+    // persistent launches of the fused kernel (MGB_PERSIST_AUTO, every workgroup resident)
+    int persist_req = MGB_PERSIST_AUTO;
+    int persist = 0;
+    uint32_t* bar = nullptr;           // [G * kBarStride] grid-wait flags
+    uint32_t* bar_err = nullptr;       // set by a wait that timed out
+    uint64_t spin_limit = 0;           // wall-clock ticks
+    bool failed = false;               // a persistent launch timed out: results void
     // host mirrors for read-back
     std::vector<float> class_w;        // [n_tbl][2]
     std::vector<uint16_t> h_tbl;       // [n]
@@ -1006,6 +1014,8 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
     return MGB_OK;
 }
 
+void update_persist(mgb_sim* s);
+
 mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     const int32_t n = h.n;
     hipStream_t st = s->stream;
@@ -1028,6 +1038,11 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     MGB_TRY(dalloc_n(s, &s->cnt, np));
     MGB_HIP_TRY(hipMemsetAsync(s->ctl, 0, sizeof(Ctl), st));
     MGB_HIP_TRY(hipMemcpyAsync(s->type, h.type.data(), (size_t)n, hipMemcpyHostToDevice, st));
+    if (s->tiled && s->tile_cc) {
+        // CLASS kinds: which tiles hold only LIFB neurons of their two cached classes
+        MGB_TRY(dalloc_n(s, &s->tile_lean, (size_t)s->G));
+        MGB_HIP_TRY(launch_tile_lean(s->tile_lo, s->G, s->type, s->tbl, s->tile_lean, st));
+    }
     MGB_HIP_TRY(hipMemcpyAsync(s->cset, h.cset.data(), (size_t)n, hipMemcpyHostToDevice, st));
     MGB_HIP_TRY(hipMemcpyAsync(s->stream_idx, h.stream.data(), 2 * (size_t)n, hipMemcpyHostToDevice, st));
     MGB_HIP_TRY(hipMemcpyAsync(s->rate, h.rate.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
@@ -1122,6 +1137,20 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     s->profile = o ? o->profile : 0;
     s->prop_block = (o && o->prop_block >= 64 && o->prop_block <= 1024) ? (o->prop_block / 64) * 64 : 256;
     s->prop_grid = (o && o->prop_grid > 0) ? o->prop_grid : 0;
+    if (s->tiled && s->tkind != kTStdp) {
+        // persistent launches: wait flags (reset before each launch) and the wait's bound
+        MGB_TRY(dalloc_n(s, &s->bar, (size_t)s->G * kBarStride));
+        MGB_TRY(dalloc_n(s, &s->bar_err, 1));
+        MGB_HIP_TRY(hipMemsetAsync(s->bar, 0, 4 * (size_t)s->G * kBarStride, st));
+        MGB_HIP_TRY(hipMemsetAsync(s->bar_err, 0, 4, st));
+        s->persist_req = o ? o->persistent : MGB_PERSIST_AUTO;
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device) != hipSuccess || khz <= 0)
+            khz = 100000;
+        const uint64_t us = (o && o->spin_limit_us > 0) ? (uint64_t)o->spin_limit_us : 2000000ull;
+        s->spin_limit = us * (uint64_t)khz / 1000ull;
+        update_persist(s);
+    }
     MGB_HIP_TRY(hipStreamSynchronize(st));
     return MGB_OK;
 }
@@ -1165,6 +1194,24 @@ StdpArgs stdp_args(const mgb_sim* s) {
     return a;
 }
 
+// Every workgroup of a fused launch of `a` (persistent form or not) fits on the
+// device at once.
+bool resident(const mgb_sim* s, const TiledArgs& a, bool persistent) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess) return false;
+    return (int64_t)a.G <= (int64_t)cus * tiled_blocks_per_cu(a, s->tblock, persistent);
+}
+
+TiledArgs tiled_args(const mgb_sim* s);
+
+// Persistent launches run when asked for (MGB_PERSIST_AUTO), the kernel has the
+// form (not STDP) and every workgroup of the current launch shape is resident.
+void update_persist(mgb_sim* s) {
+    s->persist = 0;
+    if (!s->tiled || s->tkind == kTStdp || s->persist_req != MGB_PERSIST_AUTO || !s->bar) return;
+    s->persist = resident(s, tiled_args(s), true) ? 1 : 0;
+}
+
 TiledArgs tiled_args(const mgb_sim* s) {
     TiledArgs a{};
     a.n = s->n; a.D = s->D; a.B = s->B; a.G = s->G; a.chunk = s->chunk;
@@ -1183,7 +1230,7 @@ TiledArgs tiled_args(const mgb_sim* s) {
     a.tbl_off = s->tbl_off; a.seq = s->seq; a.tbl_w = s->tbl_w;
     a.row_ptr = s->row_ptr; a.syn_t = s->syn_t; a.syn16 = s->syn16; a.offs16 = s->offs16; a.offs32 = s->offs32;
     a.prow = s->prow;
-    a.tile_cc = s->tile_cc; a.tile_ccls = s->tile_ccls;
+    a.tile_cc = s->tile_cc; a.tile_ccls = s->tile_ccls; a.tile_lean = s->tile_lean;
     a.ring_t = s->ring_t; a.ring_slab = s->ring_slab; a.rw = s->trw;
     a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
     a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
@@ -1195,9 +1242,8 @@ TiledArgs tiled_args(const mgb_sim* s) {
         a.ltp = s->ltp; a.ltd = s->ltd; a.lut_n = s->lut_n; a.lut_lds = s->lut_lds;
         a.w_max = s->sc[4]; a.w_min = s->sc[5];
     }
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess) cus = 0;
-    a.xcd_map = (int64_t)a.G <= (int64_t)cus * tiled_blocks_per_cu(a, s->tblock) ? 1 : 0;
+    a.bar = s->bar; a.bar_err = s->bar_err; a.spin_limit = s->spin_limit;
+    a.xcd_map = resident(s, a, false) ? 1 : 0;
     return a;
 }
 
@@ -1254,6 +1300,17 @@ mgb_status issue_windows(mgb_sim* s, int32_t nsteps, bool finalize, bool capturi
     int32_t off = 0, wk = 0;
     if (s->tiled) {
         const TiledArgs ta = tiled_args(s);
+        const int32_t nfull = nsteps / s->window;
+        if (s->persist && nfull >= 2) {
+            // the run's whole windows in one launch
+            SpanGuard g1{s, kSpanProp, prof, nfull};
+            MGB_TRY(g1.begin());
+            MGB_HIP_TRY(launch_tiled_persistent(ta, 0, 0, s->window, nfull, s->tblock, s->stream));
+            MGB_TRY(g1.end());
+            s->launches += 2;
+            off = nfull * s->window;
+            wk = nfull;
+        }
         while (off < nsteps) {
             const int32_t nst = std::min(s->window, nsteps - off);
             SpanGuard g1{s, kSpanProp, prof};
@@ -1340,7 +1397,8 @@ mgb_status validate_opts(const mgb_opts* o) {
     if (!o) return MGB_OK;
     if (o->weight_format < MGB_WF_AUTO || o->weight_format > MGB_WF_FP8 || o->window < 0 ||
         o->record_capacity < 0 || o->propagation < MGB_PROP_AUTO || o->propagation > MGB_PROP_TILED ||
-        o->tile_size < 0 || o->device < 0) {
+        o->tile_size < 0 || o->device < 0 || o->persistent < MGB_PERSIST_AUTO || o->persistent > MGB_PERSIST_OFF ||
+        o->spin_limit_us < 0) {
         set_error("invalid mgb_opts");
         return MGB_ERR_INVALID;
     }
@@ -1723,6 +1781,10 @@ mgb_status mgb_run(mgb_sim* s, int32_t nsteps) {
         set_error("invalid arguments to mgb_run");
         return MGB_ERR_INVALID;
     }
+    if (s->failed) {
+        set_error("an earlier persistent launch of this simulator timed out; its state is void");
+        return MGB_ERR_HIP;
+    }
     MGB_HIP_TRY(hipSetDevice(s->device));
     if (s->profile) {
         // the previous run's events may still be pending; reuse the pool after it ends
@@ -1761,6 +1823,17 @@ mgb_status mgb_sync(mgb_sim* s) {
     }
     MGB_HIP_TRY(hipSetDevice(s->device));
     MGB_HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->bar_err && !s->failed) {
+        uint32_t e = 0;
+        MGB_HIP_TRY(hipMemcpy(&e, s->bar_err, 4, hipMemcpyDeviceToHost));
+        s->failed = e != 0;
+    }
+    if (s->failed) {
+        set_error("a persistent launch's wait between windows exceeded its bound (%llu clock ticks): a workgroup "
+                  "was not resident or a window ran longer than the bound; the simulator's state is void",
+                  (unsigned long long)s->spin_limit);
+        return MGB_ERR_HIP;
+    }
     return MGB_OK;
 }
 
@@ -1776,6 +1849,7 @@ mgb_status mgb_set_launch(mgb_sim* s, int32_t block, int32_t chunk) {
         else s->prop_block = block;
     }
     if (chunk > 0) s->chunk = chunk;
+    update_persist(s);
     if (s->gexec) {
         (void)hipGraphExecDestroy(s->gexec);
         s->gexec = nullptr;
@@ -1817,6 +1891,7 @@ mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
     out->block = s->tiled ? s->tblock : s->prop_block;
     out->segments = std::max<int32_t>(1, (int32_t)s->h_seg_lo.size() - 1);
     out->word_bytes = !s->tiled ? 0 : (s->tkind == kTClass16 ? 2 : (s->tkind == kTF32 || s->tkind == kTStdp ? 8 : 4));
+    out->persistent = s->persist;
     out->last_run_ms = 0.0;
     if (s->timed) {
         float ms = 0.f;

@@ -195,6 +195,7 @@ struct TiledArgs {
     const int64_t* prow;    // first slot of each padded row of syn16 / syn_t / w_t
     const float* tile_cc;   // CLASS kinds: [G][2][2][kClassCache] class cache of each tile (k_tile_ccache)
     const uint32_t* tile_ccls;   // [G][2] its classes
+    const uint8_t* tile_lean;    // [G] 1: every neuron an LIFB neuron of one of those classes
     uint32_t* ring_t;
     size_t ring_slab;       // words per tile in ring_t: RW * D * B rounded up to 16 bytes
     Ctl* ctl;
@@ -236,6 +237,11 @@ struct TiledArgs {
     // 1: workgroup b runs tile (b % kXcds) * (G / kXcds) + ... (a contiguous run of
     // tiles per XCD), set when every workgroup of the launch is resident at once
     int32_t xcd_map;
+    // persistent launches: one flag per tile (kBarStride words apart) for the wait
+    // between windows, its error word, and the wait's bound in wall-clock ticks
+    uint32_t* bar;
+    uint32_t* bar_err;
+    uint64_t spin_limit;
 };
 
 enum RingMode { kRingClass = 0, kRingFixed = 1 };
@@ -258,16 +264,26 @@ constexpr int kStdpHist = 16;       // spike steps kept per neuron for deferred 
 // work item of 2-16 words is aligned vector loads
 constexpr int kSubAlign = 8;
 constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 each
+constexpr int kBarStride = 32;      // words between the flags of a persistent launch's grid wait
 // Dynamic LDS of the fused kernel (rw ring words per slot and target); kTStdp also
 // caches lut_lds entries of each STDP table.
 size_t tiled_lds_bytes(int D, int B, int kind, int rw, int lut_lds = 0);
 // Class cache of every tile of the CLASS kinds (cc [G][2][2][kClassCache], cls [G][2]).
 hipError_t launch_tile_ccache(const int32_t* tile_lo, int32_t G, const uint16_t* tbl, const uint32_t* tbl_off,
                               const float* seq, float* cc, uint32_t* cls, hipStream_t st);
+// lean[g] = 1 when every neuron of tile g is an LIFB neuron of one of the tile's two
+// cached classes (needs the neuron types, uploaded after the tiles are built).
+hipError_t launch_tile_lean(const int32_t* tile_lo, int32_t G, const uint8_t* type, const uint16_t* tbl,
+                            uint8_t* lean, hipStream_t st);
 // Workgroups of `block` threads of the fused kernel for `a` that one CU holds at once.
-int tiled_blocks_per_cu(const TiledArgs& a, int block);
+int tiled_blocks_per_cu(const TiledArgs& a, int block, bool persistent = false);
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st);
+// Persistent form (not kTStdp): nwin windows of nsteps steps in one launch, after a
+// reset of the tiles' wait flags; every workgroup must be resident
+// (tiled_blocks_per_cu(a, block, true) x CUs >= G).
+hipError_t launch_tiled_persistent(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int32_t nwin,
+                                   int block, hipStream_t st);
 hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
 hipError_t set_tiled_lds_limit(int kind, size_t bytes);
 // kTStdp: brings the plastic in-synapses of the neurons the fused kernel listed

@@ -156,16 +156,24 @@ __device__ __forceinline__ void fixed_input(int type, int32_t qe, int32_t qi, fl
     }
 }
 
+// LIFB step (mgcore.cu:164-196) given this step's input sum be + bi.
+__device__ __forceinline__ bool integrate_lifb(NState& st, int32_t s, float input) {
+    if (st.refr > 0) {
+        --st.refr;
+        return false;
+    }
+    if (st.V >= st.c4) {
+        ++st.cnt; st.V = st.c3; st.last = s; st.refr = st.refr_steps;
+        return true;
+    }
+    st.V = st.V + fmaf(-0.005f, st.V, input);
+    return false;
+}
+
 __device__ __forceinline__ bool integrate_step(int type, NState& st, int32_t s, float be, float bi) {
     bool fired = false;
-    if (type == T_LIFB) {                                        // mgcore.cu:164-196
-        if (st.refr > 0) {
-            --st.refr;
-        } else if (st.V >= st.c4) {
-            fired = true; ++st.cnt; st.V = st.c3; st.last = s; st.refr = st.refr_steps;
-        } else {
-            st.V = st.V + fmaf(-0.005f, st.V, be + bi);
-        }
+    if (type == T_LIFB) {
+        fired = integrate_lifb(st, s, be + bi);
     } else if (type == T_LIFE) {                                 // :197-233
         st.Ie = fmaf(st.Ie, 0.98f, be);
         st.Ii = fmaf(st.Ii, 0.99f, -bi);
@@ -519,6 +527,53 @@ __device__ __forceinline__ void ring_slice_out(const uint32_t* ring, uint32_t* g
     }
 }
 
+// Words that workgroups of one persistent launch pass to each other between
+// windows (spike-list entries and counts) are written and read at agent scope,
+// which on gfx950 bypasses the XCD-private L2 (sc1).
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid-wide wait between two windows of a persistent launch.  Each thread first
+// waits for its own stores to complete (s_waitcnt vmcnt(0): the agent-scope
+// stores above are then visible at agent scope), then the workgroup publishes
+// `epoch` in its flag (kBarStride words apart, one cache line each) and waits
+// until every workgroup's flag holds at least `epoch`, one flag per thread.
+// The wait is bounded: past `limit` clock ticks, or once another workgroup has
+// reported a timeout, it sets *err and returns false, so every workgroup leaves
+// the kernel (the host turns *err into an error).
+__device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t G, uint32_t g, uint32_t epoch, uint32_t* err,
+                                          uint64_t limit) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);    // vmcnt(0)
+    __syncthreads();
+    if (threadIdx.x == 0) st_agent(&bar[(size_t)g * kBarStride], epoch);
+    int bad = 0;
+    for (uint32_t h = threadIdx.x; h < G && !bad; h += blockDim.x) {
+        const uint64_t c0 = (uint64_t)wall_clock64();
+        while ((int32_t)(ld_agent(&bar[(size_t)h * kBarStride]) - epoch) < 0) {
+            if (ld_agent(err) != 0u) {
+                bad = 1;
+                break;
+            }
+            if ((uint64_t)wall_clock64() - c0 > limit) {
+                st_agent(err, 1u);
+                bad = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return __syncthreads_or(bad) == 0;
+}
+
+__global__ void k_bar_reset(uint32_t* bar, uint32_t G) {
+    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < G; h += gridDim.x * blockDim.x)
+        st_agent(&bar[(size_t)h * kBarStride], 0u);
+}
+
 // One window of the fused tiled step (CLASS format).  Workgroup g (targets
 // [tile_lo[g], tile_lo[g+1]) of segment tile_seg[g]):
 //   1. loads its ring slice ring_t[g] (D x B words) into LDS;
@@ -559,8 +614,17 @@ __device__ __forceinline__ void ring_slice_out(const uint32_t* ring, uint32_t* g
 // With a.apply_only the launch brings every plastic synapse up to date through
 // the pending window in a.w_t (a copy), touching nothing else: the weights
 // the reference holds after the last step run.
-template <int CH, int WK>
-__global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_t wk, int32_t nsteps) {
+//
+// PW (persistent): the launch runs nwin windows of nsteps steps each, every
+// workgroup resident for all of them (the host checks occupancy).  The ring
+// slice stays in LDS and, for tiles of at most one neuron per thread, the
+// neuron state in registers, from the first window to the last; the window
+// lists pass between workgroups through agent-scope accesses and a grid_wait
+// between windows.  Not for kTStdp.
+template <int CH, int WK, bool PW>
+__global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_t wk, int32_t nsteps,
+                                                int32_t nwin) {
+    static_assert(!(PW && WK == kTStdp), "the persistent form has no STDP");
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ uint32_t wsum[16];
     __shared__ unsigned long long se_blk, lsp_r;
@@ -570,7 +634,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     __shared__ float ccache[2][2][kClassCache];
     __shared__ uint32_t ccls[2];
     const int32_t D = a.D, B = a.B, bd = (int32_t)blockDim.x;
-    const int32_t tid = (int32_t)threadIdx.x;
+    const int32_t tid0 = (int32_t)threadIdx.x;
     // the workgroup's tile.  Workgroups go round-robin to the kXcds XCDs
     // (blockIdx.x % kXcds), each with its own L2; neighbouring tiles share the
     // cache lines at their sub-row boundaries and their row offsets, so each XCD
@@ -625,422 +689,412 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const size_t lbase = (size_t)slo * (size_t)a.window;    // segment's part of each spike list
     uint32_t* tcnt = a.tcnt + q;                            // tcnt[b * nseg]
     const int32_t nseg = a.nseg;
-    // the window's index comes from ctl; the start step, spike count and first
-    // entries of all three spike lists load with it (one round trip instead of two,
-    // the index then picks one; the other lists' words, which other workgroups may
-    // be writing, are not used)
-    const uint32_t li = min((uint32_t)tid, (uint32_t)nq * (uint32_t)a.window - 1u);
-    const int64_t tb0 = ctl->tbase[0], tb1 = ctl->tbase[1], tb2 = ctl->tbase[2];
-    const uint32_t n0 = tcnt[0], n1 = tcnt[nseg], n2 = tcnt[2 * nseg];
-    const size_t ws = a.win_stride;
-    const uint32_t l0 = a.win_buf[lbase + li], l1 = a.win_buf[ws + lbase + li], l2 = a.win_buf[2 * ws + lbase + li];
-    const int64_t s0 = ctl->base_step + off;
-    const int64_t kwin = ctl->base_window + wk;
-    const int bcur = (int)(kwin % 3), bprev = (int)((kwin + 2) % 3), bnext = (int)((kwin + 1) % 3);
-    if (full && lt == 0 && tid == 0) tcnt[bnext * nseg] = 0u;
-    if (kPl && full && g == 0 && tid == 0) a.fcnt[(kwin + 1) & 1] = 0u;
-    if (full && g == 0 && tid == 0) ctl->tbase[bcur] = s0;
-    if (tid == 0) {
-        se_blk = 0ull;
-        lsp_n = 0u;
-    }
-    const int32_t* lastA = kPl ? a.last3 + (size_t)bprev * a.last_stride : nullptr;   // window k-1 start
-    const int32_t* lastB = kPl ? a.last3 + (size_t)bcur * a.last_stride : a.last;     // window k-1 end
-    int32_t* lastC = kPl ? a.last3 + (size_t)bnext * a.last_stride : a.last;          // window k end
-    if (kPl) {
-        for (int32_t j = tid; j < tn; j += bd) {
-            cL[j] = a.cnt[t0 + j];
-            tfL[j] = a.tf[t0 + j];
-        }
-        const int32_t* hg = a.hist + (size_t)t0 * kStdpHist;
-        for (int32_t k = tid; k < tn * kStdpHist; k += bd) hL[k] = hg[k];
-        for (int32_t k = tid; k < a.lut_lds; k += bd) {
-            lutL[k] = a.ltp[k];
-            lutL[a.lut_lds + k] = a.ltd[k];
-        }
-    }
-    if (kCls && tid < 4 * kClassCache) {
-        // (precomputed per tile by k_tile_ccache: one load instead of a chain of three)
-        ccache[tid / (2 * kClassCache)][(tid / kClassCache) & 1][tid % kClassCache] =
-            a.tile_cc[(size_t)g * 4 * kClassCache + tid];
-        if (tid < 2) ccls[tid] = a.tile_ccls[2 * g + tid];
-    }
-    // 1. ring slice -> LDS.  The slots the previous window consumed ([sprev, s0)
-    // mod D) were cleared by it and not stored back (step 4): they are zero here.
-    const int64_t sprev = bprev == 0 ? tb0 : (bprev == 1 ? tb1 : tb2);
-    // 0 before the first window; at most D (a network without synapses has D = 1
-    // and windows of up to kMaxWindow steps)
-    const int32_t nprev = (int32_t)min(s0 - sprev, (int64_t)D);
-    const int32_t rprev = (int32_t)(sprev % D);
-    uint32_t* gring = a.ring_t + (size_t)g * a.ring_slab;
-    const uint32_t E = bprev == 0 ? n0 : (bprev == 1 ? n1 : n2);
-    const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride + lbase;
-    // metadata of list entry e: first slot of its sub-row in this tile, length |
-    // ring slot of its spike step << 26, BF16 / FP8 row scale bits or (kTStdp)
-    // the source's last spike before the window
-    auto meta = [&](uint32_t ent, uint32_t& beg, uint32_t& lm, uint32_t& aux) {
-        const uint32_t src = ent & kIdMask;
-        const size_t ob = obase + (size_t)(src - (uint32_t)slo) * G1;
-        uint32_t o0, o1;
-        if (a.offs16) { o0 = a.offs16[ob]; o1 = a.offs16[ob + 1]; }
-        else { o0 = a.offs32[ob]; o1 = a.offs32[ob + 1]; }
-        const int32_t sD = (int32_t)((sprev + (int64_t)(ent >> 26)) % D);
-        // padded sub-rows: starts are multiples of kSubAlign, the low bits of the next
-        // entry hold the padding
-        constexpr uint32_t am = (uint32_t)kSubAlign - 1u;
-        beg = (uint32_t)a.prow[src] + (o0 & ~am);
-        lm = ((o1 & ~am) - (o0 & ~am) - (o1 & am)) | ((uint32_t)sD << 26);
-        if (WK == kTFP8 || WK == kTBF16) aux = __float_as_uint(a.row_scale[src]);
-        else if (kPl) aux = (uint32_t)lastA[src];
-        else aux = 0u;
-    };
-    // the first batch's list entries are loaded before their count is known (the
-    // index clamped into the segment's part of the list, which holds nq * window
-    // entries), and their metadata loads before the ring copy, so that these
-    // latencies overlap; entries past the count are never used
-    uint32_t pbeg = 0u, plm = 0u, paux = 0u;
-    const uint32_t ent0 = bprev == 0 ? l0 : (bprev == 1 ? l1 : l2);
-    if ((uint32_t)tid < min((uint32_t)bd, E)) meta(ent0, pbeg, plm, paux);
-    if (full) ring_slice_in(ring, gring, D, tn, rprev, nprev, RW);
+    const int64_t sbase = ctl->base_step + off;
+    const int64_t wbase = ctl->base_window + wk;
+    const int32_t nw = PW ? nwin : 1;
     // a tile of at most one neuron per thread: the neuron's state and out-degree are
-    // loaded now (after the ring copy, which they would delay), their latency
-    // hidden by the propagation phase
+    // loaded at the start of each window (after the ring copy, which they would
+    // delay), their latency hidden by the propagation phase
     const bool pre = tn <= bd && full && !(kPl && CH >= 16);  // (not where registers would spill)
-    NState pst;
-    int ptype = T_RAW;
-    int32_t pdeg = 0;
-    if (pre && tid < tn) {
-        ptype = (int)a.type[t0 + tid];
-        neuron_load(a, t0 + tid, ptype, kCls, pst, lastB);
-        if (ptype != T_RAW && ptype != T_LIF0) pdeg = (int32_t)(a.row_ptr[t0 + tid + 1] - a.row_ptr[t0 + tid]);
-    }
-    __syncthreads();
-
-    // 2. previous window's spikes into the tile
-    for (uint32_t b0 = 0; b0 < E; b0 += (uint32_t)bd) {
-        const uint32_t nb = min((uint32_t)bd, E - b0);
-        uint32_t items = 0u;
-        if ((uint32_t)tid < nb) {
-            uint32_t beg = pbeg, lm = plm, aux = paux;
-            if (b0 > 0) meta(L[b0 + (uint32_t)tid], beg, lm, aux);
-            sp_beg[tid] = beg;
-            sp_len[tid] = lm;
-            if (kAux) sp_aux[tid] = aux;
-            items = ((lm & kIdMask) + CH - 1) / CH;
+    uint32_t* gring = a.ring_t + (size_t)g * a.ring_slab;
+    for (int32_t kw = 0; kw < nw; ++kw) {
+        // (persistent form: the thread index is opaque to the compiler in each window,
+        // so that the per-thread addresses derived from it are formed inside the window
+        // instead of being hoisted out of the loop and held in registers for all of it)
+        int32_t tid = tid0;
+        if constexpr (PW) asm volatile("" : "+v"(tid));
+        const int64_t s0 = sbase + (int64_t)kw * nsteps;
+        const int64_t kwin = wbase + kw;
+        const int bcur = (int)(kwin % 3), bprev = (int)((kwin + 2) % 3), bnext = (int)((kwin + 1) % 3);
+        // The previous window's list: its start step, spike count and this thread's
+        // first entry.  In a launch's first window the window's index comes from ctl,
+        // and the words of all three lists load with it (one round trip instead of
+        // two, the index then picks one; the other lists' words, which other
+        // workgroups may be writing, are not used).  Later windows of a persistent
+        // launch know the index and read the list other workgroups just wrote.
+        const uint32_t li = min((uint32_t)tid, (uint32_t)nq * (uint32_t)a.window - 1u);
+        const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride + lbase;
+        int64_t sprev;
+        uint32_t E, ent0;
+        if (kw == 0) {
+            const size_t ws = a.win_stride;
+            const int64_t tb0 = ctl->tbase[0], tb1 = ctl->tbase[1], tb2 = ctl->tbase[2];
+            const uint32_t n0 = tcnt[0], n1 = tcnt[nseg], n2 = tcnt[2 * nseg];
+            const uint32_t l0 = a.win_buf[lbase + li], l1 = a.win_buf[ws + lbase + li];
+            const uint32_t l2 = a.win_buf[2 * ws + lbase + li];
+            sprev = bprev == 0 ? tb0 : (bprev == 1 ? tb1 : tb2);
+            E = bprev == 0 ? n0 : (bprev == 1 ? n1 : n2);
+            ent0 = bprev == 0 ? l0 : (bprev == 1 ? l1 : l2);
+        } else {
+            sprev = s0 - nsteps;
+            E = ld_agent(&tcnt[bprev * nseg]);
+            ent0 = ld_agent(&L[li]);
         }
-        const uint32_t incl = block_incl_scan(items, wsum);
-        const uint32_t total = wsum[(blockDim.x >> 6) - 1];
-        if ((uint32_t)tid < nb) sp_start[tid] = incl - items;
-        for (uint32_t r0 = 0; r0 < total; r0 += kItems) {
-            const uint32_t rn = min(kItems, total - r0);
-            // item table: each spike writes its own work items of this round
+        if (full && lt == 0 && tid == 0) {
+            if (PW) st_agent(&tcnt[bnext * nseg], 0u);
+            else tcnt[bnext * nseg] = 0u;
+        }
+        if (kPl && full && g == 0 && tid == 0) a.fcnt[(kwin + 1) & 1] = 0u;
+        if (full && g == 0 && tid == 0) ctl->tbase[bcur] = s0;
+        if (tid == 0) {
+            se_blk = 0ull;
+            lsp_n = 0u;
+        }
+        const int32_t* lastA = kPl ? a.last3 + (size_t)bprev * a.last_stride : nullptr;   // window k-1 start
+        const int32_t* lastB = kPl ? a.last3 + (size_t)bcur * a.last_stride : a.last;     // window k-1 end
+        int32_t* lastC = kPl ? a.last3 + (size_t)bnext * a.last_stride : a.last;          // window k end
+        if (kPl) {
+            for (int32_t j = tid; j < tn; j += bd) {
+                cL[j] = a.cnt[t0 + j];
+                tfL[j] = a.tf[t0 + j];
+            }
+            const int32_t* hg = a.hist + (size_t)t0 * kStdpHist;
+            for (int32_t k = tid; k < tn * kStdpHist; k += bd) hL[k] = hg[k];
+            for (int32_t k = tid; k < a.lut_lds; k += bd) {
+                lutL[k] = a.ltp[k];
+                lutL[a.lut_lds + k] = a.ltd[k];
+            }
+        }
+        if (kCls && kw == 0 && tid < 4 * kClassCache) {
+            // (precomputed per tile by k_tile_ccache: one load instead of a chain of three)
+            ccache[tid / (2 * kClassCache)][(tid / kClassCache) & 1][tid % kClassCache] =
+                a.tile_cc[(size_t)g * 4 * kClassCache + tid];
+            if (tid < 2) ccls[tid] = a.tile_ccls[2 * g + tid];
+        }
+        // 1. ring slice -> LDS.  The slots the previous window consumed ([sprev, s0)
+        // mod D) were cleared by it and not stored back (step 4): they are zero here.
+        // 0 before the first window; at most D (a network without synapses has D = 1
+        // and windows of up to kMaxWindow steps)
+        const int32_t nprev = (int32_t)min(s0 - sprev, (int64_t)D);
+        const int32_t rprev = (int32_t)(sprev % D);
+        // metadata of list entry e: first slot of its sub-row in this tile, length |
+        // ring slot of its spike step << 26, BF16 / FP8 row scale bits or (kTStdp)
+        // the source's last spike before the window
+        auto meta = [&](uint32_t ent, uint32_t& beg, uint32_t& lm, uint32_t& aux) {
+            const uint32_t src = ent & kIdMask;
+            const size_t ob = obase + (size_t)(src - (uint32_t)slo) * G1;
+            uint32_t o0, o1;
+            if (a.offs16) { o0 = a.offs16[ob]; o1 = a.offs16[ob + 1]; }
+            else { o0 = a.offs32[ob]; o1 = a.offs32[ob + 1]; }
+            const int32_t sD = (int32_t)((sprev + (int64_t)(ent >> 26)) % D);
+            // padded sub-rows: starts are multiples of kSubAlign, the low bits of the next
+            // entry hold the padding
+            constexpr uint32_t am = (uint32_t)kSubAlign - 1u;
+            beg = (uint32_t)a.prow[src] + (o0 & ~am);
+            lm = ((o1 & ~am) - (o0 & ~am) - (o1 & am)) | ((uint32_t)sD << 26);
+            if (WK == kTFP8 || WK == kTBF16) aux = __float_as_uint(a.row_scale[src]);
+            else if (kPl) aux = (uint32_t)lastA[src];
+            else aux = 0u;
+        };
+        // the first batch's list entries are loaded before their count is known (the
+        // index clamped into the segment's part of the list, which holds nq * window
+        // entries), and their metadata loads before the ring copy, so that these
+        // latencies overlap; entries past the count are never used
+        uint32_t pbeg = 0u, plm = 0u, paux = 0u;
+        if ((uint32_t)tid < min((uint32_t)bd, E)) meta(ent0, pbeg, plm, paux);
+        if (full && kw == 0) ring_slice_in(ring, gring, D, tn, rprev, nprev, RW);
+        NState pst;
+        int ptype = T_RAW;
+        int32_t pdeg = 0;
+        // CLASS kinds: every neuron of the tile (one per thread) an LIFB neuron of one of
+        // the tile's two cached classes (precomputed per tile)
+        const bool tile_lean = kCls && pre && a.tile_lean && a.tile_lean[g] != 0;
+        if (pre && tid < tn) {
+            ptype = (int)a.type[t0 + tid];
+            neuron_load(a, t0 + tid, ptype, kCls, pst, lastB);
+            if (ptype != T_RAW && ptype != T_LIF0) pdeg = (int32_t)(a.row_ptr[t0 + tid + 1] - a.row_ptr[t0 + tid]);
+        }
+        __syncthreads();
+
+        // 2. previous window's spikes into the tile
+        for (uint32_t b0 = 0; b0 < E; b0 += (uint32_t)bd) {
+            const uint32_t nb = min((uint32_t)bd, E - b0);
+            uint32_t items = 0u;
             if ((uint32_t)tid < nb) {
-                const uint32_t q0 = max(incl - items, r0), q1 = min(incl, r0 + rn);
-                for (uint32_t q = q0; q < q1; ++q) itm[q - r0] = (uint16_t)tid;
+                uint32_t beg = pbeg, lm = plm, aux = paux;
+                if (b0 > 0) meta(PW ? ld_agent(&L[b0 + (uint32_t)tid]) : L[b0 + (uint32_t)tid], beg, lm, aux);
+                sp_beg[tid] = beg;
+                sp_len[tid] = lm;
+                if (kAux) sp_aux[tid] = aux;
+                items = ((lm & kIdMask) + CH - 1) / CH;
             }
-            __syncthreads();
-            // U items per thread in flight: U * CH synapse words loaded before the
-            // first LDS atomic
-            // (kTStdp: at most 8 synapses, whose STDP state then fits in the registers)
-            constexpr int U = CH >= 16 ? 1 : (kPl ? (CH >= 8 ? 1 : (CH == 1 ? 4 : 8 / CH)) : 16 / CH);
-            for (uint32_t it = (uint32_t)tid; it < rn; it += (uint32_t)(U * bd)) {
-                // branch-free body, so that all U * CH loads are in flight before the
-                // first wait: lanes past the round's items re-read the last item and
-                // elements past an item's end re-read its last element; both add 0
-                uint32_t cnt[U], kb[U];
-                int32_t sD[U], t2[U], la[U];
-                float rs[U];
-#pragma unroll
-                for (int h = 0; h < U; ++h) {
-                    const uint32_t q0 = it + (uint32_t)(h * bd);
-                    const uint32_t q = min(q0, rn - 1u);
-                    const uint32_t j = itm[q];
-                    const uint32_t c = r0 + q - sp_start[j];
-                    const uint32_t meta = sp_len[j];
-                    const uint32_t len = meta & kIdMask;
-                    sD[h] = (int32_t)(meta >> 26);
-                    kb[h] = sp_beg[j] + c * CH;
-                    cnt[h] = (q0 < rn) ? min((uint32_t)CH, len - c * CH) : 0u;
-                    rs[h] = (WK == kTFP8 || WK == kTBF16) ? __uint_as_float(sp_aux[j]) : 1.0f;
-                    if (kPl) {
-                        // the spike's step: the window is shorter than the ring, so its
-                        // slot determines it
-                        const int32_t dd = sD[h] - rprev;
-                        t2[h] = (int32_t)sprev + dd + (dd < 0 ? D : 0);
-                        la[h] = (int32_t)sp_aux[j];
-                    }
+            const uint32_t incl = block_incl_scan(items, wsum);
+            const uint32_t total = wsum[(blockDim.x >> 6) - 1];
+            if ((uint32_t)tid < nb) sp_start[tid] = incl - items;
+            for (uint32_t r0 = 0; r0 < total; r0 += kItems) {
+                const uint32_t rn = min(kItems, total - r0);
+                // item table: each spike writes its own work items of this round
+                if ((uint32_t)tid < nb) {
+                    const uint32_t q0 = max(incl - items, r0), q1 = min(incl, r0 + rn);
+                    for (uint32_t q = q0; q < q1; ++q) itm[q - r0] = (uint16_t)tid;
                 }
-                uint32_t x[U][CH];
-                float wv[U][CH];
-#pragma unroll
-                for (int h = 0; h < U; ++h) {
-                    if constexpr (WK == kTClass16 && CH >= 2) {
-                        // the item's CH words as aligned vector loads (sub-rows start at
-                        // multiples of kSubAlign words; the words past the sub-row's end
-                        // are padding, the next sub-row or the slack after the last one)
-                        const uint16_t* p16 = a.syn16 + kb[h];
-                        if constexpr (CH == 2) {
-                            const uint32_t v = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p16));
-                            x[h][0] = v & 0xFFFFu;
-                            x[h][1] = v >> 16;
-                        } else if constexpr (CH == 4) {
-                            const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p16));
-                            x[h][0] = v.x & 0xFFFFu;
-                            x[h][1] = v.x >> 16;
-                            x[h][2] = v.y & 0xFFFFu;
-                            x[h][3] = v.y >> 16;
-                        } else {
-#pragma unroll
-                            for (int v8 = 0; v8 < CH / 8; ++v8) {
-                                const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p16) + v8);
-                                x[h][8 * v8 + 0] = v.x & 0xFFFFu;
-                                x[h][8 * v8 + 1] = v.x >> 16;
-                                x[h][8 * v8 + 2] = v.y & 0xFFFFu;
-                                x[h][8 * v8 + 3] = v.y >> 16;
-                                x[h][8 * v8 + 4] = v.z & 0xFFFFu;
-                                x[h][8 * v8 + 5] = v.z >> 16;
-                                x[h][8 * v8 + 6] = v.w & 0xFFFFu;
-                                x[h][8 * v8 + 7] = v.w >> 16;
-                            }
+                __syncthreads();
+                // U items per thread in flight: U * CH synapse words loaded before the
+                // first LDS atomic
+                // (kTStdp: at most 8 synapses, whose STDP state then fits in the registers)
+                constexpr int U = CH >= 16 ? 1 : (kPl ? (CH >= 8 ? 1 : (CH == 1 ? 4 : 8 / CH)) : 16 / CH);
+                for (uint32_t it = (uint32_t)tid; it < rn; it += (uint32_t)(U * bd)) {
+                    // branch-free body, so that all U * CH loads are in flight before the
+                    // first wait: lanes past the round's items re-read the last item and
+                    // elements past an item's end re-read its last element; both add 0
+                    uint32_t cnt[U], kb[U];
+                    int32_t sD[U], t2[U], la[U];
+                    float rs[U];
+    #pragma unroll
+                    for (int h = 0; h < U; ++h) {
+                        const uint32_t q0 = it + (uint32_t)(h * bd);
+                        const uint32_t q = min(q0, rn - 1u);
+                        const uint32_t j = itm[q];
+                        const uint32_t c = r0 + q - sp_start[j];
+                        const uint32_t meta = sp_len[j];
+                        const uint32_t len = meta & kIdMask;
+                        sD[h] = (int32_t)(meta >> 26);
+                        kb[h] = sp_beg[j] + c * CH;
+                        cnt[h] = (q0 < rn) ? min((uint32_t)CH, len - c * CH) : 0u;
+                        rs[h] = (WK == kTFP8 || WK == kTBF16) ? __uint_as_float(sp_aux[j]) : 1.0f;
+                        if (kPl) {
+                            // the spike's step: the window is shorter than the ring, so its
+                            // slot determines it
+                            const int32_t dd = sD[h] - rprev;
+                            t2[h] = (int32_t)sprev + dd + (dd < 0 ? D : 0);
+                            la[h] = (int32_t)sp_aux[j];
                         }
-                        continue;
                     }
-                    if constexpr (WK != kTClass16 && CH >= 2) {
-                        // 32-bit words and FP32 weights: aligned vector loads likewise
-                        // (kTStdp rewrites its weights below: cached loads)
-                        const uint32_t* p32 = a.syn_t + kb[h];
-                        const u32x2* f2 = reinterpret_cast<const u32x2*>(a.w_t + kb[h]);
-                        const u32x4* f4 = reinterpret_cast<const u32x4*>(a.w_t + kb[h]);
-                        if constexpr (CH == 2) {
-                            const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p32));
-                            x[h][0] = v.x;
-                            x[h][1] = v.y;
-                            if (WK == kTF32 || kPl) {
-                                const u32x2 f = kPl ? *f2 : __builtin_nontemporal_load(f2);
-                                wv[h][0] = __uint_as_float(f.x);
-                                wv[h][1] = __uint_as_float(f.y);
+                    uint32_t x[U][CH];
+                    float wv[U][CH];
+    #pragma unroll
+                    for (int h = 0; h < U; ++h) {
+                        if constexpr (WK == kTClass16 && CH >= 2) {
+                            // the item's CH words as aligned vector loads (sub-rows start at
+                            // multiples of kSubAlign words; the words past the sub-row's end
+                            // are padding, the next sub-row or the slack after the last one)
+                            const uint16_t* p16 = a.syn16 + kb[h];
+                            if constexpr (CH == 2) {
+                                const uint32_t v = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p16));
+                                x[h][0] = v & 0xFFFFu;
+                                x[h][1] = v >> 16;
+                            } else if constexpr (CH == 4) {
+                                const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p16));
+                                x[h][0] = v.x & 0xFFFFu;
+                                x[h][1] = v.x >> 16;
+                                x[h][2] = v.y & 0xFFFFu;
+                                x[h][3] = v.y >> 16;
+                            } else {
+    #pragma unroll
+                                for (int v8 = 0; v8 < CH / 8; ++v8) {
+                                    const u32x4 v =
+                                        __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p16) + v8);
+                                    x[h][8 * v8 + 0] = v.x & 0xFFFFu;
+                                    x[h][8 * v8 + 1] = v.x >> 16;
+                                    x[h][8 * v8 + 2] = v.y & 0xFFFFu;
+                                    x[h][8 * v8 + 3] = v.y >> 16;
+                                    x[h][8 * v8 + 4] = v.z & 0xFFFFu;
+                                    x[h][8 * v8 + 5] = v.z >> 16;
+                                    x[h][8 * v8 + 6] = v.w & 0xFFFFu;
+                                    x[h][8 * v8 + 7] = v.w >> 16;
+                                }
                             }
-                        } else {
-#pragma unroll
-                            for (int v4 = 0; v4 < CH / 4; ++v4) {
-                                const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p32) + v4);
-                                x[h][4 * v4 + 0] = v.x;
-                                x[h][4 * v4 + 1] = v.y;
-                                x[h][4 * v4 + 2] = v.z;
-                                x[h][4 * v4 + 3] = v.w;
+                            continue;
+                        }
+                        if constexpr (WK != kTClass16 && CH >= 2) {
+                            // 32-bit words and FP32 weights: aligned vector loads likewise
+                            // (kTStdp rewrites its weights below: cached loads)
+                            const uint32_t* p32 = a.syn_t + kb[h];
+                            const u32x2* f2 = reinterpret_cast<const u32x2*>(a.w_t + kb[h]);
+                            const u32x4* f4 = reinterpret_cast<const u32x4*>(a.w_t + kb[h]);
+                            if constexpr (CH == 2) {
+                                const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p32));
+                                x[h][0] = v.x;
+                                x[h][1] = v.y;
                                 if (WK == kTF32 || kPl) {
-                                    const u32x4 f = kPl ? f4[v4] : __builtin_nontemporal_load(f4 + v4);
-                                    wv[h][4 * v4 + 0] = __uint_as_float(f.x);
-                                    wv[h][4 * v4 + 1] = __uint_as_float(f.y);
-                                    wv[h][4 * v4 + 2] = __uint_as_float(f.z);
-                                    wv[h][4 * v4 + 3] = __uint_as_float(f.w);
+                                    const u32x2 f = kPl ? *f2 : __builtin_nontemporal_load(f2);
+                                    wv[h][0] = __uint_as_float(f.x);
+                                    wv[h][1] = __uint_as_float(f.y);
                                 }
-                            }
-                        }
-                        continue;
-                    }
-                    const uint32_t last = max(cnt[h], 1u) - 1u;
-#pragma unroll
-                    for (int u = 0; u < CH; ++u) {
-                        const uint32_t k = kb[h] + min((uint32_t)u, last);
-                        if (WK == kTClass16) x[h][u] = __builtin_nontemporal_load(&a.syn16[k]);
-                        else x[h][u] = __builtin_nontemporal_load(&a.syn_t[k]);
-                        if (WK == kTF32) wv[h][u] = __builtin_nontemporal_load(&a.w_t[k]);
-                        else if (kPl) wv[h][u] = a.w_t[k];           // rewritten below: cached load
-                    }
-                }
-                // kTStdp, per plastic element: j's history newest first gives its last spike
-                // at or before t2 (for the axon update) and the range [mlo, mhi] of
-                // deferred dendrite updates in (lo, t2); the axon table loads of all
-                // elements are then issued together
-                int32_t rng[U][CH];      // mlo | mhi << 8 (mhi < mlo: none); -1: not plastic
-                int32_t adt[U][CH];      // dt of the axon update
-                float ax[U][CH];
-                if (kPl) {
-#pragma unroll
-                    for (int h = 0; h < U; ++h)
-#pragma unroll
-                        for (int u = 0; u < CH; ++u) {
-                            const uint32_t xw = x[h][u];
-                            const int32_t loc = (int32_t)(xw & 0x7FFu);
-                            int32_t lj = 0, mlo = kStdpHist, mhi = 0;
-                            rng[h][u] = -1;
-                            // (words past the item's end are padding or another tile's)
-                            if ((uint32_t)u < cnt[h] && ((xw >> 16) & 1u)) {
-                                const int32_t* hj = hL + loc * kStdpHist;
-                                const int32_t c = cL[loc];
-                                const int32_t lo = max(la[h] == 0 ? -1 : la[h], tfL[loc]);
-                                const int32_t nm = min(c, kStdpHist);
-                                bool found = false;
-                                for (int32_t m = 0; m < nm; ++m) {
-                                    const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
-                                    if (e > t2[h]) continue;           // j fired after t2 in that window
-                                    if (!found) {
-                                        lj = e;
-                                        found = true;
+                            } else {
+    #pragma unroll
+                                for (int v4 = 0; v4 < CH / 4; ++v4) {
+                                    const u32x4 v =
+                                        __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p32) + v4);
+                                    x[h][4 * v4 + 0] = v.x;
+                                    x[h][4 * v4 + 1] = v.y;
+                                    x[h][4 * v4 + 2] = v.z;
+                                    x[h][4 * v4 + 3] = v.w;
+                                    if (WK == kTF32 || kPl) {
+                                        const u32x4 f = kPl ? f4[v4] : __builtin_nontemporal_load(f4 + v4);
+                                        wv[h][4 * v4 + 0] = __uint_as_float(f.x);
+                                        wv[h][4 * v4 + 1] = __uint_as_float(f.y);
+                                        wv[h][4 * v4 + 2] = __uint_as_float(f.z);
+                                        wv[h][4 * v4 + 3] = __uint_as_float(f.w);
                                     }
-                                    if (e == t2[h]) continue;
-                                    if (e <= lo) break;
-                                    mlo = min(mlo, m);
-                                    mhi = m;
                                 }
-                                rng[h][u] = mlo | (mhi << 8);
                             }
-                            adt[h][u] = t2[h] - lj;
+                            continue;
                         }
-#pragma unroll
-                    for (int h = 0; h < U; ++h)
-#pragma unroll
-                        for (int u = 0; u < CH; ++u)
-                            ax[h][u] = rng[h][u] >= 0 ? dw(adt[h][u]) : 0.0f;
-                }
-#pragma unroll
-                for (int h = 0; h < U; ++h)
-#pragma unroll
-                    for (int u = 0; u < CH; ++u) {
-                        // words past the item's end (padding, another tile's words) and
-                        // lanes past the round's items: a distinct in-slice entry per
-                        // lane, adding 0
-                        const bool valid = (uint32_t)u < cnt[h];
-                        const uint32_t xw = x[h][u];
-                        if (WK == kTClass) {
-                            int32_t slot = sD[h] + (int32_t)((xw >> 26) & 31u) + 1;
-                            slot -= (slot >= D) ? D : 0;
-                            const int32_t loc = (int32_t)(xw & kIdMask) - t0;
-                            const int32_t at = valid ? slot * tn + loc : (tid % tn);
-                            atomicAdd(&ring[at], valid ? ((xw >> 31) ? 0x10000u : 1u) : 0u);
-                        } else if (WK == kTClass16) {
-                            int32_t slot = sD[h] + (int32_t)((xw >> 10) & 31u) + 1;
-                            slot -= (slot >= D) ? D : 0;
-                            const int32_t loc = (int32_t)(xw & 0x3FFu);
-                            const int32_t at = valid ? slot * tn + loc : (tid % tn);
-                            atomicAdd(&ring[at], valid ? ((xw >> 15) ? 0x10000u : 1u) : 0u);
-                        } else {
-                            int32_t slot = sD[h] + (int32_t)((xw >> 11) & 31u) + 1;
-                            slot -= (slot >= D) ? D : 0;
-                            const int32_t loc = (int32_t)(xw & 0x7FFu);
-                            float w;
-                            if (WK == kTBF16) w = bf16_to_f32((uint16_t)(xw >> 16)) * rs[h];
-                            else if (WK == kTFP8) w = fp8e4m3_to_f32((uint8_t)(xw >> 16)) * rs[h];
-                            else w = wv[h][u];
-                            const bool pl = kPl && rng[h][u] >= 0;
-                            if (pl) {
-                                // deferred dendrite updates before t2, oldest first
-                                const int32_t* hj = hL + loc * kStdpHist;
-                                const int32_t c = cL[loc];
-                                const int32_t mlo = rng[h][u] & 0xFF, mhi = rng[h][u] >> 8;
-                                for (int32_t m = mhi; m >= mlo; --m) {
-                                    const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
-                                    w = clipw(w + dw(la[h] - e), a.w_max, a.w_min);
-                                }
-                            }
-                            const bool exc = w > 0.f;
-                            const int32_t v = (int32_t)rintf(w * fsc.x);
-                            // (one ring: both signs into one fixed-point sum)
-                            const int32_t at = valid ? ((exc || RW == 1 ? 0 : D) + slot) * tn + loc : (tid % tn);
-                            if (full) atomicAdd(reinterpret_cast<int32_t*>(ring) + at, valid ? v : 0);
-                            if (pl && valid)                           // axon update of t2
-                                a.w_t[kb[h] + (uint32_t)u] = clipw(w + ax[h][u], a.w_max, a.w_min);
+                        const uint32_t last = max(cnt[h], 1u) - 1u;
+    #pragma unroll
+                        for (int u = 0; u < CH; ++u) {
+                            const uint32_t k = kb[h] + min((uint32_t)u, last);
+                            if (WK == kTClass16) x[h][u] = __builtin_nontemporal_load(&a.syn16[k]);
+                            else x[h][u] = __builtin_nontemporal_load(&a.syn_t[k]);
+                            if (WK == kTF32) wv[h][u] = __builtin_nontemporal_load(&a.w_t[k]);
+                            else if (kPl) wv[h][u] = a.w_t[k];           // rewritten below: cached load
                         }
                     }
+                    // kTStdp, per plastic element: j's history newest first gives its last spike
+                    // at or before t2 (for the axon update) and the range [mlo, mhi] of
+                    // deferred dendrite updates in (lo, t2); the axon table loads of all
+                    // elements are then issued together
+                    int32_t rng[U][CH];      // mlo | mhi << 8 (mhi < mlo: none); -1: not plastic
+                    int32_t adt[U][CH];      // dt of the axon update
+                    float ax[U][CH];
+                    if (kPl) {
+    #pragma unroll
+                        for (int h = 0; h < U; ++h)
+    #pragma unroll
+                            for (int u = 0; u < CH; ++u) {
+                                const uint32_t xw = x[h][u];
+                                const int32_t loc = (int32_t)(xw & 0x7FFu);
+                                int32_t lj = 0, mlo = kStdpHist, mhi = 0;
+                                rng[h][u] = -1;
+                                // (words past the item's end are padding or another tile's)
+                                if ((uint32_t)u < cnt[h] && ((xw >> 16) & 1u)) {
+                                    const int32_t* hj = hL + loc * kStdpHist;
+                                    const int32_t c = cL[loc];
+                                    const int32_t lo = max(la[h] == 0 ? -1 : la[h], tfL[loc]);
+                                    const int32_t nm = min(c, kStdpHist);
+                                    bool found = false;
+                                    for (int32_t m = 0; m < nm; ++m) {
+                                        const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
+                                        if (e > t2[h]) continue;           // j fired after t2 in that window
+                                        if (!found) {
+                                            lj = e;
+                                            found = true;
+                                        }
+                                        if (e == t2[h]) continue;
+                                        if (e <= lo) break;
+                                        mlo = min(mlo, m);
+                                        mhi = m;
+                                    }
+                                    rng[h][u] = mlo | (mhi << 8);
+                                }
+                                adt[h][u] = t2[h] - lj;
+                            }
+    #pragma unroll
+                        for (int h = 0; h < U; ++h)
+    #pragma unroll
+                            for (int u = 0; u < CH; ++u)
+                                ax[h][u] = rng[h][u] >= 0 ? dw(adt[h][u]) : 0.0f;
+                    }
+    #pragma unroll
+                    for (int h = 0; h < U; ++h)
+    #pragma unroll
+                        for (int u = 0; u < CH; ++u) {
+                            // words past the item's end (padding, another tile's words) and
+                            // lanes past the round's items: a distinct in-slice entry per
+                            // lane, adding 0
+                            const bool valid = (uint32_t)u < cnt[h];
+                            const uint32_t xw = x[h][u];
+                            if (WK == kTClass) {
+                                int32_t slot = sD[h] + (int32_t)((xw >> 26) & 31u) + 1;
+                                slot -= (slot >= D) ? D : 0;
+                                const int32_t loc = (int32_t)(xw & kIdMask) - t0;
+                                const int32_t at = valid ? slot * tn + loc : (tid % tn);
+                                atomicAdd(&ring[at], valid ? ((xw >> 31) ? 0x10000u : 1u) : 0u);
+                            } else if (WK == kTClass16) {
+                                int32_t slot = sD[h] + (int32_t)((xw >> 10) & 31u) + 1;
+                                slot -= (slot >= D) ? D : 0;
+                                const int32_t loc = (int32_t)(xw & 0x3FFu);
+                                const int32_t at = valid ? slot * tn + loc : (tid % tn);
+                                atomicAdd(&ring[at], valid ? ((xw >> 15) ? 0x10000u : 1u) : 0u);
+                            } else {
+                                int32_t slot = sD[h] + (int32_t)((xw >> 11) & 31u) + 1;
+                                slot -= (slot >= D) ? D : 0;
+                                const int32_t loc = (int32_t)(xw & 0x7FFu);
+                                float w;
+                                if (WK == kTBF16) w = bf16_to_f32((uint16_t)(xw >> 16)) * rs[h];
+                                else if (WK == kTFP8) w = fp8e4m3_to_f32((uint8_t)(xw >> 16)) * rs[h];
+                                else w = wv[h][u];
+                                const bool pl = kPl && rng[h][u] >= 0;
+                                if (pl) {
+                                    // deferred dendrite updates before t2, oldest first
+                                    const int32_t* hj = hL + loc * kStdpHist;
+                                    const int32_t c = cL[loc];
+                                    const int32_t mlo = rng[h][u] & 0xFF, mhi = rng[h][u] >> 8;
+                                    for (int32_t m = mhi; m >= mlo; --m) {
+                                        const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
+                                        w = clipw(w + dw(la[h] - e), a.w_max, a.w_min);
+                                    }
+                                }
+                                const bool exc = w > 0.f;
+                                const int32_t v = (int32_t)rintf(w * fsc.x);
+                                // (one ring: both signs into one fixed-point sum)
+                                const int32_t at = valid ? ((exc || RW == 1 ? 0 : D) + slot) * tn + loc : (tid % tn);
+                                if (full) atomicAdd(reinterpret_cast<int32_t*>(ring) + at, valid ? v : 0);
+                                if (pl && valid)                           // axon update of t2
+                                    a.w_t[kb[h] + (uint32_t)u] = clipw(w + ax[h][u], a.w_max, a.w_min);
+                            }
+                        }
+                }
+                __syncthreads();
             }
-            __syncthreads();
+            __syncthreads();   // wsum / sp_* are rewritten by the next batch
         }
-        __syncthreads();   // wsum / sp_* are rewritten by the next batch
-    }
-    if (kPl) {
-        // 2b. neurons that fired kStdpHist - 1 times since tf (for apply_only: all that
-        // fired since tf) go to this window's flush list (k_stdp_flush, launched next)
-        uint32_t* fc = a.fcnt + (kwin & 1);
-        int32_t* fli = a.flist + (size_t)(kwin & 1) * a.flist_cap * 3;
-        for (int32_t j = tid; j < tn; j += bd) {
-            const int32_t n_since = cL[j] - a.cntf[t0 + j];
-            if (full ? n_since >= kStdpHist - 1 : n_since > 0) {
-                const uint32_t e = atomicAdd(fc, 1u);
-                fli[3 * e] = t0 + j;
-                fli[3 * e + 1] = cL[j];
-                fli[3 * e + 2] = tfL[j];
-                if (full) {
-                    a.tf[t0 + j] = (int32_t)(s0 - 1);
-                    a.cntf[t0 + j] = cL[j];
+        if (kPl) {
+            // 2b. neurons that fired kStdpHist - 1 times since tf (for apply_only: all that
+            // fired since tf) go to this window's flush list (k_stdp_flush, launched next)
+            uint32_t* fc = a.fcnt + (kwin & 1);
+            int32_t* fli = a.flist + (size_t)(kwin & 1) * a.flist_cap * 3;
+            for (int32_t j = tid; j < tn; j += bd) {
+                const int32_t n_since = cL[j] - a.cntf[t0 + j];
+                if (full ? n_since >= kStdpHist - 1 : n_since > 0) {
+                    const uint32_t e = atomicAdd(fc, 1u);
+                    fli[3 * e] = t0 + j;
+                    fli[3 * e + 1] = cL[j];
+                    fli[3 * e + 2] = tfL[j];
+                    if (full) {
+                        a.tf[t0 + j] = (int32_t)(s0 - 1);
+                        a.cntf[t0 + j] = cL[j];
+                    }
                 }
             }
+            if (!full) return;
         }
-        if (!full) return;
-    }
 
-    // 3. neurons of the tile through this window's steps.  Spikes go to an LDS
-    // list (the staging area of step 2, free now; one LDS atomic per wave and
-    // step) that joins the segment's window list with one global atomic at the
-    // end; lanes that find it full append to HBM directly.
-    uint32_t* lsp = sp_beg;
-    uint32_t* wbuf = a.win_buf + (size_t)bcur * a.win_stride + lbase;
-    const int32_t slot0 = (int32_t)(s0 % D);
-    unsigned long long se_local = 0ull;
-    for (int32_t j0 = 0; j0 < tn; j0 += bd) {
-        const int32_t j = j0 + tid;
-        const bool active = j < tn;
-        const int32_t i = t0 + j;
-        const int type = active ? (pre ? ptype : (int)a.type[i]) : T_RAW;
-        NState st;
-        int32_t outdeg = 0;
-        if (pre) {
-            st = pst;
-            outdeg = pdeg;
-        } else if (active) {
-            neuron_load(a, i, type, kCls, st, lastB);
-            if (type != T_RAW && type != T_LIF0) outdeg = (int32_t)(a.row_ptr[i + 1] - a.row_ptr[i]);
-        }
-        const bool integ = integrates(type);
-        // wave-uniform fast path: every active lane an LIFB neuron of a cached class
-        int32_t csl = -1;
-        if (kCls && active && integ) csl = (st.cls == ccls[0]) ? 0 : ((st.cls == ccls[1]) ? 1 : -1);
-        const bool lean = kCls && __all(!active || (type == T_LIFB && csl >= 0));
-        int32_t slot = slot0;
-        for (int32_t w = 0; w < nsteps; ++w) {
-            const int32_t s = (int32_t)(s0 + w);
-            bool fired = false;
-            // class sums of a cached class: one LDS read per side (the table for
-            // counts past the cache)
-            auto cached_input = [&](uint32_t x, float& be, float& bi) {
-                const uint32_t ke = x & 0xFFFFu, ki = x >> 16;
-                const float* cc = &ccache[csl][0][0];
-                be = cc[min(ke, (uint32_t)kClassCache - 1u)];
-                bi = cc[kClassCache + min(ki, (uint32_t)kClassCache - 1u)];
-                if (ke >= (uint32_t)kClassCache) be = a.seq[st.off_e + ke];
-                if (ki >= (uint32_t)kClassCache) bi = a.seq[st.off_i + ki];
-            };
-            if (lean) {
-                if (active) {
-                    uint32_t* r = &ring[slot * tn + j];
-                    const uint32_t x = *r;
-                    *r = 0u;                                          // clearInput
-                    float be, bi;
-                    cached_input(x, be, bi);
-                    fired = integrate_step(T_LIFB, st, s, be, bi);
-                }
-            } else if (type == T_POISSON) {
-                fired = poisson_step(st, s, w);
-            } else if (integ) {
-                float be, bi;
-                uint32_t* r = &ring[slot * tn + j];
-                if (kCls) {
-                    const uint32_t x = *r;
-                    *r = 0u;                                          // clearInput
-                    if (csl >= 0) cached_input(x, be, bi);
-                    else class_input(x, a.seq, st, be, bi);
-                } else {
-                    uint32_t* ri = r + (size_t)D * tn;
-                    const int32_t qe = (int32_t)*r, qi = RW == 2 ? (int32_t)*ri : 0;
-                    *r = 0u;
-                    if (RW == 2) *ri = 0u;
-                    fixed_input(type, qe, qi, fsc.y, be, bi);
-                }
-                fired = integrate_step(type, st, s, be, bi);
+        // 3. neurons of the tile through this window's steps.  Spikes go to an LDS
+        // list (the staging area of step 2, free now; one LDS atomic per wave and
+        // step) that joins the segment's window list with one global atomic at the
+        // end; lanes that find it full append to HBM directly.
+        uint32_t* lsp = sp_beg;
+        uint32_t* wbuf = a.win_buf + (size_t)bcur * a.win_stride + lbase;
+        const int32_t slot0 = (int32_t)(s0 % D);
+        unsigned long long se_local = 0ull;
+        for (int32_t j0 = 0; j0 < tn; j0 += bd) {
+            // a wave without neurons here skips the steps (its instructions would still
+            // take the SIMD's cycles)
+            if (j0 + (tid & ~63) >= tn) continue;
+            const int32_t j = j0 + tid;
+            const bool active = j < tn;
+            const int32_t i = t0 + j;
+            const int type = active ? (pre ? ptype : (int)a.type[i]) : T_RAW;
+            NState st;
+            int32_t outdeg = 0;
+            if (pre) {
+                st = pst;
+                outdeg = pdeg;
+            } else if (active) {
+                neuron_load(a, i, type, kCls, st, lastB);
+                if (type != T_RAW && type != T_LIF0) outdeg = (int32_t)(a.row_ptr[i + 1] - a.row_ptr[i]);
             }
-            if (kPl && fired) a.hist[(size_t)i * kStdpHist + ((st.cnt - 1) & (kStdpHist - 1))] = s;
-            const uint64_t m = __ballot(fired);
-            if (m != 0ull) {
+            const bool integ = integrates(type);
+            // wave-uniform fast path: every active lane an LIFB neuron of a cached class
+            int32_t csl = -1;
+            if (kCls && active && integ) csl = (st.cls == ccls[0]) ? 0 : ((st.cls == ccls[1]) ? 1 : -1);
+            // spikes of step w: one wave ballot; the wave's leader reserves their places in
+            // the LDS list (one LDS atomic per wave and step)
+            auto append = [&](bool fired, int32_t w, int32_t s) {
+                const uint64_t m = __ballot(fired);
+                if (m == 0ull) return;
                 const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
                 const uint32_t pc = (uint32_t)__popcll(m);
                 uint32_t lb = 0u;
@@ -1059,7 +1113,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                     rb = __shfl(rb, (int)leader);
                     const uint32_t o = lb + rank_below(m) - over0;
                     if (fired && lb + rank_below(m) >= cap) {
-                        wbuf[gb + o] = (uint32_t)i | ((uint32_t)w << 26);
+                        if (PW) st_agent(&wbuf[gb + o], (uint32_t)i | ((uint32_t)w << 26));
+                        else wbuf[gb + o] = (uint32_t)i | ((uint32_t)w << 26);
                         if (a.rec) {
                             const uint64_t at = rb + o;
                             if (at < a.rec_cap) a.rec[at] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)i;
@@ -1072,36 +1127,141 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                     if (r < cap) lsp[r] = (uint32_t)i | ((uint32_t)w << 26);
                     se_local += (unsigned long long)outdeg;
                 }
+            };
+            if (tile_lean) {
+                // every neuron an LIFB neuron of a cached class (tile_lean): the steps
+                // (mgcore.cu:164-196) as selects, the next step's ring word loaded ahead,
+                // spikes kept as a bit per step; the consumed rows are cleared and the
+                // spikes appended after the steps
+                const float* cc = &ccache[csl < 0 ? 0 : csl][0][0];
+                int32_t slot = slot0;
+                uint32_t x = active ? ring[slot * tn + j] : 0u;
+                uint64_t fmask = 0ull;
+                for (int32_t w = 0; w < nsteps; ++w) {
+                    const uint32_t ke = x & 0xFFFFu, ki = x >> 16;
+                    float be = cc[min(ke, (uint32_t)kClassCache - 1u)];
+                    float bi = cc[kClassCache + min(ki, (uint32_t)kClassCache - 1u)];
+                    if (ke >= (uint32_t)kClassCache) be = a.seq[st.off_e + ke];
+                    if (ki >= (uint32_t)kClassCache) bi = a.seq[st.off_i + ki];
+                    slot = (slot + 1 == D) ? 0 : slot + 1;
+                    x = (active && w + 1 < nsteps) ? ring[slot * tn + j] : 0u;
+                    const bool ref = st.refr > 0;
+                    const bool fire = active && !ref && st.V >= st.c4;
+                    const float vn = st.V + fmaf(-0.005f, st.V, be + bi);
+                    st.V = fire ? st.c3 : (ref ? st.V : vn);
+                    st.refr = fire ? st.refr_steps : (ref ? st.refr - 1 : st.refr);
+                    fmask |= (uint64_t)fire << w;
+                }
+                if (fmask) {
+                    st.cnt += __popcll(fmask);
+                    st.last = (int32_t)(s0 + 63 - __clzll((long long)fmask));
+                }
+                for (uint64_t fm = fmask; __any(fm != 0ull);) {   // one step's spikes per lane and pass
+                    const bool f = fm != 0ull;
+                    const int32_t w = f ? __ffsll((unsigned long long)fm) - 1 : 0;
+                    fm &= fm - 1ull;
+                    append(f, w, (int32_t)(s0 + w));
+                }
+            } else {
+                const bool lean = kCls && __all(!active || (type == T_LIFB && csl >= 0));
+                int32_t slot = slot0;
+                for (int32_t w = 0; w < nsteps; ++w) {
+                    const int32_t s = (int32_t)(s0 + w);
+                    bool fired = false;
+                    // class sums of a cached class: one LDS read per side (the table for
+                    // counts past the cache)
+                    auto cached_input = [&](uint32_t x, float& be, float& bi) {
+                        const uint32_t ke = x & 0xFFFFu, ki = x >> 16;
+                        const float* cc = &ccache[csl][0][0];
+                        be = cc[min(ke, (uint32_t)kClassCache - 1u)];
+                        bi = cc[kClassCache + min(ki, (uint32_t)kClassCache - 1u)];
+                        if (ke >= (uint32_t)kClassCache) be = a.seq[st.off_e + ke];
+                        if (ki >= (uint32_t)kClassCache) bi = a.seq[st.off_i + ki];
+                    };
+                    if (lean) {
+                        if (active) {
+                            uint32_t* r = &ring[slot * tn + j];
+                            const uint32_t x = *r;
+                            *r = 0u;                                          // clearInput
+                            float be, bi;
+                            cached_input(x, be, bi);
+                            fired = integrate_lifb(st, s, be + bi);
+                        }
+                    } else if (type == T_POISSON) {
+                        fired = poisson_step(st, s, w);
+                    } else if (integ) {
+                        float be, bi;
+                        uint32_t* r = &ring[slot * tn + j];
+                        if (kCls) {
+                            const uint32_t x = *r;
+                            *r = 0u;                                          // clearInput
+                            if (csl >= 0) cached_input(x, be, bi);
+                            else class_input(x, a.seq, st, be, bi);
+                        } else {
+                            uint32_t* ri = r + (size_t)D * tn;
+                            const int32_t qe = (int32_t)*r, qi = RW == 2 ? (int32_t)*ri : 0;
+                            *r = 0u;
+                            if (RW == 2) *ri = 0u;
+                            fixed_input(type, qe, qi, fsc.y, be, bi);
+                        }
+                        fired = integrate_step(type, st, s, be, bi);
+                    }
+                    if (PW && active && !integ) {
+                        // the slice stays in LDS across windows: the consumed row of a neuron
+                        // that does not read its input is cleared here (a per-window launch
+                        // reloads consumed rows as zeros)
+                        ring[slot * tn + j] = 0u;
+                        if (!kCls && RW == 2) ring[((size_t)D + slot) * tn + j] = 0u;
+                    }
+                    if (kPl && fired) a.hist[(size_t)i * kStdpHist + ((st.cnt - 1) & (kStdpHist - 1))] = s;
+                    append(fired, w, s);
+                    slot = (slot + 1 == D) ? 0 : slot + 1;
+                }
             }
-            slot = (slot + 1 == D) ? 0 : slot + 1;
+            if (active) neuron_store(a, i, type, st, lastC);
         }
-        if (active) neuron_store(a, i, type, st, lastC);
-    }
-    if (se_local) atomicAdd(&se_blk, se_local);
-    __syncthreads();
-    const uint32_t nl = min(lsp_n, (uint32_t)kTiledBatch);
-    if (tid == 0) {
-        if (se_blk) atomicAdd((unsigned long long*)&ctl->se, se_blk);
-        if (nl) {
-            lsp_g = atomicAdd(&tcnt[bcur * nseg], nl);
-            lsp_r = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)nl);
+        if (se_local) atomicAdd(&se_blk, se_local);
+        __syncthreads();
+        if (PW && tile_lean && kw + 1 < nw) {
+            // the lean steps leave the rows they consumed for here, after every wave's
+            // last read; the slice stays in LDS for the next window
+            const int32_t nr = min(nsteps, D);
+            int32_t sl = slot0;
+            for (int32_t r = 0; r < nr; ++r) {
+                for (int32_t jj = tid; jj < tn; jj += bd) ring[sl * tn + jj] = 0u;
+                sl = (sl + 1 == D) ? 0 : sl + 1;
+            }
         }
-    }
-    // 4. LDS -> ring slice, except the slots this window consumed (cleared above,
-    // or never read for neurons that do not integrate); meanwhile the list's
-    // offset comes back from the atomic above
-    ring_slice_out(ring, gring, D, tn, (int32_t)(s0 % D), nsteps, RW);
-    __syncthreads();
-    for (uint32_t k = (uint32_t)tid; k < nl; k += (uint32_t)bd) {
-        const uint32_t ent = lsp[k];
-        wbuf[lsp_g + k] = ent;
-        if (a.rec) {
-            const uint64_t at = lsp_r + k;
-            if (at < a.rec_cap)
-                a.rec[at] = ((uint64_t)(uint32_t)(s0 + (int64_t)(ent >> 26)) << 32) | (ent & kIdMask);
-            else ctl->rec_overflow = 1;
+        const uint32_t nl = min(lsp_n, (uint32_t)kTiledBatch);
+        if (tid == 0) {
+            if (se_blk) atomicAdd((unsigned long long*)&ctl->se, se_blk);
+            if (nl) {
+                lsp_g = atomicAdd(&tcnt[bcur * nseg], nl);
+                lsp_r = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)nl);
+            }
         }
-    }
+        // 4. LDS -> ring slice, except the slots this window consumed (cleared above,
+        // or never read for neurons that do not integrate); meanwhile the list's
+        // offset comes back from the atomic above
+        if (!PW || kw + 1 == nw) ring_slice_out(ring, gring, D, tn, (int32_t)(s0 % D), nsteps, RW);
+        __syncthreads();
+        for (uint32_t k = (uint32_t)tid; k < nl; k += (uint32_t)bd) {
+            const uint32_t ent = lsp[k];
+            if (PW) st_agent(&wbuf[lsp_g + k], ent);
+            else wbuf[lsp_g + k] = ent;
+            if (a.rec) {
+                const uint64_t at = lsp_r + k;
+                if (at < a.rec_cap)
+                    a.rec[at] = ((uint64_t)(uint32_t)(s0 + (int64_t)(ent >> 26)) << 32) | (ent & kIdMask);
+                else ctl->rec_overflow = 1;
+            }
+        }
+        if constexpr (PW) {
+            if (kw + 1 < nw &&
+                !grid_wait(a.bar, (uint32_t)a.G, (uint32_t)g, (uint32_t)(kw + 1), a.bar_err, a.spin_limit))
+                return;
+        }
+    }   // windows
 }
 
 // Flush of deferred dendrite updates (kTStdp): for each neuron j the fused
@@ -1245,41 +1405,41 @@ size_t tiled_lds_bytes(int D, int B, int kind, int rw, int lut_lds) {
 }
 
 namespace {
-template <int WK>
-void launch_tiled_kind(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block, size_t lds,
-                       hipStream_t st) {
-    switch (a.chunk) {
-        case 1:
-            hipLaunchKernelGGL((k_tiled<1, WK>), dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
-            break;
-        case 2:
-            hipLaunchKernelGGL((k_tiled<2, WK>), dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
-            break;
-        case 4:
-            hipLaunchKernelGGL((k_tiled<4, WK>), dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
-            break;
-        case 16:
-            hipLaunchKernelGGL((k_tiled<16, WK>), dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
-            break;
-        default:
-            hipLaunchKernelGGL((k_tiled<8, WK>), dim3(a.G), dim3(block), lds, st, a, off, wk, nsteps);
-            break;
+template <int CH, int WK, bool PW>
+const void* tiled_fn() {
+    return reinterpret_cast<const void*>(&k_tiled<CH, WK, PW>);
+}
+
+template <int WK, bool PW>
+const void* tiled_fn_chunk(int chunk) {
+    switch (chunk) {
+        case 1: return tiled_fn<1, WK, PW>();
+        case 2: return tiled_fn<2, WK, PW>();
+        case 4: return tiled_fn<4, WK, PW>();
+        case 16: return tiled_fn<16, WK, PW>();
+        default: return tiled_fn<8, WK, PW>();
     }
 }
 
-template <int WK>
-hipError_t set_lds_kind(size_t bytes) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_tiled<1, WK>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_tiled<2, WK>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_tiled<4, WK>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_tiled<8, WK>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_tiled<16, WK>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    return e;
+// The fused kernel for (kind, chunk); the persistent form exists for every kind but kTStdp.
+const void* tiled_fn_of(int kind, int chunk, bool pw) {
+    switch (kind) {
+        case kTBF16: return pw ? tiled_fn_chunk<kTBF16, true>(chunk) : tiled_fn_chunk<kTBF16, false>(chunk);
+        case kTFP8: return pw ? tiled_fn_chunk<kTFP8, true>(chunk) : tiled_fn_chunk<kTFP8, false>(chunk);
+        case kTF32: return pw ? tiled_fn_chunk<kTF32, true>(chunk) : tiled_fn_chunk<kTF32, false>(chunk);
+        case kTClass16:
+            return pw ? tiled_fn_chunk<kTClass16, true>(chunk) : tiled_fn_chunk<kTClass16, false>(chunk);
+        case kTStdp: return pw ? nullptr : tiled_fn_chunk<kTStdp, false>(chunk);
+        default: return pw ? tiled_fn_chunk<kTClass, true>(chunk) : tiled_fn_chunk<kTClass, false>(chunk);
+    }
+}
+
+hipError_t launch_fn(const void* f, const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int32_t nwin,
+                     int block, size_t lds, hipStream_t st) {
+    if (!f) return hipErrorInvalidDeviceFunction;
+    TiledArgs aa = a;
+    void* args[] = {&aa, &off, &wk, &nsteps, &nwin};
+    return hipLaunchKernel(f, dim3(a.G), dim3(block), args, lds, st);
 }
 }  // namespace
 
@@ -1296,37 +1456,54 @@ __global__ void k_tile_ccache(const int32_t* tile_lo, const uint16_t* tbl, const
     if (side == 0 && k == 0) cls[2 * g + c] = cl;
 }
 
+// The class cache holds the classes of the tile's last and first neurons.
+__global__ void k_tile_lean(const int32_t* tile_lo, const uint8_t* type, const uint16_t* tbl, uint8_t* lean) {
+    const int32_t g = (int32_t)blockIdx.x, tid = (int32_t)threadIdx.x;
+    const int32_t t0 = tile_lo[g], t1 = tile_lo[g + 1];
+    const uint32_t cl0 = tbl[t1 - 1], cl1 = tbl[t0];
+    int ok = 1;
+    for (int32_t i = t0 + tid; i < t1; i += (int32_t)blockDim.x)
+        ok &= type[i] == T_LIFB && (tbl[i] == cl0 || tbl[i] == cl1);
+    ok = __syncthreads_and(ok);
+    if (tid == 0) lean[g] = ok ? 1 : 0;
+}
+
 hipError_t launch_tile_ccache(const int32_t* tile_lo, int32_t G, const uint16_t* tbl, const uint32_t* tbl_off,
                               const float* seq, float* cc, uint32_t* cls, hipStream_t st) {
     if (G <= 0) return hipSuccess;
+    if (!tile_lo || !tbl || !tbl_off || !seq || !cc || !cls) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_tile_ccache, dim3(G), dim3(4 * kClassCache), 0, st, tile_lo, tbl, tbl_off, seq, cc, cls);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_lean(const int32_t* tile_lo, int32_t G, const uint8_t* type, const uint16_t* tbl,
+                            uint8_t* lean, hipStream_t st) {
+    if (G <= 0) return hipSuccess;
+    if (!tile_lo || !type || !tbl || !lean) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tile_lean, dim3(G), dim3(256), 0, st, tile_lo, type, tbl, lean);
     return hipGetLastError();
 }
 
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st) {
     const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, a.kind == kTStdp ? a.lut_lds : 0);
-    switch (a.kind) {
-        case kTBF16: launch_tiled_kind<kTBF16>(a, off, wk, nsteps, block, lds, st); break;
-        case kTFP8: launch_tiled_kind<kTFP8>(a, off, wk, nsteps, block, lds, st); break;
-        case kTF32: launch_tiled_kind<kTF32>(a, off, wk, nsteps, block, lds, st); break;
-        case kTClass16: launch_tiled_kind<kTClass16>(a, off, wk, nsteps, block, lds, st); break;
-        case kTStdp: launch_tiled_kind<kTStdp>(a, off, wk, nsteps, block, lds, st); break;
-        default: launch_tiled_kind<kTClass>(a, off, wk, nsteps, block, lds, st); break;
-    }
-    return hipGetLastError();
+    return launch_fn(tiled_fn_of(a.kind, a.chunk, false), a, off, wk, nsteps, 1, block, lds, st);
 }
 
-template <int WK>
-int blocks_per_cu_kind(int chunk, int block, size_t lds) {
-    const void* f;
-    switch (chunk) {
-        case 1: f = (const void*)k_tiled<1, WK>; break;
-        case 2: f = (const void*)k_tiled<2, WK>; break;
-        case 4: f = (const void*)k_tiled<4, WK>; break;
-        case 16: f = (const void*)k_tiled<16, WK>; break;
-        default: f = (const void*)k_tiled<8, WK>; break;
-    }
+hipError_t launch_tiled_persistent(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int32_t nwin,
+                                   int block, hipStream_t st) {
+    if (a.kind == kTStdp || !a.bar || !a.bar_err) return hipErrorInvalidValue;
+    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, 0);
+    hipLaunchKernelGGL(k_bar_reset, dim3((a.G + 255) / 256), dim3(256), 0, st, a.bar, (uint32_t)a.G);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_fn(tiled_fn_of(a.kind, a.chunk, true), a, off, wk, nsteps, nwin, block, lds, st);
+}
+
+int tiled_blocks_per_cu(const TiledArgs& a, int block, bool persistent) {
+    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, a.kind == kTStdp ? a.lut_lds : 0);
+    const void* f = tiled_fn_of(a.kind, a.chunk, persistent);
+    if (!f) return 0;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, lds) != hipSuccess) {
         (void)hipGetLastError();
@@ -1335,27 +1512,15 @@ int blocks_per_cu_kind(int chunk, int block, size_t lds) {
     return nb;
 }
 
-int tiled_blocks_per_cu(const TiledArgs& a, int block) {
-    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, a.kind == kTStdp ? a.lut_lds : 0);
-    switch (a.kind) {
-        case kTBF16: return blocks_per_cu_kind<kTBF16>(a.chunk, block, lds);
-        case kTFP8: return blocks_per_cu_kind<kTFP8>(a.chunk, block, lds);
-        case kTF32: return blocks_per_cu_kind<kTF32>(a.chunk, block, lds);
-        case kTClass16: return blocks_per_cu_kind<kTClass16>(a.chunk, block, lds);
-        case kTStdp: return blocks_per_cu_kind<kTStdp>(a.chunk, block, lds);
-        default: return blocks_per_cu_kind<kTClass>(a.chunk, block, lds);
-    }
-}
-
 hipError_t set_tiled_lds_limit(int kind, size_t bytes) {
-    switch (kind) {
-        case kTBF16: return set_lds_kind<kTBF16>(bytes);
-        case kTFP8: return set_lds_kind<kTFP8>(bytes);
-        case kTF32: return set_lds_kind<kTF32>(bytes);
-        case kTClass16: return set_lds_kind<kTClass16>(bytes);
-        case kTStdp: return set_lds_kind<kTStdp>(bytes);
-        default: return set_lds_kind<kTClass>(bytes);
-    }
+    for (int pw = 0; pw < 2; ++pw)
+        for (int chunk : {1, 2, 4, 8, 16}) {
+            const void* f = tiled_fn_of(kind, chunk, pw != 0);
+            if (!f) continue;
+            const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
 }
 
 hipError_t launch_stdp_flush(const TiledArgs& a, int32_t off, int32_t wk, hipStream_t st) {

@@ -19,6 +19,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_CAPACITY = range(6)
 RAW, LIF0, POISSON, LIF, LIFE, LIFB = range(6)
 WF_AUTO, WF_CLASS, WF_F32, WF_BF16, WF_FP8 = range(5)
 PROP_AUTO, PROP_DIRECT, PROP_TILED = range(3)
+PERSIST_AUTO, PERSIST_OFF = range(2)
 CONN_BERNOULLI, CONN_FIXED_INDEGREE = 1, 2
 F_V, F_IEXC, F_IINH, F_REFR, F_LAST, F_CNT = range(6)
 
@@ -36,7 +37,8 @@ class Opts(C.Structure):
     _fields_ = [("device", C.c_int32), ("weight_format", C.c_int32), ("window", C.c_int32),
                 ("use_graph", C.c_int32), ("record_capacity", C.c_int64), ("prop_block", C.c_int32),
                 ("prop_grid", C.c_int32), ("profile", C.c_int32), ("propagation", C.c_int32),
-                ("tile_size", C.c_int32), ("chunk", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("tile_size", C.c_int32), ("chunk", C.c_int32), ("persistent", C.c_int32),
+                ("spin_limit_us", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class NetDesc(C.Structure):
@@ -76,7 +78,7 @@ class Info(C.Structure):
                 ("stdp_ms", C.c_double), ("neuron_launches", C.c_int64), ("prop_launches", C.c_int64),
                 ("stdp_launches", C.c_int64), ("propagation", C.c_int32), ("tile_size", C.c_int32),
                 ("tiles", C.c_int32), ("block", C.c_int32), ("segments", C.c_int32),
-                ("word_bytes", C.c_int32)]
+                ("word_bytes", C.c_int32), ("persistent", C.c_int32)]
 
 
 _lib = None
@@ -131,8 +133,11 @@ def _addr(a):
 
 
 def make_opts(weight_format=WF_AUTO, window=0, use_graph=0, record_capacity=0, device=0, prop_block=0,
-              prop_grid=0, profile=0, propagation=0, tile_size=0, chunk=0):
+              prop_grid=0, profile=0, propagation=0, tile_size=0, chunk=0, persistent=PERSIST_AUTO,
+              spin_limit_us=0):
     o = Opts()
+    o.persistent = persistent
+    o.spin_limit_us = spin_limit_us
     o.device = device
     o.weight_format = weight_format
     o.window = window
