@@ -129,8 +129,8 @@ SWEEP_NU_HZ = (10.0, 15.0, 20.0, 25.0)
 
 def sweep_params(n_nets=64, n_lo=2000, n_hi=50000, size_seed=2024):
     """Network b of the sweep: N LIF neurons drawn log-uniform on [n_lo, n_hi]
-    (80 % E / 20 % I) plus 0.1 N Poisson inputs, g = SWEEP_G[b % 4], Poisson rate
-    SWEEP_NU_HZ[(b // 4) % 4], seed b."""
+    (80 % E / 20 % I), g = SWEEP_G[b % 4], Poisson rate SWEEP_NU_HZ[(b // 4) % 4],
+    seed b."""
     rng = np.random.default_rng(size_seed)
     sizes = np.exp(rng.uniform(math.log(n_lo), math.log(n_hi), n_nets))
     out = []
@@ -140,20 +140,30 @@ def sweep_params(n_nets=64, n_lo=2000, n_hi=50000, size_seed=2024):
     return out
 
 
-def brunel_sweep_net(n, g, nu, seed, p=0.1, k_cap=5000, v_thresh=0.020):
+SWEEP_K_EXT = 1000          # external Poisson synapses per LIF neuron
+
+
+def brunel_sweep_net(n, g, nu, seed, p=0.1, k_cap=5000, v_thresh=0.020, k_ext=SWEEP_K_EXT):
     """One network of the sweep in generator form (the keyword arguments of
-    native.Simulator.generated).  Connectivity: Bernoulli with p_eff = min(p,
-    k_cap / sources), so the expected in-degree is at most k_cap."""
+    native.Simulator.generated).  Recurrent connectivity: Bernoulli with p_eff =
+    min(p, k_cap / N) between the LIF neurons, so the expected recurrent in-degree is
+    at most k_cap.  External drive: every LIF neuron receives k_ext synapses from a
+    pool of max(2000, N / 10) Poisson neurons firing at nu (fixed in-degree), so that
+    networks of every size are driven (with the pool's in-degree set by p instead,
+    networks below ~20k neurons never reach threshold)."""
     nE = int(round(0.8 * n))
     nI = n - nE
-    nP = int(round(0.1 * n))
-    p_eff = min(p, k_cap / float(nP + n))
+    nP = max(2000, int(round(0.1 * n)))
+    p_eff = min(p, k_cap / float(n))
     J = float(f32(1e-4))
     wi = float(f32(-g * 1e-4))
     dmin, dmax = float(f32(0.001)), float(f32(0.002))
     pops = [dict(num=nP, type=N.POISSON, rate=nu * DT), dict(num=nE, type=N.LIFB), dict(num=nI, type=N.LIFB)]
     projs = []
-    for s, t, w in ((0, 1, J), (0, 2, J), (1, 1, J), (1, 2, J), (2, 1, wi), (2, 2, wi)):
+    for s, t in ((0, 1), (0, 2)):
+        projs.append(dict(src=s, tar=t, kind=N.CONN_FIXED_INDEGREE, k=k_ext, wmin=J, wmax=J, dmin=dmin, dmax=dmax,
+                          index=len(projs), seed=seed))
+    for s, t, w in ((1, 1, J), (1, 2, J), (2, 1, wi), (2, 2, wi)):
         projs.append(dict(src=s, tar=t, kind=N.CONN_BERNOULLI, p=p_eff, wmin=w, wmax=w, dmin=dmin, dmax=dmax,
                           index=len(projs), seed=seed))
     return dict(pops=pops, projs=projs, lifconst=lifb_consts(v_thresh), step=DT, dt=DT, seed=seed,

@@ -187,6 +187,30 @@ static float philox_delay(uint64_t seed, uint64_t f, uint32_t proj, uint32_t tag
     return floor_ + (float)r * dt;
 }
 
+/* Fixed in-degree draw k of local target n in projection j: local source,
+ * weight and delay (seconds).  Target-major: draw f = n * K + k. */
+static void fixk_draw(uint64_t seed, uint32_t j, int32_t sn, int32_t K, uint64_t n, uint64_t k, float wmin,
+                      float wmax, float dmin, float dmax, float dw, float dt, int32_t* m, float* w, float* d) {
+    const uint64_t f = n * (uint64_t)K + k;
+    const uint32_t x = philox_pick(seed, f, j, TAG_FIXK);
+    *m = (int32_t)(((uint64_t)x * (uint64_t)sn) >> 32);
+    *w = philox_weight(seed, f, j, TAG_FIXK_W, wmin, wmax, dw);
+    *d = philox_delay(seed, f, j, TAG_FIXK_D, dmin, dmax, dt);
+}
+
+void orc_fixk_targets(uint64_t philox_seed, int32_t j, int32_t sn, int32_t K, float wmin, float wmax, float dmin,
+                      float dmax, float dw, float dt, float step, const int32_t* tloc, int32_t ntl, int32_t* src,
+                      float* w, int32_t* dsteps) {
+    for (int32_t q = 0; q < ntl; ++q)
+        for (int32_t k = 0; k < K; ++k) {
+            float d;
+            const size_t o = (size_t)q * (size_t)K + (size_t)k;
+            fixk_draw(philox_seed, (uint32_t)j, sn, K, (uint64_t)tloc[q], (uint64_t)k, wmin, wmax, dmin, dmax, dw,
+                      dt, &src[o], &w[o], &d);
+            dsteps[o] = (int)roundf(d / step);                 /* mgsim.cpp:49, as orc_build_csr */
+        }
+}
+
 int orc_gen_net(const orc_pop* pops, int32_t n_pops, const orc_proj* pros, int32_t n_pros,
                 uint32_t mt_seed, uint32_t rand_seed, uint64_t philox_seed,
                 float dw, float dt, orc_synlist* out) {
@@ -283,11 +307,10 @@ int orc_gen_net(const orc_pop* pops, int32_t n_pops, const orc_proj* pros, int32
             /* new: K sources per target drawn with replacement, target-major */
             for (uint64_t n = 0; n < (uint64_t)tn && rc == 0; ++n)
                 for (uint64_t k = 0; k < (uint64_t)pr->k && rc == 0; ++k) {
-                    uint64_t f = n * (uint64_t)pr->k + k;
-                    uint32_t x = philox_pick(philox_seed, f, (uint32_t)j, TAG_FIXK);
-                    int32_t m = (int32_t)(((uint64_t)x * (uint64_t)sn) >> 32);
-                    float w = philox_weight(philox_seed, f, (uint32_t)j, TAG_FIXK_W, pr->wmin, pr->wmax, dw);
-                    float d = philox_delay(philox_seed, f, (uint32_t)j, TAG_FIXK_D, pr->dmin, pr->dmax, dt);
+                    int32_t m;
+                    float w, d;
+                    fixk_draw(philox_seed, (uint32_t)j, sn, pr->k, n, k, pr->wmin, pr->wmax, pr->dmin, pr->dmax, dw,
+                              dt, &m, &w, &d);
                     rc = synbuf_push(&b, s0 + m, t0 + (int32_t)n, w, d, pl);
                 }
         }
@@ -627,70 +650,75 @@ static void step_f32(orc_sim* S, int32_t s) {
 }
 
 /* ---- one step, FP64 (accuracy oracle; plain double arithmetic) -------------- */
-static void step_f64(orc_sim* S, int32_t s) {
-    const int32_t D = S->D, n = S->n, slot = s % D;
-    const orc_csr* c = S->csr;
+/* Neuron i's update at step s in FP64 (accuracy oracle): reads and clears its
+ * ring slot, sets fired[i]. */
+static inline void neuron_f64(orc_sim* S, int32_t i, int32_t s) {
+    const int32_t D = S->D, slot = s % D;
     const double k005 = (double)0.005f, k098 = (double)0.98f, k099 = (double)0.99f;
     const double e_l = (double)-0.06f, e_i = (double)-0.08f, i_off = (double)0.02f;
-    for (int32_t i = 0; i < n; ++i) {
-        const float* C = S->lc + 30 * S->cset[i];
-        size_t loc = (size_t)i * (size_t)D + (size_t)slot;
-        double be = S->bexcd[loc], bi = S->binhd[loc];
-        int t = S->type[i];
-        int clear = 0;
-        if (t == T_POISSON) {
-            float u = orc_poisson_u(S->seed, (uint32_t)i, (uint32_t)s);
-            uint8_t f = S->rate[i] > u;
-            S->fired[i] = f;
-            if (f) { S->cnt[i]++; S->last[i] = s; }
-        } else if (t == T_LIF) {
-            S->fired[i] = 0;
-            if (S->refr[i] > 0) {
-                --S->refr[i];
-            } else {
-                double V = (double)C[17] * S->Vd[i] + (double)C[14] + S->Ied[i] * (double)C[12] +
-                           S->Iid[i] * (double)C[13];
-                S->Ied[i] *= (double)C[15];
-                S->Iid[i] *= (double)C[16];
-                if (V >= (double)C[4]) {
-                    S->fired[i] = 1; S->cnt[i]++; V = (double)C[3]; S->last[i] = s;
-                    S->refr[i] = S->refr_steps[S->cset[i]];
-                } else {
-                    S->Ied[i] += be;
-                    S->Iid[i] += bi;
-                }
-                S->Vd[i] = V;
-            }
-            clear = 1;
-        } else if (t == T_LIFB) {
-            S->fired[i] = 0;
-            if (S->refr[i] > 0) {
-                --S->refr[i];
-            } else if (S->Vd[i] >= (double)C[4]) {
-                S->fired[i] = 1; S->cnt[i]++; S->Vd[i] = (double)C[3]; S->last[i] = s;
+    const float* C = S->lc + 30 * S->cset[i];
+    size_t loc = (size_t)i * (size_t)D + (size_t)slot;
+    double be = S->bexcd[loc], bi = S->binhd[loc];
+    int t = S->type[i];
+    int clear = 0;
+    if (t == T_POISSON) {
+        float u = orc_poisson_u(S->seed, (uint32_t)i, (uint32_t)s);
+        uint8_t f = S->rate[i] > u;
+        S->fired[i] = f;
+        if (f) { S->cnt[i]++; S->last[i] = s; }
+    } else if (t == T_LIF) {
+        S->fired[i] = 0;
+        if (S->refr[i] > 0) {
+            --S->refr[i];
+        } else {
+            double V = (double)C[17] * S->Vd[i] + (double)C[14] + S->Ied[i] * (double)C[12] +
+                       S->Iid[i] * (double)C[13];
+            S->Ied[i] *= (double)C[15];
+            S->Iid[i] *= (double)C[16];
+            if (V >= (double)C[4]) {
+                S->fired[i] = 1; S->cnt[i]++; V = (double)C[3]; S->last[i] = s;
                 S->refr[i] = S->refr_steps[S->cset[i]];
             } else {
-                double V = S->Vd[i];
-                S->Vd[i] = V + ((be + bi) - k005 * V);
+                S->Ied[i] += be;
+                S->Iid[i] += bi;
             }
-            clear = 1;
-        } else if (t == T_LIFE) {
-            S->fired[i] = 0;
-            S->Ied[i] = be + S->Ied[i] * k098;
-            S->Iid[i] = -bi + S->Iid[i] * k099;
-            if (S->refr[i] > 0) {
-                --S->refr[i];
-            } else if (S->Vd[i] >= (double)C[4]) {
-                S->fired[i] = 1; S->cnt[i]++; S->Vd[i] = (double)C[3]; S->last[i] = s;
-                S->refr[i] = S->refr_steps[S->cset[i]];
-            } else {
-                double V = S->Vd[i];
-                S->Vd[i] = V + ((e_l - V) + S->Ied[i] * (0.0 - V) + S->Iid[i] * (e_i - V) + i_off) * k005;
-            }
-            clear = 1;
+            S->Vd[i] = V;
         }
-        if (clear) { S->bexcd[loc] = 0.0; S->binhd[loc] = 0.0; }
+        clear = 1;
+    } else if (t == T_LIFB) {
+        S->fired[i] = 0;
+        if (S->refr[i] > 0) {
+            --S->refr[i];
+        } else if (S->Vd[i] >= (double)C[4]) {
+            S->fired[i] = 1; S->cnt[i]++; S->Vd[i] = (double)C[3]; S->last[i] = s;
+            S->refr[i] = S->refr_steps[S->cset[i]];
+        } else {
+            double V = S->Vd[i];
+            S->Vd[i] = V + ((be + bi) - k005 * V);
+        }
+        clear = 1;
+    } else if (t == T_LIFE) {
+        S->fired[i] = 0;
+        S->Ied[i] = be + S->Ied[i] * k098;
+        S->Iid[i] = -bi + S->Iid[i] * k099;
+        if (S->refr[i] > 0) {
+            --S->refr[i];
+        } else if (S->Vd[i] >= (double)C[4]) {
+            S->fired[i] = 1; S->cnt[i]++; S->Vd[i] = (double)C[3]; S->last[i] = s;
+            S->refr[i] = S->refr_steps[S->cset[i]];
+        } else {
+            double V = S->Vd[i];
+            S->Vd[i] = V + ((e_l - V) + S->Ied[i] * (0.0 - V) + S->Iid[i] * (e_i - V) + i_off) * k005;
+        }
+        clear = 1;
     }
+    if (clear) { S->bexcd[loc] = 0.0; S->binhd[loc] = 0.0; }
+}
+
+static void step_f64(orc_sim* S, int32_t s) {
+    const int32_t D = S->D, n = S->n;
+    const orc_csr* c = S->csr;
+    for (int32_t i = 0; i < n; ++i) neuron_f64(S, i, s);
     for (int32_t i = 0; i < n; ++i) {
         if (!S->fired[i]) continue;
         S->se += c->row_ptr[i + 1] - c->row_ptr[i];
@@ -741,8 +769,9 @@ int64_t orc_run(orc_sim* S, int32_t nsteps, int32_t* ids, int64_t cap, int64_t* 
     return overflow ? -1 : w;
 }
 
-/* Multithreaded step loop of a static FP32 network (timing baseline only, not a
- * checker).  The neuron phase runs in parallel over neurons.  For propagation
+/* Multithreaded step loop of a static network (FP32, fixed point or FP64; the
+ * CPU timing baseline, and a faster checker for long runs).  The neuron phase
+ * runs in parallel over neurons.  For propagation
  * every CSR row is kept sorted by target (a stable sort, so the synapses of one
  * (source, target) pair keep list order) and each thread owns a contiguous
  * target range, which it finds in every fired row by binary search: each target
@@ -787,7 +816,7 @@ static void mt_sort_rows(orc_sim* S, int32_t nthreads) {
                 const int64_t k = b + idx[j];
                 S->mt_col[b + j] = c->col[k];
                 S->mt_dly[b + j] = c->dly[k];
-                S->mt_w[b + j] = S->w[k];
+                S->mt_w[b + j] = c->w[k];             /* static network: the CSR weights */
             }
         }
         free(idx);
@@ -804,18 +833,23 @@ static int64_t lower_bound32(const int32_t* a, int64_t lo, int64_t hi, int32_t v
     return lo;
 }
 
-int64_t orc_run_mt(orc_sim* S, int32_t nsteps, int32_t nthreads) {
-    if (S->acc != ORC_ACC_F32 || S->stdp || nthreads < 1) return -1;
-    const int32_t n = S->n, D = S->D;
+int64_t orc_run_mt(orc_sim* S, int32_t nsteps, int32_t nthreads, int32_t* ids, int64_t cap, int64_t* step_off) {
+    if (S->stdp || nthreads < 1) return -1;
+    const int32_t n = S->n, D = S->D, acc = S->acc;
     const orc_csr* c = S->csr;
     if (!S->mt_col) mt_sort_rows(S, nthreads);
-    int64_t spikes = 0, se = 0;
+    int64_t spikes = 0, se = 0, wr = 0;
+    int overflow = 0;
     for (int32_t st = 0; st < nsteps; ++st) {
         const int32_t s = (int32_t)(S->step + st);
+        if (step_off) step_off[st] = wr;
 #pragma omp parallel num_threads(nthreads) reduction(+ : spikes, se)
         {
 #pragma omp for schedule(static, 1024)
-            for (int32_t i = 0; i < n; ++i) neuron_f32(S, i, s);
+            for (int32_t i = 0; i < n; ++i) {
+                if (acc == ORC_ACC_F64) neuron_f64(S, i, s);
+                else neuron_f32(S, i, s);
+            }
             /* the implicit barrier of the loop above: fired[] is complete */
             const int32_t nt = omp_get_num_threads(), me = omp_get_thread_num();
             const int32_t t_lo = (int32_t)((int64_t)n * me / nt), t_hi = (int32_t)((int64_t)n * (me + 1) / nt);
@@ -824,21 +858,35 @@ int64_t orc_run_mt(orc_sim* S, int32_t nsteps, int32_t nthreads) {
                 if (me == 0) {
                     ++spikes;
                     se += c->row_ptr[i + 1] - c->row_ptr[i];
+                    if (ids) {                        /* ascending id within the step */
+                        if (wr < cap) ids[wr] = i;
+                        else overflow = 1;
+                    }
+                    wr++;
                 }
                 const int64_t b = lower_bound32(S->mt_col, c->row_ptr[i], c->row_ptr[i + 1], t_lo);
                 const int64_t e = lower_bound32(S->mt_col, b, c->row_ptr[i + 1], t_hi);
                 for (int64_t k = b; k < e; ++k) {
                     const float w = S->mt_w[k];
                     const size_t loc = (size_t)S->mt_col[k] * (size_t)D + (size_t)((s + S->mt_dly[k] + D) % D);
-                    if (w > 0) S->bexc[loc] += w;
-                    else S->binh[loc] += w;
+                    if (acc == ORC_ACC_F64) {
+                        if (w > 0) S->bexcd[loc] += (double)w;
+                        else S->binhd[loc] += (double)w;
+                    } else if (acc == ORC_ACC_FIXED) {
+                        if (w > 0) S->qexc[loc] += (int32_t)rintf(w * S->scale_exc);
+                        else S->qinh[loc] += (int32_t)rintf(w * S->scale_inh);
+                    } else {
+                        if (w > 0) S->bexc[loc] += w;
+                        else S->binh[loc] += w;
+                    }
                 }
             }
         }
     }
+    if (step_off) step_off[nsteps] = wr;
     S->step += nsteps;
     S->se += se;
-    return spikes;
+    return overflow ? -2 : spikes;
 }
 
 void orc_get_state(const orc_sim* S, int field, double* out) {

@@ -90,6 +90,13 @@ int orc_gen_net(const orc_pop* pops, int32_t n_pops, const orc_proj* pros, int32
                 uint32_t mt_seed, uint32_t rand_seed, uint64_t philox_seed,
                 float dw, float dt, orc_synlist* out);
 void orc_free_synlist(orc_synlist* s);
+/* The fixed in-degree connector's draws for chosen targets alone (target-major, so
+ * each target's K draws stand apart from the others'): for each local target
+ * tloc[q] of projection j (K draws from sn sources), local source, weight and
+ * quantised delay (roundf(delay / step)) of draw k at [q * K + k]. */
+void orc_fixk_targets(uint64_t philox_seed, int32_t j, int32_t sn, int32_t K, float wmin, float wmax, float dmin,
+                      float dmax, float dw, float dt, float step, const int32_t* tloc, int32_t ntl, int32_t* src,
+                      float* w, int32_t* dsteps);
 
 /* ---- CSR build (simulator/mgsim/adj/mgsim.cpp:8-144, npart = 1) ---------- */
 typedef struct {
@@ -142,9 +149,12 @@ orc_sim* orc_create(const orc_desc* d);
  * ids (capacity cap); step_off[k] is the offset of step k (nsteps+1 entries).
  * Returns the number of spikes written, or -1 if cap was exceeded. */
 int64_t orc_run(orc_sim* s, int32_t nsteps, int32_t* ids, int64_t cap, int64_t* step_off);
-/* Multithreaded FP32 step loop of a static network (OpenMP; timing baseline only,
- * not a checker): returns the spike count, -1 for other modes. */
-int64_t orc_run_mt(orc_sim* S, int32_t nsteps, int32_t nthreads);
+/* Multithreaded step loop of a static network (OpenMP; every mode, no STDP): the
+ * same results as orc_run, bit for bit (each thread owns a target range and adds
+ * the fired rows' synapses into it in ascending source order, as orc_run does).
+ * ids / cap / step_off as for orc_run (ids may be NULL).  Returns the spike count,
+ * -1 with STDP, -2 if cap was exceeded. */
+int64_t orc_run_mt(orc_sim* S, int32_t nsteps, int32_t nthreads, int32_t* ids, int64_t cap, int64_t* step_off);
 enum { ORC_F_V = 0, ORC_F_IEXC = 1, ORC_F_IINH = 2, ORC_F_REFR = 3, ORC_F_LAST = 4, ORC_F_CNT = 5 };
 /* Float fields are written as double. */
 void orc_get_state(const orc_sim* s, int field, double* out);

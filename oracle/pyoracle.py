@@ -93,9 +93,12 @@ def lib():
         L.orc_free_csr.argtypes = [C.POINTER(Csr)]
         L.orc_create.argtypes = [C.POINTER(Desc)]
         L.orc_create.restype = C.c_void_p
+        L.orc_fixk_targets.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                       C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, _i32p, C.c_int32,
+                                       _i32p, _f32p, _i32p]
         L.orc_run.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int64, _i64p]
         L.orc_run.restype = C.c_int64
-        L.orc_run_mt.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.orc_run_mt.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, C.c_int64, _i64p]
         L.orc_run_mt.restype = C.c_int64
         L.orc_get_state.argtypes = [C.c_void_p, C.c_int, _f64p]
         L.orc_get_weights.argtypes = [C.c_void_p, _f64p]
@@ -193,6 +196,18 @@ def gen_net(pops, projs, mt_seed=0, rand_seed=1, philox_seed=0, dw=0.01, dt=1e-4
                       arr(out.delay, np.float32), arr(out.plastic, np.uint8))
     lib().orc_free_synlist(C.byref(out))
     return res
+
+
+def fixk_targets(philox_seed, j, sn, k, wmin, wmax, dmin, dmax, tloc, dw=0.01, dt=1e-4, step=1e-4):
+    """Draws of the fixed in-degree connector for the local targets tloc of projection j:
+    (local source, weight, delay steps) arrays of shape (len(tloc), k), in list order."""
+    tl = np.ascontiguousarray(tloc, dtype=np.int32)
+    src = np.zeros((tl.size, k), np.int32)
+    w = np.zeros((tl.size, k), np.float32)
+    d = np.zeros((tl.size, k), np.int32)
+    lib().orc_fixk_targets(philox_seed, j, sn, k, wmin, wmax, dmin, dmax, dw, dt, step, _ptr(tl, _i32p), tl.size,
+                           _ptr(src, _i32p), _ptr(w, _f32p), _ptr(d, _i32p))
+    return src, w, d
 
 
 @dataclass
@@ -306,12 +321,24 @@ class OracleSim:
             raise RuntimeError("spike capacity exceeded")
         return [ids[off[k]:off[k + 1]].copy() for k in range(nsteps)]
 
-    def run_mt(self, nsteps, nthreads):
-        """Multithreaded FP32 steps (timing baseline; static networks): spike count."""
-        n = lib().orc_run_mt(self.h, nsteps, nthreads)
+    def run_mt(self, nsteps, nthreads, record=False, cap=None):
+        """Multithreaded steps of a static network (every mode; the same results as
+        run()).  record: also return the per-step ascending spike-id arrays."""
+        if not record:
+            n = lib().orc_run_mt(self.h, nsteps, nthreads, None, 0, None)
+            if n < 0:
+                raise RuntimeError("run_mt needs an oracle without STDP")
+            return int(n)
+        if cap is None:
+            cap = max(1024, self.n * nsteps // 8)
+        ids = np.zeros(cap, dtype=np.int32)
+        off = np.zeros(nsteps + 1, dtype=np.int64)
+        n = lib().orc_run_mt(self.h, nsteps, nthreads, _ptr(ids, _i32p), cap, _ptr(off, _i64p))
+        if n == -2:
+            raise RuntimeError("spike capacity exceeded")
         if n < 0:
-            raise RuntimeError("run_mt needs an FP32 oracle without STDP")
-        return int(n)
+            raise RuntimeError("run_mt needs an oracle without STDP")
+        return [ids[off[k]:off[k + 1]].copy() for k in range(nsteps)]
 
     def state(self, field_):
         out = np.zeros(self.n, dtype=np.float64)

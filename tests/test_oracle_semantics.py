@@ -218,23 +218,28 @@ def test_brunel_rate_matches_reference_benchdata():
     assert abs(rate - 27.1845) / 27.1845 < 0.04, rate
 
 
-def test_multithreaded_step_loop_equals_sequential():
-    """The oracle's OpenMP step loop (the all-cores CPU baseline) leaves the sequential
-    loop's state on a Brunel network, where every (target, sign) has one weight."""
+@pytest.mark.parametrize("acc", [po.ACC_F32, po.ACC_FIXED, po.ACC_F64])
+def test_multithreaded_step_loop_equals_sequential(acc):
+    """The oracle's OpenMP step loop (the all-cores CPU baseline, and the checker of long
+    runs) gives the sequential loop's spikes and state in every accumulation mode, on a
+    Brunel network and on a COBA network with mixed-sign, unequal weights."""
     from tests import harness as H
     from tests import netmodels as M
-    net = M.config1(scale=0.3, poisson_rate=0.01)
-    syn = H.host_synapses(net, seed=3)
-    _, o1 = H.oracle(net, syn, seed=3)
-    _, o2 = H.oracle(net, syn, seed=3)
-    o1.run(1500)
-    n = o2.run_mt(1500, 4)
-    assert n == int(o1.state(po.F_CNT).sum()) > 1000
-    assert np.array_equal(o1.state(po.F_CNT), o2.state(po.F_CNT))
-    assert np.array_equal(o1.state(po.F_V), o2.state(po.F_V))
-    assert o1.syn_events() == o2.syn_events()
-    o1.close()
-    o2.close()
+    for net, seed in ((M.config1(scale=0.3, poisson_rate=0.01), 3), (M.vogel_ref(200_000), 4)):
+        syn = H.host_synapses(net, seed=seed)
+        sc = (2.0 ** 30, 2.0 ** 30)
+        _, o1 = H.oracle(net, syn, seed=seed, acc=acc, scale=sc)
+        _, o2 = H.oracle(net, syn, seed=seed, acc=acc, scale=sc)
+        want = o1.run(1500)
+        got = o2.run_mt(1000, 4, record=True) + o2.run_mt(500, 3, record=True)
+        assert sum(len(x) for x in want) > 1000
+        assert all(np.array_equal(x, y) for x, y in zip(want, got))
+        for f in (po.F_V, po.F_IEXC, po.F_IINH, po.F_REFR, po.F_LAST, po.F_CNT):
+            assert np.array_equal(o1.state(f), o2.state(f))
+        assert o1.syn_events() == o2.syn_events()
+        assert o2.run_mt(10, 2) == sum(len(x) for x in o1.run(10))
+        o1.close()
+        o2.close()
 
 
 # ---- LIFE (mgcore.cu:197-233) and LIF (mgcore.cu:132-163) pinned by closed forms --------
@@ -361,3 +366,29 @@ def test_lif_single_input_response(acc, sign):
         assert math.isclose(sim.state(po.F_V)[1], want_v, rel_tol=tol), k
         assert math.isclose(sim.state(field)[1], float(w) * cd ** k, rel_tol=tol), k
     assert sim.state(po.F_CNT)[1] == 0
+
+
+def test_fixed_indegree_targets_alone_equal_the_full_generator():
+    """orc_fixk_targets (a chosen target's K draws, used to spot-check device-generated
+    networks too large for the full host generator) gives the full generator's synapses
+    of those targets, in list order, for every projection."""
+    net = M.config2(scale=0.01, K=300)
+    syn = H.host_synapses(net, seed=5)
+    bounds = np.cumsum([0] + [p[0] for p in net.pops])
+    base = 0
+    for j, pr in enumerate(net.projs):
+        sn = net.pops[pr["src"]][0]
+        tn = net.pops[pr["tar"]][0]
+        K = pr["k"]
+        tloc = np.array([0, 1, tn // 2, tn - 1], np.int32)
+        src, w, d = po.fixk_targets(5, j, sn, K, pr["wmin"], pr["wmax"], pr["dmin"], pr["dmax"], tloc)
+        for q, n in enumerate(tloc):
+            sl = slice(base + n * K, base + (n + 1) * K)
+            assert np.array_equal(syn.tar[sl], np.full(K, bounds[pr["tar"]] + n))
+            assert np.array_equal(syn.src[sl], src[q] + bounds[pr["src"]])
+            assert np.array_equal(syn.weight[sl].view(np.uint32), w[q].view(np.uint32))
+            q_d = np.floor((syn.delay[sl].astype(np.float32) / np.float32(1e-4)).astype(np.float64) + 0.5)
+            assert np.array_equal(q_d.astype(np.int32), d[q])
+            assert len(np.unique(d[q])) > 5            # delays drawn on the 1.0-2.0 ms grid
+        base += tn * K
+    assert base == syn.n
