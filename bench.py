@@ -49,15 +49,19 @@ def fused_roofline(b, info0, steps, se, spikes, n_lif, n_poisson, kname, lif_sta
     + 4 B list entry; per synaptic event 4 B target + 4 B FP32 weight + 1 B delay.
     layout_bytes_per_launch = what this layout itself must move: word_bytes per event,
     12 B per spike, 4 B per spike and tile of its segment, the ring rows live across
-    windows in and out (each tile's slice is as wide as the tile: one column per neuron), 32 B
-    per LIFB (48 B LIFE, lif_layout) / 20 B per Poisson neuron of state in and out."""
+    windows in and out (each tile's slice is as wide as the tile: one column per neuron; none
+    with persistent launches, whose slices stay in LDS), 32 B per LIFB (48 B LIFE, lif_layout) /
+    20 B per Poisson neuron of state in and out.  With persistent launches a "launch" here is
+    one window (the profiled spans count the windows of each launch)."""
     launches = b["prop_launches"]
     if not launches or b["prop_ms"] <= 0:
         return None
     avg_s = b["prop_ms"] / 1e3 / launches
     alg = steps * ((lif_state + 16.0) * n_lif + 4.0 * n_poisson) + 12.0 * spikes + 9.0 * se
     rw = 1 if b["weight_format"] == N.WF_CLASS else 2       # ring words per (slot, target)
-    live = max(info0["ring_len"] - info0["window"], 0)
+    # (a persistent launch keeps the ring slices in LDS across its windows: they move once per
+    # launch of tens of windows, not per window)
+    live = 0 if b.get("persistent") else max(info0["ring_len"] - info0["window"], 0)
     g_seg = b["tiles"] / max(1, b["segments"])
     lay = (b["word_bytes"] * se + 12.0 * spikes + 4.0 * spikes * g_seg
            + launches * (8.0 * rw * live * info0["n_neurons"] + lif_layout * n_lif + 20.0 * n_poisson))
@@ -122,9 +126,11 @@ def cpu_baseline(sim, budget_s=12.0, max_steps=4000, seed=0):
 
 def batched_leg(args, shapes=None, log=None):
     """The batched sweep: 64 independent Brunel networks (sizes log-uniform on
-    [2k, 50k] LIF neurons, g x Poisson rate grid, seeds 0-63) advanced together by one
-    simulator (mgb_create_batch_generated; every fused-kernel launch covers all
-    networks), against the same 64 networks run one after another, each alone."""
+    [2k, 50k] LIF neurons, g x Poisson rate grid, seeds 0-63, 1000 external Poisson
+    synapses per neuron) advanced together by one simulator (mgb_create_batch_generated;
+    every fused-kernel launch covers all networks), against the same 64 networks run one
+    after another, each alone; then the same batch with BF16 weights, whose per-network
+    rates over 1 s are compared with the exact batch's."""
     log = log or (lambda m: None)
     params, nets = W.sweep_nets(args.nets)
     sizes = [W.net_size(x) for x in nets]
@@ -137,9 +143,9 @@ def batched_leg(args, shapes=None, log=None):
     setup_s = time.perf_counter() - t0
     info0 = sim.info()
     log(f"# batched: {len(nets)} networks, {info0['n_neurons']} neurons, {info0['n_synapses']} synapses, "
-        f"{info0['segments']} segments, {info0['tiles']} tiles of {info0['tile_size']}, setup {setup_s:.2f} s, "
-        f"{info0['device_bytes'] / 2**30:.2f} GiB")
-    sim.run(args.warmup)
+        f"{info0['segments']} segments, {info0['tiles']} tiles of {info0['tile_size']}, persistent "
+        f"{info0['persistent']}, setup {setup_s:.2f} s, {info0['device_bytes'] / 2**30:.2f} GiB")
+    sim.run(args.settle + args.warmup)
     sim.sync()
     a = sim.info()
     t1 = time.perf_counter()
@@ -148,22 +154,21 @@ def batched_leg(args, shapes=None, log=None):
     wall = time.perf_counter() - t1
     b = sim.info()
     counts = sim.fire_counts()
-    sim.close()
     se = b["syn_events"] - a["syn_events"]
     spikes = b["spikes"] - a["spikes"]
     roof = None
+    n_poisson = sum(p["num"] for x in nets for p in x["pops"] if p["type"] == N.POISSON)
     if b["propagation"] == N.PROP_TILED:
-        n_poisson = sum(p["num"] for x in nets for p in x["pops"] if p["type"] == N.POISSON)
         roof = fused_roofline(b, info0, args.steps, se, spikes, info0["n_neurons"] - n_poisson, n_poisson,
                               "k_tiled (fused, batched segments)")
     log(f"# batched: {args.steps} steps in {wall * 1e3:.1f} ms, {se:.4g} events, {se / wall:.4g} SE/s, "
         f"RTF {wall / (args.steps * DT):.4f}")
     # the same networks one after another, each in its own simulator
-    seq_wall, seq_se, identical, max_rate_err = 0.0, 0, True, 0.0
+    seq_wall, seq_se, identical = 0.0, 0, True
     base = 0
     for k, net in enumerate(nets):
         s1 = N.Simulator.generated(**net, use_graph=args.graph, profile=0)
-        s1.run(args.warmup)
+        s1.run(args.settle + args.warmup)
         s1.sync()
         a1 = s1.info()
         t = time.perf_counter()
@@ -173,24 +178,19 @@ def batched_leg(args, shapes=None, log=None):
         seq_se += s1.info()["syn_events"] - a1["syn_events"]
         c1 = s1.fire_counts()
         s1.close()
-        cb = counts[base:base + sizes[k]]
-        identical = identical and bool(np.array_equal(c1, cb))
-        r1, rb = float(c1.sum()), float(cb.sum())
-        if r1 > 0:
-            max_rate_err = max(max_rate_err, abs(rb - r1) / r1)
+        identical = identical and bool(np.array_equal(c1, counts[base:base + sizes[k]]))
         base += sizes[k]
     log(f"# sequential: {seq_se:.4g} events in {seq_wall * 1e3:.1f} ms, {seq_se / seq_wall:.4g} SE/s; "
         f"per-network fire counts identical: {identical}")
     bf16 = None
     if not args.no_batch_bf16:
-        # the same batch with BF16 weights (per-source scales, fixed-point sums): rate error
-        # of every network against the exact run above
+        # the same batch with BF16 weights (per-source scales, fixed-point sums)
         table = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
         tb = json.load(open(table)).get(f"brunel_sweep{len(nets)}_bf16") if os.path.exists(table) else None
         shb = dict(tile_size=tb["tile_size"], prop_block=tb["block"], chunk=tb["chunk"]) if tb else {}
         sb = N.Simulator.batch_generated(nets, use_graph=args.graph, weight_format=N.WF_BF16, **shb)
         ib = sb.info()
-        sb.run(args.warmup)
+        sb.run(args.settle + args.warmup)
         sb.sync()
         a2 = sb.info()
         t = time.perf_counter()
@@ -198,34 +198,48 @@ def batched_leg(args, shapes=None, log=None):
         sb.sync()
         wall2 = time.perf_counter() - t
         se2 = sb.info()["syn_events"] - a2["syn_events"]
-        c2 = sb.fire_counts()
+        # rate error: every network's E and I rates over the next 1 s, BF16 against exact
+        acc_steps = int(round(1.0 / DT))
+        c_ex0, c_bf0 = sim.fire_counts(), sb.fire_counts()
+        sim.run(acc_steps)
+        sb.run(acc_steps)
+        d_ex, d_bf = sim.fire_counts() - c_ex0, sb.fire_counts() - c_bf0
         sb.close()
-        errs, base = [], 0
-        for k in range(len(nets)):
-            r_ex = float(counts[base:base + sizes[k]].sum())
-            r_bf = float(c2[base:base + sizes[k]].sum())
-            if r_ex > 0:
-                errs.append(abs(r_bf - r_ex) / r_ex)
+        errs, rates, base = [], [], 0
+        for k, net in enumerate(nets):
+            nP, nE = net["pops"][0]["num"], net["pops"][1]["num"]
+            for lo, hi in ((nP, nP + nE), (nP + nE, sizes[k])):
+                r_ex = float(d_ex[base + lo:base + hi].sum()) / (hi - lo)
+                r_bf = float(d_bf[base + lo:base + hi].sum()) / (hi - lo)
+                rates.append(r_ex)
+                if r_ex > 0:
+                    errs.append(abs(r_bf - r_ex) / r_ex)
             base += sizes[k]
         bf16 = {"value": se2 / wall2, "rtf": wall2 / (args.steps * DT), "tiles": ib["tiles"],
                 "tile_size": ib["tile_size"], "word_bytes": ib["word_bytes"],
-                "max_rate_err_vs_exact": max(errs) if errs else 0.0,
-                "mean_rate_err_vs_exact": float(np.mean(errs)) if errs else 0.0}
-        log(f"# batched BF16: {se2 / wall2:.4g} SE/s, RTF {wall2 / (args.steps * DT):.4f}, rate error vs exact "
-            f"max {bf16['max_rate_err_vs_exact'] * 100:.3f} % mean {bf16['mean_rate_err_vs_exact'] * 100:.3f} %")
+                "speedup_vs_sequential_exact": (se2 / wall2) / (seq_se / seq_wall),
+                "rate_error_window_s": acc_steps * DT,
+                "max_pop_rate_err_vs_exact": max(errs) if errs else 0.0,
+                "mean_pop_rate_err_vs_exact": float(np.mean(errs)) if errs else 0.0,
+                "pop_rates_hz_exact": [round(min(rates), 2), round(float(np.median(rates)), 2), round(max(rates), 2)]}
+        log(f"# batched BF16: {se2 / wall2:.4g} SE/s, RTF {wall2 / (args.steps * DT):.4f}; population rates over "
+            f"1 s: {min(rates):.1f}..{max(rates):.1f} Hz, error vs exact max {bf16['max_pop_rate_err_vs_exact'] * 100:.2f}"
+            f" % mean {bf16['mean_pop_rate_err_vs_exact'] * 100:.3f} %")
+    sim.close()
     return {
         "workload": f"brunel_sweep{len(nets)}",
         "description": (f"{len(nets)} independent Brunel networks: N log-uniform on [2000, 50000] LIFB "
-                        f"(80/20 E/I) + 0.1 N Poisson, g in {list(W.SWEEP_G)}, Poisson rate in "
-                        f"{list(W.SWEEP_NU_HZ)} Hz, Bernoulli p = min(0.1, 5000/sources), J = 0.1 mV, "
-                        f"delays U{{1.0..2.0}} ms, seeds 0-{len(nets) - 1}"),
+                        f"(80/20 E/I), recurrent Bernoulli p = min(0.1, 5000/N), {W.SWEEP_K_EXT} external "
+                        f"Poisson synapses per neuron from max(2000, N/10) Poisson neurons, g in "
+                        f"{list(W.SWEEP_G)}, Poisson rate in {list(W.SWEEP_NU_HZ)} Hz, J = 0.1 mV, delays "
+                        f"U{{1.0..2.0}} ms, seeds 0-{len(nets) - 1}"),
         "value": se / wall, "unit": "syn_events/s", "rtf": wall / (args.steps * DT),
         "neurons": info0["n_neurons"], "synapses": info0["n_synapses"], "segments": info0["segments"],
         "weight_format": f"f32 per-class ({info0['word_bytes']} B synapse word; exact)", "graph": args.graph,
-        "setup_s": round(setup_s, 2), "syn_events": se,
+        "setup_s": round(setup_s, 2), "syn_events": se, "settle_steps": args.settle,
         "sequential_value": seq_se / seq_wall, "sequential_rtf": seq_wall / (args.steps * DT),
         "speedup_vs_sequential": (se / wall) / (seq_se / seq_wall),
-        "fire_counts_identical_to_sequential": identical, "max_rate_err_vs_sequential": max_rate_err,
+        "fire_counts_identical_to_sequential": identical,
         "roofline": roof, "shapes": shapes, "bf16": bf16,
     }
 
@@ -418,7 +432,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default 10,000; 50,000 for --config 4)")
-    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4, 5),
                     help="1: Brunel 10k Bernoulli; 2: Brunel 100k K=10000 (default, plus the batched leg); "
                          "3: Vogels-Abbott COBA in four weight formats; 4: config 2 with STDP on E->E; "
@@ -428,6 +442,13 @@ def main(argv=None):
     ap.add_argument("--nets", type=int, default=64, help="networks in the batched sweep")
     ap.add_argument("--stdp-path", default="fused", choices=("fused", "direct"),
                     help="--config 4: fused window kernel (FP32 weights) or the direct kernels")
+    ap.add_argument("--settle", type=int, default=2000,
+                    help="untimed steps run first, so that the timed steps see the network past its start-up "
+                         "transient (default 2000 = 200 ms)")
+    ap.add_argument("--long", type=int, default=10000,
+                    help="steady-state run after the timed steps (reported beside them; the roofline comes "
+                         "from it); 0 = none")
+    ap.add_argument("--no-config1", action="store_true", help="skip the config-1 leg of the default run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg of the default run")
     ap.add_argument("--no-batch-bf16", action="store_true", help="skip the BF16 run of the batched leg")
     ap.add_argument("--scale", type=float, default=1.0)
@@ -503,7 +524,49 @@ def main(argv=None):
         }
         print(json.dumps(out))
         return 0
-    if args.config == 2:
+    main_cfg = 2 if args.config == 2 else 1
+    r = brunel_leg(args, main_cfg, tuned_table, log, cpu=not args.no_cpu)
+    side = None
+    if args.config == 2 and not args.no_config1:
+        side = brunel_leg(args, 1, tuned_table, log, cpu=False)
+    batched = None
+    if args.config == 2 and not args.no_batch:
+        batched = batched_leg(args, batch_shapes, log)
+
+    out = {
+        "metric": "syn. events/s & realtime factor, 1 MI355X; plain vs batched-BF16 @<=2% rate err",
+        "value": r["value"],
+        "unit": "syn_events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (network generated on the device from seed %d)" % args.seed,
+        "config": r["config"],
+        "rtf": r["rtf"],
+        "syn_events": r["syn_events"],
+        "settle_steps": args.settle,
+        "steady": r["steady"],
+        "roofline": r["roofline"],
+        "autotune": r["autotune"],
+        "cpu_baseline": r["cpu_baseline"],
+        "config1": side,
+        "batched": batched,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def brunel_leg(args, cfg, tuned_table, log, cpu=False):
+    """Brunel config 2 (100k LIFB, fixed in-degree K = 10,000, 1e9 synapses) or config 1
+    (10k LIFB, Bernoulli p = 0.1): settle, warm up, time args.steps steps, then a long
+    untimed-by-contract run (args.long steps) that gives the steady-state figures and the
+    roofline of the fused kernel."""
+    if cfg == 2:
         pops, projs, desc, ntot = brunel_fixed_k(args.scale, seed=args.seed)
         workload = "brunel100k_fixed_indegree_K10000"
     else:
@@ -517,8 +580,9 @@ def main(argv=None):
                                      prop_block=block, chunk=chunk)
 
     tuned = None
-    tile, block, chunk = args.tile, args.block, args.chunk
-    if not args.tune and not (tile or block or chunk) and args.scale == 1.0:
+    explicit = cfg == args.config and (args.tile or args.block or args.chunk)
+    tile, block, chunk = (args.tile, args.block, args.chunk) if explicit else (0, 0, 0)
+    if not args.tune and not explicit and args.scale == 1.0:
         shapes = tuned_table.get(workload)
         if shapes:
             tile, block, chunk = shapes["tile_size"], shapes["block"], shapes["chunk"]
@@ -539,98 +603,94 @@ def main(argv=None):
     tuned_chunk = chunk if chunk else 8
     sim.sync()
     setup_s = time.perf_counter() - t0
-    sim._bench_types = np.concatenate([np.full(p["num"], p["type"], np.int32) for p in pops])
-    sim._bench_rate = np.concatenate([np.full(p["num"], p.get("rate", 0.0), np.float32) for p in pops])
-    sim._bench_lifconst = lc
     info0 = sim.info()
-    print(f"# setup {setup_s:.2f} s: {info0['n_neurons']} neurons, {info0['n_synapses']} synapses, "
-          f"D={info0['ring_len']}, window={info0['window']}, format={info0['weight_format']}, "
-          f"{info0['device_bytes'] / 2**30:.2f} GiB on device", file=sys.stderr, flush=True)
+    log(f"# config {cfg} setup {setup_s:.2f} s: {info0['n_neurons']} neurons, {info0['n_synapses']} synapses, "
+        f"D={info0['ring_len']}, window={info0['window']}, format={info0['weight_format']}, "
+        f"persistent={info0['persistent']}, {info0['device_bytes'] / 2**30:.2f} GiB on device")
+    n_poisson = int(sum(p["num"] for p in pops if p["type"] == N.POISSON))
+    n_lif = info0["n_neurons"] - n_poisson
 
+    def timed(nsteps):
+        a = sim.info()
+        t1 = time.perf_counter()
+        sim.run(nsteps)
+        sim.sync()
+        wall = time.perf_counter() - t1
+        b = sim.info()
+        se = b["syn_events"] - a["syn_events"]
+        spikes = b["spikes"] - a["spikes"]
+        return wall, se, spikes, b
+
+    # settle (stated, untimed): the network leaves its start-up transient (first spikes of
+    # the integrating neurons arrive after the longest delay) before anything is timed
+    sim.run(args.settle)
     sim.run(args.warmup)
     sim.sync()
-    a = sim.info()
-    t1 = time.perf_counter()
-    sim.run(args.steps)
-    sim.sync()
-    t2 = time.perf_counter()
-    b = sim.info()
-    wall = t2 - t1
-    se = b["syn_events"] - a["syn_events"]
-    spikes = b["spikes"] - a["spikes"]
+    wall, se, spikes, b = timed(args.steps)
     sim_s = args.steps * DT
     value = se / wall
     rates_hz = spikes / ntot / sim_s
-    print(f"# timed {args.steps} steps in {wall * 1e3:.1f} ms (device {b['last_run_ms']:.1f} ms): "
-          f"{se:.4g} events, {spikes} spikes ({rates_hz:.2f} Hz mean), {value:.4g} SE/s, "
-          f"RTF {wall / sim_s:.4f}", file=sys.stderr, flush=True)
-
+    log(f"# config {cfg} timed {args.steps} steps (after {args.settle} settle + {args.warmup} warm-up) in "
+        f"{wall * 1e3:.2f} ms: {se:.4g} events, {spikes} spikes ({rates_hz:.2f} Hz mean), {value:.4g} SE/s, "
+        f"RTF {wall / sim_s:.4f}")
+    roof_src = (b, args.steps, se, spikes, f"timed run (steps {args.settle + args.warmup}.."
+                                               f"{args.settle + args.warmup + args.steps})")
+    steady = None
+    if args.long:
+        lw, lse, lsp, lb = timed(args.long)
+        s0 = args.settle + args.warmup + args.steps
+        steady = {"steps": args.long, "first_step": s0, "value": lse / lw, "rtf": lw / (args.long * DT),
+                  "ms_per_step": lw * 1e3 / args.long, "mean_rate_hz": lsp / ntot / (args.long * DT)}
+        log(f"# config {cfg} steady: {args.long} steps from step {s0} in {lw * 1e3:.1f} ms, {lse / lw:.4g} SE/s, "
+            f"RTF {lw / (args.long * DT):.4f}, {steady['mean_rate_hz']:.2f} Hz")
+        roof_src = (lb, args.long, lse, lsp, f"steady run (steps {s0}..{s0 + args.long})")
     roof = None
-    if b["prop_launches"] > 0:
-        n_poisson = int(sum(p["num"] for p in pops if p["type"] == N.POISSON))
-        n_lif = info0["n_neurons"] - n_poisson
-        if b["propagation"] == N.PROP_TILED:
-            roof = fused_roofline(b, info0, args.steps, se, spikes, n_lif, n_poisson,
-                                  "k_tiled (fused neuron update + CLASS propagation)")
-            # measured HBM bytes per launch: FETCH_SIZE / WRITE_SIZE counter passes of this
-            # workload scaled by a copy kernel of known size (profiles/traffic_*.json)
-            tfile = os.path.join(ROOT, "profiles", "traffic_" + workload.replace("_fixed_indegree", "") + ".json")
-            if roof and os.path.exists(tfile):
-                t = json.load(open(tfile))
-                if (t.get("tile_size"), t.get("block"), t.get("chunk"), t.get("word_bytes")) == \
-                        (b["tile_size"], b["block"], tuned_chunk, b["word_bytes"]):
-                    roof["traffic"] = round(t["traffic_bytes_per_launch"])
-        else:
-            avg_s = b["prop_ms"] / 1e3 / b["prop_launches"]
-            alg = 12.0 * spikes + 9.0 * se
-            ach = alg / b["prop_launches"] / avg_s / 1e9
-            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_prop (direct)",
-                    "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(alg / b["prop_launches"])}
+    rb, rsteps, rse, rsp, rname = roof_src
+    if rb["prop_launches"] > 0 and rb["propagation"] == N.PROP_TILED:
+        roof = fused_roofline(rb, info0, rsteps, rse, rsp, n_lif, n_poisson,
+                              "k_tiled (fused neuron update + CLASS propagation)")
         if roof:
-            print(f"# {roof['kernel']}: {roof['launches'] if 'launches' in roof else b['prop_launches']} launches, "
-                  f"avg {roof['avg_launch_us']:.2f} us, {roof['bytes_per_launch'] / 1e6:.3f} MB algorithmic -> "
-                  f"{roof['achieved']:.1f} GB/s", file=sys.stderr, flush=True)
+            roof["run"] = rname
+            roof["persistent"] = info0["persistent"]
+            # measured HBM bytes per window: FETCH_SIZE / WRITE_SIZE counter passes of this
+            # workload in the same regime and launch shape, scaled by a copy of known size
+            tfile = os.path.join(ROOT, "profiles", "traffic_" + workload.replace("_fixed_indegree", "") + ".json")
+            if os.path.exists(tfile):
+                t = json.load(open(tfile))
+                if (t.get("tile_size"), t.get("block"), t.get("chunk"), t.get("word_bytes"), t.get("persistent")) == \
+                        (rb["tile_size"], rb["block"], tuned_chunk, rb["word_bytes"], info0["persistent"]):
+                    roof["traffic"] = round(t["traffic_bytes_per_window"])
+                    roof["traffic_source"] = t["source"]
+            log(f"# config {cfg} {roof['kernel']}: {roof['launches']} windows ({roof['run']}), avg "
+                f"{roof['avg_launch_us']:.2f} us per window, {roof['bytes_per_launch'] / 1e6:.3f} MB algorithmic -> "
+                f"{roof['achieved']:.1f} GB/s ({roof['frac']:.3f} of peak)")
+    elif rb["prop_launches"] > 0:
+        avg_s = rb["prop_ms"] / 1e3 / rb["prop_launches"]
+        alg = 12.0 * rsp + 9.0 * rse
+        ach = alg / rb["prop_launches"] / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_prop (direct)",
+                "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(alg / rb["prop_launches"]),
+                "run": rname}
 
-    cpu = None
-    if not args.no_cpu:
-        print("# cpu baseline (oracle) ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(sim, budget_s=args.cpu_budget, seed=args.seed)
-        print(f"# cpu: {cpu['value']:.4g} SE/s with {cpu['cores']} threads over {cpu['steps']} steps in "
-              f"{cpu['seconds']:.1f} s; one thread {cpu['single_thread']['value']:.4g} SE/s", file=sys.stderr,
-              flush=True)
+    cpu_res = None
+    if cpu:
+        log("# cpu baseline (oracle) ...")
+        sim._bench_types = np.concatenate([np.full(p["num"], p["type"], np.int32) for p in pops])
+        sim._bench_rate = np.concatenate([np.full(p["num"], p.get("rate", 0.0), np.float32) for p in pops])
+        sim._bench_lifconst = lc
+        cpu_res = cpu_baseline(sim, budget_s=args.cpu_budget, seed=args.seed)
+        log(f"# cpu: {cpu_res['value']:.4g} SE/s with {cpu_res['cores']} threads over {cpu_res['steps']} steps in "
+            f"{cpu_res['seconds']:.1f} s; one thread {cpu_res['single_thread']['value']:.4g} SE/s")
     sim.close()
-
-    batched = None
-    if args.config == 2 and not args.no_batch:
-        batched = batched_leg(args, batch_shapes, log)
-
-    out = {
-        "metric": "syn. events/s & realtime factor, 1 MI355X; plain vs batched-BF16 @<=2% rate err",
-        "value": value,
-        "unit": "syn_events/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (network generated on the device from seed %d)" % args.seed,
+    return {
+        "value": value, "rtf": wall / sim_s, "ms_per_step": wall * 1e3 / args.steps, "syn_events": se,
         "config": {"workload": workload, "description": desc, "neurons": info0["n_neurons"],
                    "synapses": info0["n_synapses"], "simulated_s": sim_s, "weight_format": "f32 per-class",
-                   "mean_rate_hz": round(rates_hz, 3), "setup_s": round(setup_s, 2)},
-        "rtf": wall / sim_s,
-        "syn_events": se,
-        "roofline": roof,
-        "autotune": tuned,
-        "cpu_baseline": cpu,
-        "batched": batched,
+                   "mean_rate_hz": round(rates_hz, 3), "setup_s": round(setup_s, 2),
+                   "persistent": info0["persistent"], "window_steps": info0["window"]},
+        "steady": steady, "roofline": roof, "autotune": tuned, "cpu_baseline": cpu_res,
     }
-    print(json.dumps(out))
-    return 0
-
 
 if __name__ == "__main__":
     sys.exit(main())

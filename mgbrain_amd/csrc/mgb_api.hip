@@ -131,6 +131,9 @@ struct mgb_sim {
     uint32_t* bar_err = nullptr;       // set by a wait that timed out
     uint64_t spin_limit = 0;           // wall-clock ticks
     bool failed = false;               // a persistent launch timed out: results void
+    uint32_t* h_err = nullptr;         // pinned host copy of *bar_err, written at the end of each run
+    bool err_pending = false;          // h_err is due with the stream's completion
+    int xcd_map = -1;                  // fused launches: XCD-contiguous tile order (-1: not computed yet)
     // host mirrors for read-back
     std::vector<float> class_w;        // [n_tbl][2]
     std::vector<uint16_t> h_tbl;       // [n]
@@ -232,6 +235,7 @@ void free_all(mgb_sim* s) {
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->h_err) (void)hipHostFree(s->h_err);
 }
 
 // Per-neuron host inputs common to both construction paths.
@@ -1149,8 +1153,10 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
             khz = 100000;
         const uint64_t us = (o && o->spin_limit_us > 0) ? (uint64_t)o->spin_limit_us : 2000000ull;
         s->spin_limit = us * (uint64_t)khz / 1000ull;
-        update_persist(s);
+        MGB_HIP_TRY(hipHostMalloc((void**)&s->h_err, 4, hipHostMallocDefault));
+        *s->h_err = 0u;
     }
+    if (s->tiled) update_persist(s);
     MGB_HIP_TRY(hipStreamSynchronize(st));
     return MGB_OK;
 }
@@ -1204,12 +1210,18 @@ bool resident(const mgb_sim* s, const TiledArgs& a, bool persistent) {
 
 TiledArgs tiled_args(const mgb_sim* s);
 
-// Persistent launches run when asked for (MGB_PERSIST_AUTO), the kernel has the
-// form (not STDP) and every workgroup of the current launch shape is resident.
+// The launch form of the current launch shape, computed once per shape: the
+// XCD-contiguous tile order when every workgroup is resident, and persistent launches
+// when asked for (MGB_PERSIST_AUTO), the kernel has the form (not STDP) and every
+// workgroup of its persistent form is resident.
 void update_persist(mgb_sim* s) {
     s->persist = 0;
-    if (!s->tiled || s->tkind == kTStdp || s->persist_req != MGB_PERSIST_AUTO || !s->bar) return;
-    s->persist = resident(s, tiled_args(s), true) ? 1 : 0;
+    s->xcd_map = -1;
+    if (!s->tiled) return;
+    const TiledArgs a = tiled_args(s);
+    s->xcd_map = a.xcd_map;
+    if (s->tkind == kTStdp || s->persist_req != MGB_PERSIST_AUTO || !s->bar) return;
+    s->persist = resident(s, a, true) ? 1 : 0;
 }
 
 TiledArgs tiled_args(const mgb_sim* s) {
@@ -1243,7 +1255,7 @@ TiledArgs tiled_args(const mgb_sim* s) {
         a.w_max = s->sc[4]; a.w_min = s->sc[5];
     }
     a.bar = s->bar; a.bar_err = s->bar_err; a.spin_limit = s->spin_limit;
-    a.xcd_map = resident(s, a, false) ? 1 : 0;
+    a.xcd_map = s->xcd_map >= 0 ? s->xcd_map : (resident(s, a, false) ? 1 : 0);
     return a;
 }
 
@@ -1811,6 +1823,11 @@ mgb_status mgb_run(mgb_sim* s, int32_t nsteps) {
         }
     }
     if (nsteps - done > 0) MGB_TRY(issue_windows(s, nsteps - done, true));
+    if (s->persist && s->h_err) {
+        // the wait flag's error word comes back with the run (read by mgb_sync)
+        MGB_HIP_TRY(hipMemcpyAsync(s->h_err, s->bar_err, 4, hipMemcpyDeviceToHost, s->stream));
+        s->err_pending = true;
+    }
     MGB_HIP_TRY(hipEventRecord(s->ev1, s->stream));
     s->timed = true;
     return MGB_OK;
@@ -1823,10 +1840,9 @@ mgb_status mgb_sync(mgb_sim* s) {
     }
     MGB_HIP_TRY(hipSetDevice(s->device));
     MGB_HIP_TRY(hipStreamSynchronize(s->stream));
-    if (s->bar_err && !s->failed) {
-        uint32_t e = 0;
-        MGB_HIP_TRY(hipMemcpy(&e, s->bar_err, 4, hipMemcpyDeviceToHost));
-        s->failed = e != 0;
+    if (s->err_pending) {
+        s->failed = s->failed || *s->h_err != 0u;
+        s->err_pending = false;
     }
     if (s->failed) {
         set_error("a persistent launch's wait between windows exceeded its bound (%llu clock ticks): a workgroup "
