@@ -381,6 +381,21 @@ int orc_build_csr(int32_t n, int64_t s, const int32_t* src, const int32_t* tar,
     return 0;
 }
 
+int64_t orc_in_csr(int32_t n, int64_t s, const int32_t* col, const uint8_t* plastic, int64_t* in_ptr,
+                   int64_t* in_pos) {
+    memset(in_ptr, 0, ((size_t)n + 1) * sizeof(int64_t));
+    for (int64_t k = 0; k < s; ++k)
+        if (plastic[k]) in_ptr[col[k] + 1]++;
+    for (int32_t i = 0; i < n; ++i) in_ptr[i + 1] += in_ptr[i];
+    int64_t* cur = (int64_t*)malloc(((size_t)n + 1) * sizeof(int64_t));
+    if (!cur) return -1;
+    memcpy(cur, in_ptr, ((size_t)n + 1) * sizeof(int64_t));
+    for (int64_t k = 0; k < s; ++k)
+        if (plastic[k]) in_pos[cur[col[k]]++] = k;
+    free(cur);
+    return in_ptr[n];
+}
+
 void orc_free_csr(orc_csr* c) {
     free(c->row_ptr); free(c->col); free(c->dly); free(c->w); free(c->plastic);
     free(c->perm); free(c->in_ptr); free(c->in_pos);

@@ -118,6 +118,11 @@ int orc_build_csr(int32_t n, int64_t s, const int32_t* src, const int32_t* tar,
                   const float* weight, const float* delay_s, const uint8_t* plastic,
                   float step, orc_csr* out);
 void orc_free_csr(orc_csr* c);
+/* Incoming lists of a CSR given as arrays (e.g. one read back from the device): for
+ * every target, the CSR slots of its plastic synapses in slot order (in_ptr [n+1],
+ * in_pos [plastic count]); the STDP step uses no other order.  Returns the count. */
+int64_t orc_in_csr(int32_t n, int64_t s, const int32_t* col, const uint8_t* plastic, int64_t* in_ptr,
+                   int64_t* in_pos);
 
 /* ---- simulation (simulator/mgsim/adj/mgcore.cu:116-412, 535-554) -------- */
 enum {

@@ -96,6 +96,8 @@ def lib():
         L.orc_fixk_targets.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float,
                                        C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, _i32p, C.c_int32,
                                        _i32p, _f32p, _i32p]
+        L.orc_in_csr.argtypes = [C.c_int32, C.c_int64, _i32p, _u8p, _i64p, _i64p]
+        L.orc_in_csr.restype = C.c_int64
         L.orc_run.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int64, _i64p]
         L.orc_run.restype = C.c_int64
         L.orc_run_mt.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, C.c_int64, _i64p]
@@ -250,6 +252,22 @@ def build_csr(n, syn: SynapseList, step=1e-4) -> CsrArrays:
                     arr(c.in_ptr, n + 1, np.int64), arr(c.in_pos, s, np.int64), c.max_delay, c.min_delay)
     lib().orc_free_csr(C.byref(c))
     return res
+
+
+def csr_from_arrays(n, row_ptr, col, dly, w, D, min_delay, plastic=None):
+    """CsrArrays over a CSR given as arrays (e.g. read back from the device); with
+    plastic flags per slot, also the incoming lists the STDP step needs."""
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    if plastic is None:
+        return CsrArrays(n, row_ptr, col, dly, w, np.zeros(1, np.uint8), np.zeros(1, np.int64),
+                         np.zeros(n + 1, np.int64), np.zeros(1, np.int64), D, min_delay)
+    pl = np.ascontiguousarray(plastic, dtype=np.uint8)
+    in_ptr = np.zeros(n + 1, np.int64)
+    in_pos = np.zeros(max(1, int(np.count_nonzero(pl))), np.int64)
+    got = lib().orc_in_csr(n, col.size, _ptr(col, _i32p), _ptr(pl, _u8p), _ptr(in_ptr, _i64p), _ptr(in_pos, _i64p))
+    if got < 0:
+        raise RuntimeError("orc_in_csr failed")
+    return CsrArrays(n, row_ptr, col, dly, w, pl, np.zeros(1, np.int64), in_ptr, in_pos, D, min_delay)
 
 
 def csr_struct(cs: CsrArrays):

@@ -9,6 +9,8 @@ Bar (per workload):
     and per-population firing rates and CV of the inter-spike intervals within
     2 % over 1 s (10,000 steps) -- with the exact formats the whole second is
     bit-exact, so they are equal;
+  * with STDP on the E->E synapses (the 1e9-synapse network, fused path):
+    spikes and every weight bit-exact for 1000 steps;
   * the batched BF16 sweep against the FP64 oracle: per-population rates of
     every network within 2 % over 1 s, and the V_m error up to the first spike
     divergence.
@@ -252,3 +254,43 @@ def test_config5_sweep_corners_batched_bf16_against_fp64():
         assert max(errs) <= 0.02, errs
         o.close()
     b.close()
+
+
+def test_config4_brunel_100k_stdp_full_size():
+    """Config 2 with pair-based STDP on the E->E synapses (A+ 0.01, A- 0.0105 x W_max,
+    tau 20 ms, W in [0, 2J]) on the fused path (FP32 weights): 1000 steps against the
+    oracle in fixed point on the read-back CSR -- every spike and all 1e9 weights."""
+    _say("case: config 4 at full size (1e9 synapses, 5.8e8 plastic), fused STDP, 1000 steps")
+    net = M.config2(scale=1.0, stdp=True)
+    sc = M.stdp_consts_config4(J=1e-4)
+    g = H.gpu_generated(net, seed=0, stdp=True, stdpconst=sc, weight_format=N.WF_F32,
+                        propagation=N.PROP_TILED, record_capacity=4_000_000)
+    info = g.info()
+    n = info["n_neurons"]
+    assert info["propagation"] == N.PROP_TILED and info["window"] == 10
+    nP, nE = net.pops[0][0], net.pops[1][0]
+    rp, col, dl, w = g.csr()
+    plastic = np.zeros(col.size, np.uint8)
+    e0, e1 = int(rp[nP]), int(rp[nP + nE])
+    plastic[e0:e1] = (col[e0:e1] >= nP) & (col[e0:e1] < nP + nE)
+    assert 5.5e8 < int(plastic.sum()) < 6.1e8
+    cs = po.csr_from_arrays(n, rp, col, dl, w, info["ring_len"], info["min_delay"], plastic=plastic)
+    types, rate = net.neuron_arrays()
+    o = po.OracleSim(cs, types, rate, H.initial_v(net, 0), net.lifconst, stdpconst=sc, stdp=True, seed=0,
+                     acc=po.ACC_FIXED, scale_exc=info["fix_scale_exc"], scale_inh=info["fix_scale_inh"])
+    _say("  oracle built")
+    g.run(EXACT)
+    steps, ids = g.spikes()
+    got = H.split_steps(steps, ids, EXACT)
+    want = o.run(EXACT)
+    k = H.first_mismatch(got, want)
+    assert k < 0, f"spike lists differ first at step {k}"
+    assert sum(len(x) for x in want) > 20_000
+    wg = g.weights()
+    wo = o.weights().astype(np.float32)
+    assert np.array_equal(wg.view(np.uint32), wo.view(np.uint32))
+    changed = int(np.count_nonzero(wg != w))
+    _say(f"  {sum(len(x) for x in want)} spikes bit-exact; all weights bit-exact, {changed} changed by STDP")
+    assert changed > 1_000_000
+    g.close()
+    o.close()

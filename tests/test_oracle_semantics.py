@@ -392,3 +392,25 @@ def test_fixed_indegree_targets_alone_equal_the_full_generator():
             assert len(np.unique(d[q])) > 5            # delays drawn on the 1.0-2.0 ms grid
         base += tn * K
     assert base == syn.n
+
+
+def test_stdp_oracle_on_a_csr_given_as_arrays():
+    """The oracle built over a CSR given as arrays (as read back from the device) with
+    its incoming lists from orc_in_csr runs STDP exactly like the oracle built from the
+    synapse list (the incoming lists' order does not enter the result)."""
+    net = M.config2(scale=0.01, K=200, poisson_rate=0.3, stdp=True)
+    syn = H.host_synapses(net, seed=9)
+    sc = M.stdp_consts_config4(J=1e-4)
+    csr, o1 = H.oracle(net, syn, seed=9, stdp=True, stdpconst=sc)
+    cs = po.csr_from_arrays(net.n, csr.row_ptr, csr.col, csr.dly, csr.w, csr.D, csr.min_delay,
+                            plastic=csr.plastic)
+    assert 0 < cs.in_ptr[-1] < csr.col.size
+    types, rate = net.neuron_arrays()
+    o2 = po.OracleSim(cs, types, rate, H.initial_v(net, 9), net.lifconst, stdpconst=sc, stdp=True, seed=9)
+    a, b = o1.run(400), o2.run(400)
+    assert sum(len(x) for x in a) > 500
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    w1, w2 = o1.weights(), o2.weights()
+    assert np.array_equal(w1, w2) and np.sum(w1 != csr.w) > 100
+    o1.close()
+    o2.close()
