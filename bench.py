@@ -129,8 +129,9 @@ def batched_leg(args, shapes=None, log=None):
     [2k, 50k] LIF neurons, g x Poisson rate grid, seeds 0-63, 1000 external Poisson
     synapses per neuron) advanced together by one simulator (mgb_create_batch_generated;
     every fused-kernel launch covers all networks), against the same 64 networks run one
-    after another, each alone; then the same batch with BF16 weights, whose per-network
-    rates over 1 s are compared with the exact batch's."""
+    after another, each alone; then the same batch with BF16 weights, whose per-population
+    rates over --bf16-window seconds are compared with the exact batch's, beside the spread
+    between two exact batches that differ only in their Poisson keys."""
     log = log or (lambda m: None)
     params, nets = W.sweep_nets(args.nets)
     sizes = [W.net_size(x) for x in nets]
@@ -166,7 +167,7 @@ def batched_leg(args, shapes=None, log=None):
     # the same networks one after another, each in its own simulator
     seq_wall, seq_se, identical = 0.0, 0, True
     base = 0
-    for k, net in enumerate(nets):
+    for k, net in enumerate([] if args.no_sequential else nets):
         s1 = N.Simulator.generated(**net, use_graph=args.graph, profile=0)
         s1.run(args.settle + args.warmup)
         s1.sync()
@@ -180,8 +181,11 @@ def batched_leg(args, shapes=None, log=None):
         s1.close()
         identical = identical and bool(np.array_equal(c1, counts[base:base + sizes[k]]))
         base += sizes[k]
-    log(f"# sequential: {seq_se:.4g} events in {seq_wall * 1e3:.1f} ms, {seq_se / seq_wall:.4g} SE/s; "
-        f"per-network fire counts identical: {identical}")
+    if args.no_sequential:
+        seq_wall, seq_se, identical = None, None, None
+    else:
+        log(f"# sequential: {seq_se:.4g} events in {seq_wall * 1e3:.1f} ms, {seq_se / seq_wall:.4g} SE/s; "
+            f"per-network fire counts identical: {identical}")
     bf16 = None
     if not args.no_batch_bf16:
         # the same batch with BF16 weights (per-source scales, fixed-point sums)
@@ -198,33 +202,50 @@ def batched_leg(args, shapes=None, log=None):
         sb.sync()
         wall2 = time.perf_counter() - t
         se2 = sb.info()["syn_events"] - a2["syn_events"]
-        # rate error: every network's E and I rates over the next 1 s, BF16 against exact
-        acc_steps = int(round(1.0 / DT))
-        c_ex0, c_bf0 = sim.fire_counts(), sb.fire_counts()
+        # rate error: every network's E and I rates over the next args.bf16_window seconds,
+        # BF16 against exact; beside it the same error between the exact batch and an exact
+        # batch whose Poisson inputs draw from other keys (same networks, same initial V):
+        # the spread of a population's rate over input realisations, which bounds what a
+        # comparison of two chaotic runs can resolve
+        acc_steps = int(round(args.bf16_window / DT))
+        alt_nets = [dict(x, seed=x["seed"] + 1000) for x in nets]
+        sa = N.Simulator.batch_generated(alt_nets, use_graph=args.graph, **shp)
+        sa.run(args.settle + args.warmup + args.steps)
+        c_ex0, c_bf0, c_al0 = sim.fire_counts(), sb.fire_counts(), sa.fire_counts()
         sim.run(acc_steps)
         sb.run(acc_steps)
-        d_ex, d_bf = sim.fire_counts() - c_ex0, sb.fire_counts() - c_bf0
+        sa.run(acc_steps)
+        d_ex, d_bf, d_al = sim.fire_counts() - c_ex0, sb.fire_counts() - c_bf0, sa.fire_counts() - c_al0
         sb.close()
-        errs, rates, base = [], [], 0
+        sa.close()
+        errs, floor, rates, base = [], [], [], 0
         for k, net in enumerate(nets):
             nP, nE = net["pops"][0]["num"], net["pops"][1]["num"]
             for lo, hi in ((nP, nP + nE), (nP + nE, sizes[k])):
                 r_ex = float(d_ex[base + lo:base + hi].sum()) / (hi - lo)
                 r_bf = float(d_bf[base + lo:base + hi].sum()) / (hi - lo)
-                rates.append(r_ex)
+                r_al = float(d_al[base + lo:base + hi].sum()) / (hi - lo)
+                rates.append(r_ex / (acc_steps * DT))
                 if r_ex > 0:
                     errs.append(abs(r_bf - r_ex) / r_ex)
+                    floor.append(abs(r_al - r_ex) / r_ex)
             base += sizes[k]
         bf16 = {"value": se2 / wall2, "rtf": wall2 / (args.steps * DT), "tiles": ib["tiles"],
                 "tile_size": ib["tile_size"], "word_bytes": ib["word_bytes"],
-                "speedup_vs_sequential_exact": (se2 / wall2) / (seq_se / seq_wall),
+                "speedup_vs_sequential_exact": (se2 / wall2) / (seq_se / seq_wall) if seq_wall else None,
                 "rate_error_window_s": acc_steps * DT,
                 "max_pop_rate_err_vs_exact": max(errs) if errs else 0.0,
                 "mean_pop_rate_err_vs_exact": float(np.mean(errs)) if errs else 0.0,
+                "pops_within_2pct": [int(sum(e <= 0.02 for e in errs)), len(errs)],
+                "realisation_spread": {"max": max(floor) if floor else 0.0,
+                                       "mean": float(np.mean(floor)) if floor else 0.0,
+                                       "what": "exact batch vs exact batch with other Poisson keys"},
                 "pop_rates_hz_exact": [round(min(rates), 2), round(float(np.median(rates)), 2), round(max(rates), 2)]}
         log(f"# batched BF16: {se2 / wall2:.4g} SE/s, RTF {wall2 / (args.steps * DT):.4f}; population rates over "
-            f"1 s: {min(rates):.1f}..{max(rates):.1f} Hz, error vs exact max {bf16['max_pop_rate_err_vs_exact'] * 100:.2f}"
-            f" % mean {bf16['mean_pop_rate_err_vs_exact'] * 100:.3f} %")
+            f"{acc_steps * DT:g} s: {min(rates):.1f}..{max(rates):.1f} Hz, error vs exact max "
+            f"{bf16['max_pop_rate_err_vs_exact'] * 100:.2f} % mean {bf16['mean_pop_rate_err_vs_exact'] * 100:.3f} % "
+            f"({bf16['pops_within_2pct'][0]}/{len(errs)} within 2 %); exact vs other Poisson keys max "
+            f"{bf16['realisation_spread']['max'] * 100:.2f} % mean {bf16['realisation_spread']['mean'] * 100:.3f} %")
     sim.close()
     return {
         "workload": f"brunel_sweep{len(nets)}",
@@ -237,8 +258,9 @@ def batched_leg(args, shapes=None, log=None):
         "neurons": info0["n_neurons"], "synapses": info0["n_synapses"], "segments": info0["segments"],
         "weight_format": f"f32 per-class ({info0['word_bytes']} B synapse word; exact)", "graph": args.graph,
         "setup_s": round(setup_s, 2), "syn_events": se, "settle_steps": args.settle,
-        "sequential_value": seq_se / seq_wall, "sequential_rtf": seq_wall / (args.steps * DT),
-        "speedup_vs_sequential": (se / wall) / (seq_se / seq_wall),
+        "sequential_value": seq_se / seq_wall if seq_wall else None,
+        "sequential_rtf": seq_wall / (args.steps * DT) if seq_wall else None,
+        "speedup_vs_sequential": (se / wall) / (seq_se / seq_wall) if seq_wall else None,
         "fire_counts_identical_to_sequential": identical,
         "roofline": roof, "shapes": shapes, "bf16": bf16,
     }
@@ -399,19 +421,85 @@ def coba_leg(args, log):
     return {"workload": "vogels_abbott80k_p0.02", "description": desc, "formats": out}
 
 
-def tune_batched(args, log):
-    """Autotunes the batched sweep's launch shapes (each tile size builds the batch anew)."""
+def with_incumbent(cands, workload):
+    """Tile candidates plus the shapes table's current tile for the workload, so a
+    re-tune is measured against what it would replace."""
+    src = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
+    cur = (json.load(open(src)) if os.path.exists(src) else {}).get(workload)
+    out = list(cands)
+    if cur and cur.get("tile_size", 0) > 0 and cur["tile_size"] not in out:
+        out.append(cur["tile_size"])
+    return out
+
+
+def tune_batched(args, log, wf=N.WF_AUTO):
+    """Autotunes the batched sweep's launch shapes (each tile size builds the batch anew)
+    for the exact (CLASS) or the BF16 weight format."""
     _, nets = W.sweep_nets(args.nets)
-    probe = N.Simulator.batch_generated(nets)
+    probe = N.Simulator.batch_generated(nets, weight_format=wf)
     pinfo = probe.info()
     probe.close()
     if pinfo["propagation"] != N.PROP_TILED:
         return None
-    cands = sorted(set(T.tile_candidates(max(W.net_size(x) for x in nets), pinfo["ring_len"])[:2]
-                       + T.tile_candidates(pinfo["n_neurons"], pinfo["ring_len"])[:3]))
-    best, res = T.autotune(lambda b: N.Simulator.batch_generated(nets, tile_size=b), cands,
-                           blocks=(512, 1024), chunks=(4, 8, 16), log=lambda m: log("# " + m))
+    mw = 1024 if pinfo["word_bytes"] == 2 else 2048
+    cands = sorted(set(T.tile_candidates(max(W.net_size(x) for x in nets), pinfo["ring_len"], max_width=mw)[:2]
+                       + T.tile_candidates(pinfo["n_neurons"], pinfo["ring_len"], max_width=mw)[:3]))
+    cands = with_incumbent(cands, f"brunel_sweep{args.nets}" + ("_bf16" if wf == N.WF_BF16 else ""))
+    best, res = T.autotune(lambda b: N.Simulator.batch_generated(nets, tile_size=b, weight_format=wf), cands,
+                           blocks=(512, 1024), chunks=(4, 8, 16), steps=2000, warmup=500,
+                           log=lambda m: log("# " + m))
     return dict(best, candidates=len(res)) if best else None
+
+
+def tune_coba(args, log):
+    """Autotunes the Vogels-Abbott network's launch shapes (exact CLASS format; the
+    other formats use the same shape)."""
+    pops, projs, _desc, _ntot = W.vogels_abbott(args.scale, seed=args.seed)
+    lc = W.life_const1()
+
+    def make(tile):
+        return N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed, weight_format=N.WF_CLASS,
+                                     tile_size=tile)
+    probe = make(0)
+    pinfo = probe.info()
+    probe.close()
+    if pinfo["propagation"] != N.PROP_TILED:
+        return None
+    cands = with_incumbent(T.tile_candidates(pinfo["n_neurons"], pinfo["ring_len"], rw=2, max_width=1024),
+                           "vogels_abbott80k_p0.02")
+    best, res = T.autotune(make, cands, blocks=(256, 512, 1024), chunks=(2, 4, 8, 16), steps=2000, warmup=500,
+                           log=lambda m: log("# " + m))
+    return dict(best, candidates=len(res)) if best else None
+
+
+def tune_stdp(args, log):
+    """Autotunes the fused STDP path's launch shapes (FP32 words)."""
+    pops, projs, _desc, _ntot = W.brunel_fixed_k(args.scale, seed=args.seed, plastic_ee=True)
+    sc, lc = W.stdp_consts(), W.lifb_consts()
+
+    def make(tile):
+        return N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed, stdp=True, stdpconst=sc,
+                                     weight_format=N.WF_F32, propagation=N.PROP_TILED, tile_size=tile)
+    probe = make(0)
+    pinfo = probe.info()
+    probe.close()
+    cands = with_incumbent(T.tile_candidates(pinfo["n_neurons"], pinfo["ring_len"], max_width=2048),
+                           "brunel100k_K10000_stdp_ee")
+    best, res = T.autotune(make, cands, blocks=(512, 1024), chunks=(1, 2, 4, 8), steps=2000, warmup=500,
+                           log=lambda m: log("# " + m))
+    return dict(best, candidates=len(res)) if best else None
+
+
+def write_tuned(path, entries):
+    """Merges autotuned shapes {workload: best} into a copy of the shapes table."""
+    src = os.path.join(ROOT, "mgbrain_amd", "tuned_shapes.json")
+    table = json.load(open(src)) if os.path.exists(src) else {}
+    for k, v in entries.items():
+        if v:
+            table[k] = {"tile_size": v["tile_size"], "block": v["block"], "chunk": v["chunk"]}
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(table, f, indent=2)
 
 
 def copy_probe(nbytes=1 << 30):
@@ -451,6 +539,10 @@ def main(argv=None):
     ap.add_argument("--no-config1", action="store_true", help="skip the config-1 leg of the default run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg of the default run")
     ap.add_argument("--no-batch-bf16", action="store_true", help="skip the BF16 run of the batched leg")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="skip the batched leg's one-network-at-a-time comparison (profiling runs)")
+    ap.add_argument("--bf16-window", type=float, default=5.0,
+                    help="simulated seconds over which the batched BF16 run's population rates are compared")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1, help="replay the step loop as hipGraphs (default 1)")
@@ -458,6 +550,7 @@ def main(argv=None):
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--tune", action="store_true", help="run the autotuner instead of the tuned-shapes table")
+    ap.add_argument("--tuned-out", default="", help="with --tune: write the shapes table with the tuned entries here")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
@@ -477,6 +570,12 @@ def main(argv=None):
     if not args.steps:
         args.steps = 50000 if args.config == 4 else 10000
     if args.config == 3:
+        if args.tune:
+            best = tune_coba(args, log)
+            log(f"# tuned: {best}")
+            if args.tuned_out:
+                write_tuned(args.tuned_out, {"vogels_abbott80k_p0.02": best})
+            return 0
         r = coba_leg(args, log)
         best = r["formats"]["f32_class"]
         out = {
@@ -491,6 +590,12 @@ def main(argv=None):
         print(json.dumps(out))
         return 0
     if args.config == 4:
+        if args.tune:
+            best = tune_stdp(args, log)
+            log(f"# tuned: {best}")
+            if args.tuned_out:
+                write_tuned(args.tuned_out, {"brunel100k_K10000_stdp_ee": best})
+            return 0
         r = stdp_leg(args, log)
         out = {
             "metric": "syn. events/s & realtime factor, 1 MI355X; plain vs batched-BF16 @<=2% rate err",
@@ -511,6 +616,12 @@ def main(argv=None):
     if args.config == 5:
         if args.tune:
             batch_shapes = tune_batched(args, log)
+            bf = tune_batched(args, log, wf=N.WF_BF16)
+            log(f"# tuned: exact {batch_shapes}, bf16 {bf}")
+            if args.tuned_out:
+                write_tuned(args.tuned_out, {f"brunel_sweep{args.nets}": batch_shapes,
+                                             f"brunel_sweep{args.nets}_bf16": bf})
+            return 0
         r = batched_leg(args, batch_shapes, log)
         out = {
             "metric": "syn. events/s & realtime factor, 1 MI355X; plain vs batched-BF16 @<=2% rate err",
@@ -526,6 +637,11 @@ def main(argv=None):
         return 0
     main_cfg = 2 if args.config == 2 else 1
     r = brunel_leg(args, main_cfg, tuned_table, log, cpu=not args.no_cpu)
+    if args.tune:
+        log(f"# tuned: {r['autotune']}")
+        if args.tuned_out:
+            write_tuned(args.tuned_out, {r["config"]["workload"]: r["autotune"]["best"] if r["autotune"] else None})
+        return 0
     side = None
     if args.config == 2 and not args.no_config1:
         side = brunel_leg(args, 1, tuned_table, log, cpu=False)
@@ -592,8 +708,9 @@ def brunel_leg(args, cfg, tuned_table, log, cpu=False):
         pinfo = probe.info()
         probe.close()
         if pinfo["propagation"] == N.PROP_TILED:
-            cands = [tile] if tile else T.tile_candidates(pinfo["n_neurons"], pinfo["ring_len"])
-            best, res = T.autotune(lambda b: make(b), cands,
+            cands = [tile] if tile else with_incumbent(
+                T.tile_candidates(pinfo["n_neurons"], pinfo["ring_len"], max_width=1024), workload)
+            best, res = T.autotune(lambda b: make(b), cands, steps=2000, warmup=500,
                                    log=lambda m: print("# " + m, file=sys.stderr, flush=True))
             if best:
                 tile, block, chunk = best["tile_size"], best["block"], best["chunk"]

@@ -17,8 +17,15 @@ import math
 import time
 
 
-def tile_candidates(n, ring_len, lds_bytes=160 * 1024, batch_words=3 * 1024 + 4096, static_bytes=1024):
-    bmax = max(1, ((lds_bytes - static_bytes) // 4 - batch_words) // max(1, ring_len))
+def tile_candidates(n, ring_len, rw=1, max_width=1024, lds_bytes=160 * 1024, batch_words=3 * 1024 + 4096,
+                    static_bytes=1024, aux_words=0):
+    """Mean tile widths to try for n target neurons: about n/128 ... n/2048 wide,
+    each below the widest tile the fused kernel can hold -- its LDS ring slice
+    (rw words per slot and target, ring_len slots) beside the staging area
+    (batch_words, aux_words) -- and below the synapse word's local-target range
+    (max_width: 1024 for 16-bit CLASS words, 2048 for the 32-bit words)."""
+    room = (lds_bytes - static_bytes) // 4 - batch_words - aux_words
+    bmax = max(1, min(max_width, room // max(1, ring_len * rw)))
     out = []
     for g in (128, 256, 512, 1024, 2048):
         b = min(bmax, max(32, math.ceil(n / g)))

@@ -4,7 +4,9 @@ Bar: a simulator restored from a checkpoint and run on continues bit for bit
 like the original run on — per-step spike lists (recorded spikes included),
 fire counts, membrane potentials, plastic weights — for the fused and direct
 paths, the wide formats, STDP and batched networks.  The checkpoint is taken
-after a partial window so that pending spikes and ring contents matter.
+after a partial window so that pending spikes and ring contents matter.  A run
+split by a checkpoint into two simulators (the first closed before the second
+restores it) also equals the oracle's uninterrupted run bit for bit.
 """
 import sys
 
@@ -13,6 +15,7 @@ import pytest
 
 from mgbrain_amd import native as N
 from mgbrain_amd import workloads as W
+from oracle import pyoracle as po
 from tests import harness as H
 from tests import netmodels as M
 
@@ -89,3 +92,47 @@ def test_restore_rejects_other_layout():
     a.restore(blob)                     # the whole blob is accepted
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("kind", ["tiled_class", "direct_f32", "tiled_stdp"])
+def test_resumed_run_against_the_oracle(kind):
+    _say(f"case: checkpointed run against the oracle, {kind}")
+    sc, stdp, acc = None, False, po.ACC_F32
+    if kind == "tiled_stdp":
+        net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
+        syn = H.host_synapses(net, seed=6)
+        sc, stdp, acc = M.stdp_consts_config4(J=1e-4), True, po.ACC_FIXED
+        opts = dict(weight_format=N.WF_F32, propagation=N.PROP_TILED)
+    else:
+        net = M.config1(scale=0.3, poisson_rate=0.01)
+        syn = H.host_synapses(net, seed=31)
+        opts = dict(weight_format=N.WF_CLASS if kind == "tiled_class" else N.WF_F32,
+                    propagation=N.PROP_TILED if kind == "tiled_class" else N.PROP_DIRECT)
+        acc = po.ACC_F32 if kind == "tiled_class" else po.ACC_FIXED    # FP32 words sum in fixed point
+    seed = 6 if stdp else 31
+
+    def make():
+        return H.gpu_from_list(net, syn, seed=seed, stdp=stdp, stdpconst=sc, record_capacity=2_000_000, **opts)
+    a = make()
+    a.run(FIRST)
+    blob = a.checkpoint()
+    info = a.info()
+    a.close()
+    b = make()
+    b.restore(blob)
+    b.run(SECOND)
+    st, ids = b.spikes()
+    got = H.split_steps(st, ids, FIRST + SECOND)
+    _, o = H.oracle(net, syn, seed=seed, acc=acc, stdp=stdp, stdpconst=sc,
+                    scale=(info["fix_scale_exc"], info["fix_scale_inh"]))
+    want = o.run(FIRST + SECOND)
+    k = H.first_mismatch(got, want)
+    assert k < 0, f"spike lists differ first at step {k}"
+    assert sum(len(x) for x in want[FIRST:]) > 100
+    assert np.array_equal(b.state(N.F_V).view(np.uint32), o.state(po.F_V).astype(np.float32).view(np.uint32))
+    assert np.array_equal(b.fire_counts(), o.state(po.F_CNT).astype(np.int32))
+    assert b.info()["syn_events"] == o.syn_events()
+    if stdp:
+        assert np.array_equal(b.weights().view(np.uint32), o.weights().astype(np.float32).view(np.uint32))
+    b.close()
+    o.close()

@@ -4,7 +4,8 @@ the fixed-point FP32 format.
 1. The device's stored weights equal an independent numpy rounding of the
    list weights (BF16 round-to-nearest-even of w / scale, scale = row max |w|;
    E4M3 of w / scale, scale = row max |w| / 448), and with those weights the
-   simulation is bit-exact against the oracle's fixed-point mode.
+   simulation is bit-exact against the oracle's fixed-point mode; likewise for
+   FP8 with the caller's scales, one per source population.
 2. Against the FP64 oracle on the Vogels-Abbott COBA network (80,000 LIFE
    neurons, 1.28e8 synapses, 2 s at ~1.2 Hz): population rates within 2 % for every format;
    the membrane-potential error up to the first spike divergence is reported.
@@ -88,6 +89,51 @@ def test_stored_weights_and_bit_exact_dynamics(wf, prop):
             want[a:b] = (q * scale).astype(np.float32)
     assert np.array_equal(w.view(np.uint32), want.view(np.uint32))
     # oracle (fixed point, same scale) on the stored weights
+    info = g.info()
+    q = po.SynapseList(syn.src, syn.tar, syn.weight.copy(), syn.delay, syn.plastic)
+    q.weight[c.perm] = w
+    _, o = H.oracle(net, q, seed=9, acc=po.ACC_FIXED, scale=(info["fix_scale_exc"], info["fix_scale_inh"]))
+    g.run(1500)
+    st, ids = g.spikes()
+    k = H.first_mismatch(H.split_steps(st, ids, 1500), o.run(1500))
+    assert k < 0, k
+    assert np.array_equal(g.state(N.F_V).view(np.uint32), o.state(po.F_V).astype(np.float32).view(np.uint32))
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("prop", [N.PROP_DIRECT, N.PROP_TILED])
+def test_fp8_per_population_scales(prop):
+    """FP8 with the caller's scales, one per source population (the largest |w| of the
+    population's outgoing synapses / 448) instead of one per source row: stored weights
+    equal numpy's E4M3 rounding of w / s_pop times s_pop, and the run is bit-exact
+    against the oracle's fixed point on them."""
+    _say(f"case: ranged-weight net, FP8 with per-population scales, propagation={prop}")
+    net = ranged_net()
+    syn = H.host_synapses(net, mt_seed=78, rand_seed=6, dw=0.0001)
+    c = po.build_csr(net.n, syn)
+    bounds = np.cumsum([0] + [p[0] for p in net.pops])
+    sc = np.ones(net.n, np.float32)
+    for lo, hi in zip(bounds[:-1], bounds[1:]):
+        a, b = c.row_ptr[lo], c.row_ptr[hi]
+        if a < b:
+            sc[lo:hi] = np.float32(np.max(np.abs(c.w[a:b])) / np.float32(448.0))
+    assert len(np.unique(sc)) == 3
+    g = H.gpu_from_list(net, syn, seed=9, weight_format=N.WF_FP8, fp8_scale=sc, record_capacity=1_000_000,
+                        propagation=prop)
+    rp, col, dl, w = g.csr()
+    want, per_row = np.zeros_like(c.w), np.zeros_like(c.w)
+    for i in range(net.n):
+        a, b = c.row_ptr[i], c.row_ptr[i + 1]
+        if a == b:
+            continue
+        x = (c.w[a:b] / sc[i]).astype(np.float32)
+        want[a:b] = (e4m3_round(x).astype(np.float32) * sc[i]).astype(np.float32)
+        rs = np.float32(np.max(np.abs(c.w[a:b])) / np.float32(448.0))
+        per_row[a:b] = (e4m3_round((c.w[a:b] / rs).astype(np.float32)).astype(np.float32) * rs).astype(np.float32)
+    assert np.array_equal(w.view(np.uint32), want.view(np.uint32))
+    # the default per-row scales would have stored other values
+    assert np.sum(per_row != want) > 100
     info = g.info()
     q = po.SynapseList(syn.src, syn.tar, syn.weight.copy(), syn.delay, syn.plastic)
     q.weight[c.perm] = w

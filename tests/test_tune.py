@@ -31,3 +31,16 @@ def test_autotune_reports_the_requested_tile_size():
     assert (best["tile_size"], best["block"], best["chunk"]) == (430, 1024, 8)
     assert best["widest"] == 1024
     assert sorted({r["tile_size"] for r in res}) == [200, 430, 900]
+
+
+def test_tile_candidates_respect_the_ring_and_the_word_range():
+    # 100k targets, 16-slot ring, one ring word per slot: capped by the 16-bit word range
+    c = T.tile_candidates(100_000, 16)
+    assert max(c) == 782 and min(c) == 49 and c == sorted(c, reverse=True)
+    assert all(b <= 1024 for b in T.tile_candidates(10_000_000, 4))
+    assert max(T.tile_candidates(10_000_000, 4, max_width=2048)) == 2048
+    # a long ring with two words per slot (conductance rings) leaves room for fewer targets
+    room = (160 * 1024 - 1024) // 4 - (3 * 1024 + 4096)
+    assert max(T.tile_candidates(10_000_000, 40, rw=2)) == room // 80
+    assert max(T.tile_candidates(10_000_000, 40, rw=1)) == min(1024, room // 40)
+    assert T.tile_candidates(10, 16) == [32]
