@@ -234,17 +234,6 @@ mgb_status mgb_sync(mgb_sim* sim);
  * value.  Used by the autotuner (mgbrain_amd/tune.py); no reference counterpart. */
 mgb_status mgb_set_launch(mgb_sim* sim, int32_t block, int32_t chunk);
 
-/* Re-tiles the fused (TILED) path by measured activity: the synaptic events each
- * tile received since the simulator was built (or last rebalanced) set how many
- * tiles each segment (network of a batch) gets, so that the busiest networks are
- * cut finer and every tile of a window costs about the same; tiles stay balanced
- * by in-degree inside a segment.  Results do not depend on the tiling: runs
- * continue bit for bit.  A no-op when nothing was measured or the counts are
- * already those; MGB_ERR_UNSUPPORTED on the direct path and with STDP.
- * Checkpoints carry the tiling: load them into a simulator tiled alike.  No
- * reference counterpart (the reference has one network per device partition). */
-mgb_status mgb_rebalance(mgb_sim* sim);
-
 mgb_status mgb_get_info(mgb_sim* sim, mgb_info* out);
 
 /* Recorded spikes sorted by (step, neuron id): at most cap entries are
@@ -261,6 +250,11 @@ mgb_status mgb_read_state(mgb_sim* sim, int32_t field, void* out);
 
 /* fire_cnt of every neuron; replaces get_subnet_firecnt (mgnet.h:33, mgnet.cpp:148-158). */
 mgb_status mgb_fire_counts(mgb_sim* sim, int32_t* out);
+
+/* The fused path's tiles: tile_lo[0..tiles] (mgb_info.tiles + 1 entries), tile g
+ * holding neurons [tile_lo[g], tile_lo[g + 1]).  MGB_ERR_UNSUPPORTED on the direct
+ * path.  No reference counterpart (introspection of this build's tiling). */
+mgb_status mgb_read_tiles(mgb_sim* sim, int32_t* tile_lo);
 
 /* Device CSR in the reference's axon order: row_ptr[n+1] (axon_offs), target,
  * delay in steps and weight of each slot (SYNBlock via axon_refs,

@@ -134,12 +134,7 @@ struct mgb_sim {
     uint32_t* h_err = nullptr;         // pinned host copy of *bar_err, written at the end of each run
     bool err_pending = false;          // h_err is due with the stream's completion
     int xcd_map = -1;                  // fused launches: XCD-contiguous tile order (-1: not computed yet)
-    // activity of the tiles (not kTStdp): synaptic events per tile since the tiling was
-    // made or last balanced, over se_steps steps; the host inputs of the tiling
-    uint64_t* tile_se = nullptr;       // [G]
-    int64_t se_steps = 0;
-    std::vector<uint32_t> h_indeg;     // [2][n] excitatory / inhibitory in-degree
-    std::vector<uint8_t> h_type;       // [n]
+    uint64_t tile_hash = 0;            // FNV-1a of the tile boundaries (the ring slices' layout)
     // host mirrors for read-back
     std::vector<float> class_w;        // [n_tbl][2]
     std::vector<uint16_t> h_tbl;       // [n]
@@ -359,26 +354,19 @@ uint64_t host_u64(const void* p) {
 // A segment of Nq neurons gets at most max(ceil(Nq / B), its type runs) tiles
 // (B the requested mean tile size, default max(256, n / 256)) of at most the
 // LDS limit; the smallest per-tile weight cap meeting that is found by bisection.
-// Widest tile the fused kernel holds: its ring slice in LDS beside the staging
-// area and the static LDS; wide words hold an 11-bit tile-local target, CLASS tiles
-// of at most 1024 targets take 16-bit words (else 32-bit words with the global id).
-int32_t tile_width_cap(const mgb_sim* s, int32_t D) {
+mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D, const std::vector<uint8_t>& type,
+                        const std::vector<uint32_t>& indeg) {
     int lds_max = 65536;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
     lds_max = std::min(lds_max, 160 * 1024);
+    // the kernel's static LDS (block scan sums, event counter) shares the limit
     const int64_t per_b =
         (int64_t)(tiled_lds_bytes(D, 1, s->tkind, s->trw) - tiled_lds_bytes(D, 0, s->tkind, s->trw));
-    const int32_t bmax =
+    int32_t bmax =
         (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0, s->tkind, s->trw) - kTiledStaticLds) / per_b);
-    return std::max<int32_t>(1, std::min<int32_t>(bmax, s->tkind == kTClass ? 1024 : 2048));
-}
-
-// Tiles of the fused path: each segment's neurons cut into runs of one neuron type,
-// balanced by in-degree, about Nq / tile_size of them per segment (seg_target: a
-// given number per segment instead).  Allocates the tile tables.
-mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D, const std::vector<uint8_t>& type,
-                        const std::vector<uint32_t>& indeg, const std::vector<int32_t>* seg_target = nullptr) {
-    const int32_t bmax = tile_width_cap(s, D);
+    // wide words hold an 11-bit tile-local target; CLASS tiles of at most 1024
+    // targets take 16-bit words (else 32-bit words with the global id)
+    bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, s->tkind == kTClass ? 1024 : 2048));
     int32_t Breq = (o && o->tile_size > 0) ? o->tile_size : std::max<int32_t>(256, (n + 255) / 256);
     Breq = std::max<int32_t>(1, Breq);
     const std::vector<int32_t>& lo = s->h_seg_lo;
@@ -420,8 +408,7 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D, con
         const int32_t a = lo[q], b = lo[q + 1];
         tile0[q] = (int32_t)tlo.size();
         if (b <= a) continue;
-        const int32_t target = seg_target ? std::max<int32_t>(1, (*seg_target)[q])
-                                          : (int32_t)(((int64_t)(b - a) + Breq - 1) / Breq);
+        const int32_t target = (int32_t)(((int64_t)(b - a) + Breq - 1) / Breq);
         double t_lo = 0.0, t_hi = 0.0;
         for (int32_t i = a; i < b; ++i) {
             t_lo = std::max(t_lo, wt[i]);
@@ -443,6 +430,9 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D, con
     for (int32_t q = 0; q < nseg; ++q)
         for (int32_t g = tile0[q]; g < tile0[q + 1]; ++g) tseg[g] = (uint16_t)q;
     tlo.push_back(n);
+    uint64_t hsh = 1469598103934665603ull;
+    for (int32_t v : tlo) hsh = (hsh ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
+    s->tile_hash = hsh;
     int32_t B = 1;
     for (size_t g = 0; g + 1 < tlo.size(); ++g) B = std::max(B, tlo[g + 1] - tlo[g]);
     s->B = B;
@@ -846,8 +836,6 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         s->tkind = kTClass;
         if (s->tiled) {
             MGB_TRY(choose_tiles(s, o, n, s->D, h.type, h_indeg));
-            s->h_indeg = h_indeg;
-            s->h_type = h.type;
             // tile-sorted copy: stable sort by (source, target tile) keeps list order
             // inside every (source, tile) sub-row
             const TileTabs tt = tile_tabs(s, seg_dev);
@@ -953,8 +941,6 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
                 lifb_only = h.type[i] != MGB_LIF && h.type[i] != MGB_LIFE;
             s->trw = lifb_only ? 1 : 2;
             MGB_TRY(choose_tiles(s, o, n, s->D, h.type, h_indeg));
-            s->h_indeg = h_indeg;
-            s->h_type = h.type;
             const TileTabs tt = tile_tabs(s, seg_dev);
             // kout holds each CSR slot's source (the keys of the sort by source)
             MGB_HIP_TRY(make_tile_keys_csr(kout, s->col, tt, kin, vin, S, st));
@@ -1173,8 +1159,6 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
         s->spin_limit = us * (uint64_t)khz / 1000ull;
         MGB_HIP_TRY(hipHostMalloc((void**)&s->h_err, 4, hipHostMallocDefault));
         *s->h_err = 0u;
-        MGB_TRY(dalloc_n(s, &s->tile_se, (size_t)s->G));
-        MGB_HIP_TRY(hipMemsetAsync(s->tile_se, 0, 8 * (size_t)s->G, st));
     }
     if (s->tiled) update_persist(s);
     MGB_HIP_TRY(hipStreamSynchronize(st));
@@ -1275,7 +1259,6 @@ TiledArgs tiled_args(const mgb_sim* s) {
         a.w_max = s->sc[4]; a.w_min = s->sc[5];
     }
     a.bar = s->bar; a.bar_err = s->bar_err; a.spin_limit = s->spin_limit;
-    a.tile_se = s->tile_se;
     a.xcd_map = s->xcd_map >= 0 ? s->xcd_map : (resident(s, a, false) ? 1 : 0);
     return a;
 }
@@ -1436,198 +1419,6 @@ mgb_status validate_opts(const mgb_opts* o) {
         return MGB_ERR_INVALID;
     }
     return MGB_OK;
-}
-
-// Re-tiles the fused path (not kTStdp) with target[q] tiles for segment q: new tile
-// tables and tile-order synapse copy, built from the CSR as at creation; every
-// neuron's ring column moved to its new slice; per-tile caches, wait flags and
-// activity counters remade; the captured graph and launch form dropped.  Neuron
-// state, spike lists and counters do not depend on the tiling, and the kernel
-// accumulates in integers, so a run continues bit for bit.
-mgb_status retile(mgb_sim* s, const std::vector<int32_t>& target) {
-    hipStream_t st = s->stream;
-    const int32_t n = s->n;
-    const uint64_t S = (uint64_t)s->S;
-    MGB_HIP_TRY(hipStreamSynchronize(st));
-    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
-    s->gexec = nullptr;
-    s->graph = nullptr;
-    auto drop = [&](void* p, size_t bytes) {
-        if (!p) return;
-        dfree(s, p);
-        s->bytes -= (int64_t)bytes;
-    };
-    // the old tiles' ring and tables stay until the ring has moved
-    int32_t* old_lo = s->tile_lo;
-    uint32_t* old_of = s->tile_of;
-    uint32_t* old_ring = s->ring_t;
-    const size_t old_slab = s->ring_slab;
-    const int32_t G0 = s->G;
-    const size_t ns = (size_t)s->nseg + 1;
-    drop(s->syn16, 2 * (size_t)s->n_tslots);
-    drop(s->syn_t, 4 * (size_t)s->n_tslots);
-    drop(s->w_t, 4 * (size_t)s->n_tslots);
-    drop(s->offs16, 2 * (size_t)s->n_offs);
-    drop(s->offs32, 4 * (size_t)s->n_offs);
-    drop(s->prow, 8 * ((size_t)n + 1));
-    drop(s->tile_cc, 16 * (size_t)kClassCache * (size_t)G0);
-    drop(s->tile_ccls, 8 * (size_t)G0);
-    drop(s->tile_lean, (size_t)G0);
-    drop(s->tile_seg, 2 * (size_t)G0);
-    drop(s->seg_lo, 4 * ns);
-    drop(s->seg_tile0, 4 * ns);
-    drop(s->seg_kbase, 4 * ns);
-    drop(s->seg_obase, 8 * ns);
-    drop(s->bar, 4 * (size_t)G0 * kBarStride);
-    drop(s->tile_se, 8 * (size_t)G0);
-    s->syn16 = nullptr; s->syn_t = nullptr; s->w_t = nullptr;
-    s->offs16 = nullptr; s->offs32 = nullptr; s->prow = nullptr;
-    s->tile_cc = nullptr; s->tile_ccls = nullptr; s->tile_lean = nullptr; s->tile_seg = nullptr;
-    s->seg_lo = nullptr; s->seg_tile0 = nullptr; s->seg_kbase = nullptr; s->seg_obase = nullptr;
-    s->bar = nullptr; s->tile_se = nullptr;
-    s->tile_lo = nullptr; s->tile_of = nullptr; s->ring_t = nullptr;
-    // new tiles; the CLASS word kind is chosen anew from the widest tile, as at creation
-    const bool cls = s->tkind == kTClass || s->tkind == kTClass16;
-    if (cls) s->tkind = kTClass;
-    mgb_opts o{};
-    o.chunk = s->chunk;
-    o.prop_block = s->tblock;
-    MGB_TRY(choose_tiles(s, &o, n, s->D, s->h_type, s->h_indeg, &target));
-    Scratch X;
-    uint16_t* seg_dev = nullptr;
-    if (s->nseg > 1) {
-        std::vector<uint16_t> so((size_t)n);
-        for (int32_t q = 0; q < s->nseg; ++q)
-            for (int32_t t = s->h_seg_lo[q]; t < s->h_seg_lo[q + 1]; ++t) so[t] = (uint16_t)q;
-        MGB_TRY(X.get(&seg_dev, (size_t)n));
-        MGB_HIP_TRY(hipMemcpyAsync(seg_dev, so.data(), 2 * (size_t)n, hipMemcpyHostToDevice, st));
-        MGB_HIP_TRY(hipStreamSynchronize(st));
-    }
-    const TileTabs tt = tile_tabs(s, seg_dev);
-    uint32_t *src = nullptr, *kin = nullptr, *vin = nullptr, *kout = nullptr, *vout = nullptr;
-    MGB_TRY(X.get(&src, S));
-    MGB_TRY(X.get(&kin, S));
-    MGB_TRY(X.get(&vin, S));
-    MGB_TRY(X.get(&kout, S));
-    MGB_TRY(X.get(&vout, S));
-    MGB_HIP_TRY(slot_sources(s->row_ptr, n, S, src, st));
-    if (cls) MGB_HIP_TRY(make_tile_keys_class_csr(src, s->syn, tt, kin, vin, S, st));
-    else MGB_HIP_TRY(make_tile_keys_csr(src, s->col, tt, kin, vin, S, st));
-    X.release(src);
-    if (S > 0) {
-        const int tbits = bits_for(s->n_keys);
-        size_t tb = 0;
-        MGB_HIP_TRY(sort_pairs(nullptr, &tb, kin, kout, vin, vout, S, tbits, st));
-        uint8_t* temp = nullptr;
-        MGB_TRY(X.get(&temp, tb));
-        MGB_HIP_TRY(sort_pairs(temp, &tb, kin, kout, vin, vout, S, tbits, st));
-        MGB_HIP_TRY(hipStreamSynchronize(st));
-        X.release(temp);
-    }
-    X.release(kin);
-    X.release(vin);
-    if (cls) {
-        if (s->B <= 1024) {
-            s->tkind = kTClass16;
-            MGB_TRY(dalloc_n(s, &s->syn16, S));
-            MGB_HIP_TRY(class16_words(vout, S, tt, s->syn16, st));
-        } else {
-            MGB_TRY(dalloc_n(s, &s->syn_t, S));
-            if (S) MGB_HIP_TRY(hipMemcpyAsync(s->syn_t, vout, S * 4, hipMemcpyDeviceToDevice, st));
-        }
-    } else {
-        MGB_TRY(dalloc_n(s, &s->syn_t, S));
-        if (s->tkind == kTF32) MGB_TRY(dalloc_n(s, &s->w_t, S));
-        MGB_HIP_TRY(tile_gather_wide(vout, S, s->col, s->dly, s->w16, s->w8, s->w32, tt, s->tkind, s->syn_t,
-                                     s->w_t, st));
-    }
-    MGB_TRY(build_tile_offsets(s, n, kout, tt));
-    MGB_HIP_TRY(hipStreamSynchronize(st));
-    X.release(kout);
-    X.release(vout);
-    MGB_TRY(pad_subrows(s, n));
-    if (cls) {
-        MGB_TRY(dalloc_n(s, &s->tile_cc, (size_t)s->G * 4 * kClassCache));
-        MGB_TRY(dalloc_n(s, &s->tile_ccls, (size_t)s->G * 2));
-        MGB_HIP_TRY(launch_tile_ccache(s->tile_lo, s->G, s->tbl, s->tbl_off, s->seq, s->tile_cc, s->tile_ccls, st));
-        MGB_TRY(dalloc_n(s, &s->tile_lean, (size_t)s->G));
-        MGB_HIP_TRY(launch_tile_lean(s->tile_lo, s->G, s->type, s->tbl, s->tile_lean, st));
-    }
-    // every neuron's ring column into its new slice
-    s->ring_slab = ((size_t)s->D * (size_t)s->B * (size_t)s->trw + 3u) & ~(size_t)3u;
-    const size_t rt = (size_t)s->G * s->ring_slab;
-    MGB_TRY(dalloc_n(s, &s->ring_t, rt));
-    MGB_HIP_TRY(hipMemsetAsync(s->ring_t, 0, 4 * rt, st));
-    MGB_HIP_TRY(launch_ring_retile(old_ring, old_lo, old_of, old_slab, s->ring_t, s->tile_lo, s->tile_of,
-                                   s->ring_slab, n, s->D, s->trw, st));
-    MGB_HIP_TRY(hipStreamSynchronize(st));
-    drop(old_ring, 4 * (size_t)G0 * old_slab);
-    drop(old_lo, 4 * ((size_t)G0 + 1));
-    drop(old_of, 4 * (size_t)n);
-    MGB_TRY(dalloc_n(s, &s->bar, (size_t)s->G * kBarStride));
-    MGB_HIP_TRY(hipMemsetAsync(s->bar, 0, 4 * (size_t)s->G * kBarStride, st));
-    MGB_TRY(dalloc_n(s, &s->tile_se, (size_t)s->G));
-    MGB_HIP_TRY(hipMemsetAsync(s->tile_se, 0, 8 * (size_t)s->G, st));
-    s->se_steps = 0;
-    update_persist(s);
-    MGB_HIP_TRY(hipStreamSynchronize(st));
-    return MGB_OK;
-}
-
-// Tile counts per segment for the measured activity.  A tile's window costs about
-// kTileFixedEvents (propagation set-up, neuron steps, list flush) plus its synaptic
-// events; segment q with E_q events per window over T_q tiles then costs
-// kTileFixedEvents + E_q / T_q per tile.  For a total of G tiles, the smallest tile
-// cost C with sum_q T_q(C) <= G (T_q(C) = max(the fewest tiles that hold q's
-// neurons, ceil(E_q / (C - kTileFixedEvents)))) is found by bisection; the window
-// is then estimated at (G * kTileFixedEvents + sum_q E_q) / CUs (the work per CU)
-// + C (the last tile to start), and the G from the current count to four times it
-// with the smallest estimate wins.
-constexpr double kTileFixedEvents = 65536.0;
-
-std::vector<int32_t> balanced_tiles(const std::vector<double>& E, const std::vector<int32_t>& tmin, int32_t G0,
-                                    int cus) {
-    const size_t nq = E.size();
-    double esum = 0.0, emax = 0.0;
-    int64_t gmin = 0;
-    for (size_t q = 0; q < nq; ++q) {
-        esum += E[q];
-        emax = std::max(emax, E[q]);
-        gmin += tmin[q];
-    }
-    auto counts = [&](double C, std::vector<int32_t>* out) {
-        int64_t g = 0;
-        for (size_t q = 0; q < nq; ++q) {
-            double t = (double)tmin[q];
-            if (E[q] > 0.0) t = std::max(t, std::ceil(E[q] / (C - kTileFixedEvents)));
-            t = std::min(t, 1e9);
-            g += (int64_t)t;
-            if (out) (*out)[q] = (int32_t)t;
-        }
-        return g;
-    };
-    std::vector<int32_t> best(tmin);
-    double best_est = -1.0;
-    for (int k = 0; k <= 12; ++k) {
-        const int64_t G = std::max<int64_t>(gmin, (int64_t)G0 + (int64_t)G0 * k / 4);
-        // the smallest tile cost whose counts fit G tiles
-        double lo = kTileFixedEvents, hi = kTileFixedEvents + emax + 1.0;
-        if (counts(hi, nullptr) > G) continue;
-        for (int it = 0; it < 60; ++it) {
-            const double mid = 0.5 * (lo + hi);
-            if (counts(mid, nullptr) > G) lo = mid;
-            else hi = mid;
-        }
-        std::vector<int32_t> t(nq);
-        const int64_t g = counts(hi, &t);
-        const double est = ((double)g * kTileFixedEvents + esum) / (double)std::max(cus, 1) + hi;
-        if (best_est < 0.0 || est < best_est) {
-            best_est = est;
-            best = t;
-        }
-    }
-    return best;
 }
 
 mgb_status create_common(mgb_sim* s, const HostNeurons& h, const DevList& L, const mgb_opts* o) {
@@ -2036,7 +1827,6 @@ mgb_status mgb_run(mgb_sim* s, int32_t nsteps) {
         }
     }
     if (nsteps - done > 0) MGB_TRY(issue_windows(s, nsteps - done, true));
-    s->se_steps += nsteps;
     if (s->persist && s->h_err) {
         // the wait flag's error word comes back with the run (read by mgb_sync)
         MGB_HIP_TRY(hipMemcpyAsync(s->h_err, s->bar_err, 4, hipMemcpyDeviceToHost, s->stream));
@@ -2089,62 +1879,6 @@ mgb_status mgb_set_launch(mgb_sim* s, int32_t block, int32_t chunk) {
         s->graph = nullptr;
     }
     return MGB_OK;
-}
-
-mgb_status mgb_rebalance(mgb_sim* s) {
-    if (!s) {
-        set_error("null handle");
-        return MGB_ERR_INVALID;
-    }
-    MGB_TRY(mgb_sync(s));
-    if (s->failed) {
-        set_error("an earlier persistent launch of this simulator timed out; its state is void");
-        return MGB_ERR_HIP;
-    }
-    if (!s->tiled || s->tkind == kTStdp || !s->tile_se) {
-        set_error("mgb_rebalance: needs the fused (TILED) path without STDP");
-        return MGB_ERR_UNSUPPORTED;
-    }
-    const int32_t nseg = s->nseg;
-    const int64_t nwin = s->se_steps / std::max(s->window, 1);
-    std::vector<uint64_t> se((size_t)s->G);
-    MGB_HIP_TRY(hipMemcpy(se.data(), s->tile_se, 8 * se.size(), hipMemcpyDeviceToHost));
-    std::vector<double> E((size_t)nseg, 0.0);
-    double tot = 0.0;
-    for (int32_t q = 0; q < nseg; ++q)
-        for (int32_t g = s->h_seg_tile0[q]; g < s->h_seg_tile0[q + 1]; ++g) E[q] += (double)se[g];
-    for (double e : E) tot += e;
-    auto reset = [&]() -> mgb_status {
-        MGB_HIP_TRY(hipMemset(s->tile_se, 0, 8 * (size_t)s->G));
-        s->se_steps = 0;
-        return MGB_OK;
-    };
-    if (nwin <= 0 || tot <= 0.0) return reset();      // nothing measured: the tiling stays
-    for (double& e : E) e /= (double)nwin;
-    // the fewest tiles that hold each segment: its runs of one neuron type, cut at the
-    // widest tile (of the word kind chosen at creation for the CLASS format)
-    const int kind = s->tkind;
-    if (kind == kTClass16) s->tkind = kTClass;
-    const int32_t bmax = tile_width_cap(s, s->D);
-    s->tkind = kind;
-    std::vector<int32_t> tmin((size_t)nseg, 1), cur((size_t)nseg);
-    for (int32_t q = 0; q < nseg; ++q) {
-        int32_t t = 0, run = 0;
-        for (int32_t i = s->h_seg_lo[q]; i < s->h_seg_lo[q + 1]; ++i) {
-            if (i > s->h_seg_lo[q] && (s->h_type[i] != s->h_type[i - 1] || run == bmax)) {
-                ++t;
-                run = 0;
-            }
-            ++run;
-        }
-        tmin[q] = std::max<int32_t>(1, t + (run > 0 ? 1 : 0));
-        cur[q] = s->h_seg_tile0[q + 1] - s->h_seg_tile0[q];
-    }
-    int cus = 0;
-    MGB_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
-    const std::vector<int32_t> target = balanced_tiles(E, tmin, s->G, cus);
-    if (target == cur) return reset();
-    return retile(s, target);
 }
 
 mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
@@ -2282,6 +2016,7 @@ struct CkptHeader {
     int32_t abi, n, npad, D, window, tiled, tkind, B, G, nseg, mode, wide_kind, stdp;
     int64_t S, n_rec;
     uint64_t payload;
+    uint64_t tiling;        // hash of the tile boundaries (the ring slices' layout)
 };
 
 struct CkptSeg {
@@ -2327,6 +2062,7 @@ CkptHeader ckpt_header(const mgb_sim* s, uint64_t n_rec) {
     h.n = s->n; h.npad = s->npad; h.D = s->D; h.window = s->window; h.tiled = s->tiled;
     h.tkind = s->tkind; h.B = s->B; h.G = s->G; h.nseg = s->nseg; h.mode = s->mode;
     h.wide_kind = s->wide_kind; h.stdp = s->stdp; h.S = s->S; h.n_rec = (int64_t)n_rec;
+    h.tiling = s->tile_hash;
     for (const CkptSeg& g : ckpt_segments(s, n_rec)) h.payload += g.bytes;
     return h;
 }
@@ -2396,7 +2132,7 @@ mgb_status mgb_checkpoint_load(mgb_sim* s, const void* buf, int64_t bytes) {
     if (h.n != want.n || h.npad != want.npad || h.D != want.D || h.window != want.window ||
         h.tiled != want.tiled || h.tkind != want.tkind || h.B != want.B || h.G != want.G ||
         h.nseg != want.nseg || h.mode != want.mode || h.wide_kind != want.wide_kind ||
-        h.stdp != want.stdp || h.S != want.S || h.payload != want.payload) {
+        h.stdp != want.stdp || h.S != want.S || h.payload != want.payload || h.tiling != want.tiling) {
         set_error("checkpoint layout differs from this simulator (build it from the same network and "
                   "options)");
         return MGB_ERR_INVALID;
@@ -2416,6 +2152,21 @@ mgb_status mgb_checkpoint_load(mgb_sim* s, const void* buf, int64_t bytes) {
 }
 
 mgb_status mgb_fire_counts(mgb_sim* s, int32_t* out) { return mgb_read_state(s, MGB_F_CNT, out); }
+
+mgb_status mgb_read_tiles(mgb_sim* s, int32_t* tile_lo) {
+    if (!s || !tile_lo) {
+        set_error("null argument");
+        return MGB_ERR_INVALID;
+    }
+    if (!s->tiled) {
+        set_error("mgb_read_tiles: the simulator runs the direct path (no tiles)");
+        return MGB_ERR_UNSUPPORTED;
+    }
+    MGB_HIP_TRY(hipSetDevice(s->device));
+    MGB_HIP_TRY(hipStreamSynchronize(s->stream));
+    MGB_HIP_TRY(hipMemcpy(tile_lo, s->tile_lo, 4 * ((size_t)s->G + 1), hipMemcpyDeviceToHost));
+    return MGB_OK;
+}
 
 namespace {
 

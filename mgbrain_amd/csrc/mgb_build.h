@@ -86,11 +86,6 @@ hipError_t make_tile_keys(const int32_t* src, const int32_t* tar, const float* w
 hipError_t class16_words(const uint32_t* w32, uint64_t S, const TileTabs& tt, uint16_t* out, hipStream_t st);
 hipError_t make_tile_keys_csr(const uint32_t* src, const int32_t* col, const TileTabs& tt, uint32_t* keys,
                               uint32_t* vals, uint64_t S, hipStream_t st);
-// Re-tiling from the CSR: tile keys of a CLASS network's slots (values: the words),
-// and the source of every slot.
-hipError_t make_tile_keys_class_csr(const uint32_t* src, const uint32_t* syn, const TileTabs& tt, uint32_t* keys,
-                                    uint32_t* vals, uint64_t S, hipStream_t st);
-hipError_t slot_sources(const int64_t* row_ptr, int32_t n, uint64_t S, uint32_t* out, hipStream_t st);
 hipError_t tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col, const uint8_t* dly,
                             const uint16_t* w16, const uint8_t* w8, const float* w32, const TileTabs& tt, int kind,
                             uint32_t* syn_t, float* w_t, hipStream_t st);

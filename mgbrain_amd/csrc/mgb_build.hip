@@ -153,38 +153,6 @@ __global__ void k_tile_keys_csr(const uint32_t* src, const int32_t* col, TileTab
     }
 }
 
-// Tile keys of the CSR slots of a CLASS network (re-tiling from the CSR): slot k
-// (source src[k], word syn[k] = target | delay-1 << 26 | sign << 31); the value is
-// the word, as k_tile_keys gives it from the synapse list.
-__global__ void k_tile_keys_class_csr(const uint32_t* src, const uint32_t* syn, TileTabs tt, uint32_t* keys,
-                                      uint32_t* vals, uint64_t S) {
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
-         k += (uint64_t)gridDim.x * blockDim.x) {
-        const int32_t s = (int32_t)src[k];
-        const uint32_t w = syn[k];
-        const int32_t t = (int32_t)(w & 0x3FFFFFFu);
-        const int32_t q = tt.seg_of ? (int32_t)tt.seg_of[s] : 0;
-        const int32_t lo = tt.seg_lo[q];
-        const uint32_t gq = (uint32_t)(tt.seg_tile0[q + 1] - tt.seg_tile0[q]);
-        keys[k] = tt.seg_kbase[q] + (uint32_t)(s - lo) * gq + (tt.tile_of[t] - (uint32_t)tt.seg_tile0[q]);
-        vals[k] = w;
-    }
-}
-
-// Source of every CSR slot: the row i with row_ptr[i] <= k < row_ptr[i + 1].
-__global__ void k_slot_src(const int64_t* row_ptr, int32_t n, uint64_t S, uint32_t* out) {
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
-         k += (uint64_t)gridDim.x * blockDim.x) {
-        int32_t lo = 0, hi = n;              // row_ptr[lo] <= k < row_ptr[hi]
-        while (hi - lo > 1) {
-            const int32_t mid = lo + (hi - lo) / 2;
-            if ((uint64_t)row_ptr[mid] <= k) lo = mid;
-            else hi = mid;
-        }
-        out[k] = (uint32_t)lo;
-    }
-}
-
 // Fused-path words of the wide formats in tile order (slot[k]: the CSR slot at
 // tile-order position k): local target | delay-1 << 11 | BF16 / FP8 bits << 16
 // (kTStdp: the plastic flag << 16); FP32 weights go to w_t.
@@ -535,20 +503,6 @@ hipError_t make_tile_keys_csr(const uint32_t* src, const int32_t* col, const Til
                               uint32_t* vals, uint64_t S, hipStream_t st) {
     if (S == 0) return hipSuccess;
     hipLaunchKernelGGL(k_tile_keys_csr, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, col, tt, keys, vals, S);
-    return hipGetLastError();
-}
-
-hipError_t make_tile_keys_class_csr(const uint32_t* src, const uint32_t* syn, const TileTabs& tt, uint32_t* keys,
-                                    uint32_t* vals, uint64_t S, hipStream_t st) {
-    if (S == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_keys_class_csr, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, syn, tt, keys, vals,
-                       S);
-    return hipGetLastError();
-}
-
-hipError_t slot_sources(const int64_t* row_ptr, int32_t n, uint64_t S, uint32_t* out, hipStream_t st) {
-    if (S == 0 || n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_slot_src, dim3(blocks_for(S, 256)), dim3(256), 0, st, row_ptr, n, S, out);
     return hipGetLastError();
 }
 

@@ -242,8 +242,6 @@ struct TiledArgs {
     uint32_t* bar;
     uint32_t* bar_err;
     uint64_t spin_limit;
-    // [G] synaptic events each tile received, summed over its windows (null: not kept)
-    uint64_t* tile_se;
 };
 
 enum RingMode { kRingClass = 0, kRingFixed = 1 };
@@ -288,11 +286,6 @@ hipError_t launch_tiled_persistent(const TiledArgs& a, int32_t off, int32_t wk, 
                                    int block, hipStream_t st);
 hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
 hipError_t set_tiled_lds_limit(int kind, size_t bytes);
-// Moves every neuron's ring column from one tiling's ring slices to another's
-// (slices [RW][D][tile width], old_slab / new_slab words per tile).
-hipError_t launch_ring_retile(const uint32_t* old_ring, const int32_t* old_lo, const uint32_t* old_of,
-                              size_t old_slab, uint32_t* new_ring, const int32_t* new_lo, const uint32_t* new_of,
-                              size_t new_slab, int32_t n, int32_t D, int32_t rw, hipStream_t st);
 // kTStdp: brings the plastic in-synapses of the neurons the fused kernel listed
 // in window base_window + wk up to date (flush of deferred dendrite updates).
 hipError_t launch_stdp_flush(const TiledArgs& a, int32_t off, int32_t wk, hipStream_t st);

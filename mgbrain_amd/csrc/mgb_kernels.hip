@@ -1235,7 +1235,6 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         const uint32_t nl = min(lsp_n, (uint32_t)kTiledBatch);
         if (tid == 0) {
             if (se_blk) atomicAdd((unsigned long long*)&ctl->se, se_blk);
-            if (se_blk && a.tile_se) a.tile_se[g] += se_blk;    // the tile's own counter
             if (nl) {
                 lsp_g = atomicAdd(&tcnt[bcur * nseg], nl);
                 lsp_r = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)nl);
@@ -1522,31 +1521,6 @@ hipError_t set_tiled_lds_limit(int kind, size_t bytes) {
             if (e != hipSuccess) return e;
         }
     return hipSuccess;
-}
-
-__global__ void k_ring_retile(const uint32_t* old_ring, const int32_t* old_lo, const uint32_t* old_of,
-                              size_t old_slab, uint32_t* new_ring, const int32_t* new_lo, const uint32_t* new_of,
-                              size_t new_slab, int32_t n, int32_t D, int32_t rw) {
-    const int64_t total = (int64_t)n * D * rw;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t t = (int32_t)(i % n);               // neuron
-        const int64_t r = i / n;                          // ring row: hh * D + slot
-        const uint32_t go = old_of[t], gn = new_of[t];
-        const int32_t lo0 = old_lo[go], wo = old_lo[go + 1] - lo0;
-        const int32_t lo1 = new_lo[gn], wn = new_lo[gn + 1] - lo1;
-        new_ring[(size_t)gn * new_slab + (size_t)r * wn + (t - lo1)] =
-            old_ring[(size_t)go * old_slab + (size_t)r * wo + (t - lo0)];
-    }
-}
-
-hipError_t launch_ring_retile(const uint32_t* old_ring, const int32_t* old_lo, const uint32_t* old_of,
-                              size_t old_slab, uint32_t* new_ring, const int32_t* new_lo, const uint32_t* new_of,
-                              size_t new_slab, int32_t n, int32_t D, int32_t rw, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ring_retile, dim3(4096), dim3(256), 0, st, old_ring, old_lo, old_of, old_slab, new_ring,
-                       new_lo, new_of, new_slab, n, D, rw);
-    return hipGetLastError();
 }
 
 hipError_t launch_stdp_flush(const TiledArgs& a, int32_t off, int32_t wk, hipStream_t st) {

@@ -25,8 +25,8 @@ F_V, F_IEXC, F_IINH, F_REFR, F_LAST, F_CNT = range(6)
 
 EXPORTS = ["mgb_abi_version", "mgb_last_error", "mgb_create", "mgb_create_generated",
            "mgb_create_batch_generated", "mgb_run",
-           "mgb_sync", "mgb_set_launch", "mgb_rebalance", "mgb_get_info", "mgb_read_spikes", "mgb_clear_spikes",
-           "mgb_read_state", "mgb_fire_counts", "mgb_read_csr", "mgb_read_weights", "mgb_read_weights_noalloc",
+           "mgb_sync", "mgb_set_launch", "mgb_get_info", "mgb_read_spikes", "mgb_clear_spikes",
+           "mgb_read_state", "mgb_fire_counts", "mgb_read_tiles", "mgb_read_csr", "mgb_read_weights", "mgb_read_weights_noalloc",
            "mgb_checkpoint_size",
            "mgb_checkpoint_save", "mgb_checkpoint_load", "mgb_destroy"]
 
@@ -107,7 +107,7 @@ def lib():
         L.mgb_run.argtypes = [_p, C.c_int32]
         L.mgb_sync.argtypes = [_p]
         L.mgb_set_launch.argtypes = [_p, C.c_int32, C.c_int32]
-        L.mgb_rebalance.argtypes = [_p]
+        L.mgb_read_tiles.argtypes = [_p, _p]
         L.mgb_get_info.argtypes = [_p, C.POINTER(Info)]
         L.mgb_read_spikes.argtypes = [_p, C.c_int64, _p, _p, C.POINTER(C.c_int64)]
         L.mgb_clear_spikes.argtypes = [_p]
@@ -290,9 +290,11 @@ class Simulator:
     def set_launch(self, block=0, chunk=0):
         _check(lib().mgb_set_launch(self.h, int(block), int(chunk)))
 
-    def rebalance(self):
-        """Re-tiles the fused path by the activity measured so far (mgb_rebalance)."""
-        _check(lib().mgb_rebalance(self.h))
+    def tiles(self):
+        """Tile boundaries of the fused path: tile g holds neurons [t[g], t[g + 1])."""
+        out = np.zeros(self.info()["tiles"] + 1, dtype=np.int32)
+        _check(lib().mgb_read_tiles(self.h, out.ctypes.data))
+        return out
 
     def info(self):
         i = Info()

@@ -1,7 +1,7 @@
 """Edge cases of the HIP path against the oracle (bit-exact spike lists and state):
 no synapses, a silent network, delays beyond the 5-bit field (direct path),
 one-neuron tiles, rows of about 2^16 synapses, a one-neuron network with a self-synapse, a batch with an
-empty and a Poisson-only network, and ragged tile shapes."""
+empty and a Poisson-only network, and ragged tile shapes; the tiles' invariants (mgb_read_tiles)."""
 import sys
 
 import numpy as np
@@ -148,3 +148,30 @@ def test_batch_with_empty_and_poisson_only_networks():
     assert cb[n0:].sum() > 1000
     for x in (b, s1, s3):
         x.close()
+
+
+@pytest.mark.parametrize("wf", [N.WF_CLASS, N.WF_BF16])
+def test_tiles_partition_the_neurons(wf):
+    """mgb_read_tiles: the tiles cover every neuron once, in order, none wider than
+    the widest tile reported, none straddling two networks of a batch or mixing
+    neuron types; the direct path has none."""
+    _params, nets = W.sweep_nets(n_nets=5, n_lo=1500, n_hi=5000, size_seed=19)
+    _say(f"case: tiles of a batch of {len(nets)} sweep networks, weight_format={wf}")
+    b = N.Simulator.batch_generated(nets, weight_format=wf, propagation=N.PROP_TILED, tile_size=200)
+    info = b.info()
+    t = b.tiles()
+    assert len(t) == info["tiles"] + 1 and t[0] == 0 and t[-1] == info["n_neurons"]
+    widths = np.diff(t)
+    assert widths.min() >= 1 and widths.max() == info["tile_size"]
+    bounds = np.cumsum([0] + [W.net_size(x) for x in nets])
+    types = np.concatenate([np.full(p["num"], p["type"], np.int32) for x in nets for p in x["pops"]])
+    for lo, hi in zip(t[:-1], t[1:]):
+        net = np.searchsorted(bounds, lo, side="right") - 1
+        assert hi <= bounds[net + 1]                       # one network
+        assert np.all(types[lo:hi] == types[lo])          # one neuron type
+    b.close()
+    d = H.gpu_generated(M.config1(scale=0.1), seed=2, propagation=N.PROP_DIRECT)
+    with pytest.raises(N.MGBError) as e:
+        d.tiles()
+    assert e.value.status == N.ERR_UNSUPPORTED
+    d.close()
