@@ -4,15 +4,18 @@
 Workload (default): Brunel network of 80,000 excitatory + 20,000 inhibitory
 LIFB neurons and 10,000 Poisson inputs at 15 Hz, fixed in-degree K = 10,000
 (1.0e9 synapses), J = 0.1 mV, g = 5, delays on the 1.0-2.0 ms grid, dt = 0.1 ms,
-FP32 weights (exact per-class weights, 4 B per synapse on the device).  The
-network is generated on the device from a seed; inputs are resident in HBM
-before the timed region.
+FP32 weights (exact per-class weights: a 4 B CSR word per synapse and a 2 B
+tile-local word in the fused kernel's copy).  The network is generated on the
+device from a seed; inputs are resident in HBM before the timed region.
 
 A "step" is one simulation time step (0.1 ms of biological time) of the whole
-network.  The timed region runs --steps steps after --warmup untimed steps.
+network.  The timed region runs --steps steps after --settle + --warmup untimed
+steps; a --long run beside it gives the steady-state figures and the roofline.
 Prints one JSON line: synaptic events/s (value), realtime factor, the HBM
-roofline of the dominant kernel (spike propagation) and the CPU baseline
-(the oracle's FP32 restatement of the reference step on the host's cores).
+roofline of the dominant kernel (the fused window kernel), the CPU baseline
+(the oracle's FP32 restatement of the reference step on the host's cores), and
+the config-1 and batched-sweep legs.  --config 1..5 runs one workload alone;
+--tune runs the autotuner.
 """
 from __future__ import annotations
 
