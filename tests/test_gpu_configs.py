@@ -13,7 +13,11 @@ Bar (per workload):
     spikes and every weight bit-exact for 1000 steps;
   * the batched BF16 sweep against the FP64 oracle: per-population rates of
     every network within 2 % over 1 s, and the V_m error up to the first spike
-    divergence.
+    divergence;
+  * configs 1 and 2 against the FP64 oracle likewise (their exact run equals the
+    GPU's bit for bit): the V_m error up to the first spike divergence below 1 uV
+    and per-population rates within 2 % (or within the counting noise of two
+    realisations, where that is larger) over 1 s.
 Oracle runs use its multithreaded step loop (bit-identical to the sequential
 one, tests/test_oracle_semantics.py).
 """
@@ -89,6 +93,44 @@ def _spikes_and_stats(g, o_lists, bounds):
     return exact_all
 
 
+def _fp64_report(label, make, bounds, nsteps=STEPS):
+    """The exact run -- equal to the GPU's bit for bit, as the caller has checked --
+    against the FP64 oracle from the same initial state: the V_m error up to the first
+    spike divergence (10-step resolution) and each integrating population's rate over
+    nsteps.  Asserted: V_m error below 1 uV (FP32 rounding only), rates within 2 % or
+    within three standard deviations of the difference of two independent spike
+    counts, whichever is larger (past the divergence the two runs are two realisations
+    of the same chaotic network)."""
+    o32, o64 = make(po.ACC_F32), make(po.ACC_F64)
+    done, diverged, verr = 0, None, 0.0
+    while done < nsteps and diverged is None:
+        a = o32.run_mt(10, THREADS, record=True)
+        b = o64.run_mt(10, THREADS, record=True)
+        if H.first_mismatch(a, b) >= 0:
+            diverged = done
+        else:
+            verr = max(verr, float(np.max(np.abs(o32.state(po.F_V).astype(np.float64) - o64.state(po.F_V)))))
+        done += 10
+    if nsteps > done:
+        o32.run_mt(nsteps - done, THREADS)
+        o64.run_mt(nsteps - done, THREADS)
+    c32, c64 = o32.state(po.F_CNT), o64.state(po.F_CNT)
+    rows = []
+    for lo, hi in zip(bounds[1:-1], bounds[2:]):        # the integrating populations (after the Poisson one)
+        n32, n64 = float(c32[lo:hi].sum()), float(c64[lo:hi].sum())
+        err = abs(n32 - n64) / max(n64, 1.0)
+        tol = max(0.02, 3.0 * np.sqrt(2.0 * max(n64, 1.0)) / max(n64, 1.0))
+        rows.append((n64 / (hi - lo) / (nsteps * 1e-4), err, tol))
+    _say(f"  {label} FP32 vs FP64: first spike divergence at step {diverged}, max |V32 - V64| before it "
+         f"{verr * 1e3:.3g} mV; population rates (FP64 Hz, rate error, bound) "
+         f"{[(round(r, 3), round(e * 100, 2), round(t * 100, 2)) for r, e, t in rows]}")
+    assert verr < 1e-6
+    for _r, e, t in rows:
+        assert e <= t
+    o32.close()
+    o64.close()
+
+
 def test_config1_brunel_10k_bernoulli_full_size():
     """8000 E + 2000 I LIFB + 1000 Poisson at p = 0.002 per step, Bernoulli p = 0.1,
     J = 0.1 mV, g = 5, 1.4 ms delays: ~1.1e7 synapses, generated on the device."""
@@ -111,6 +153,7 @@ def test_config1_brunel_10k_bernoulli_full_size():
     assert g.info()["syn_events"] == o.syn_events()
     g.close()
     o.close()
+    _fp64_report("config 1", lambda acc: H.oracle(net, syn, seed=0, acc=acc)[1], bounds)
 
 
 def test_config2_brunel_100k_fixed_indegree_full_size():
@@ -177,6 +220,8 @@ def test_config2_brunel_100k_fixed_indegree_full_size():
     assert g.info()["syn_events"] == o.syn_events()
     g.close()
     o.close()
+    v0 = H.initial_v(net, 0)
+    _fp64_report("config 2", lambda acc: po.OracleSim(cs, types, rate, v0, net.lifconst, seed=0, acc=acc), bounds)
 
 
 # the batched sweep's networks at the corners of the g x nu grid: the two smallest of
