@@ -14,12 +14,12 @@ Bar (per workload):
   * the batched BF16 sweep against the FP64 oracle: per-population rates of
     every network within 2 % over 1 s, and the V_m error up to the first spike
     divergence;
-  * configs 1 and 2 against the FP64 oracle likewise (their exact run equals the
-    GPU's bit for bit): the V_m error up to the first spike divergence below 1 uV
-    and per-population rates within 2 % (or within the counting noise of two
-    realisations, where that is larger) over 1 s.
-Oracle runs use its multithreaded step loop (bit-identical to the sequential
-one, tests/test_oracle_semantics.py).
+  * configs 1, 2 and 4 against the FP64 oracle likewise (their exact run equals
+    the GPU's bit for bit): the V_m error up to the first spike divergence below
+    1 uV and per-population rates within 2 % (or within the counting noise of two
+    realisations, where that is larger) over 1 s (config 4: over its 1000 steps).
+Oracle runs without STDP use its multithreaded step loop (bit-identical to the
+sequential one, tests/test_oracle_semantics.py).
 """
 import os
 import sys
@@ -339,3 +339,41 @@ def test_config4_brunel_100k_stdp_full_size():
     assert changed > 1_000_000
     g.close()
     o.close()
+    del wg, wo
+    # against FP64: the exact run (the fixed-point oracle, equal to the GPU's bit for bit
+    # above) and the FP64 oracle from the same state, to the first spike divergence and
+    # over the 1000 steps
+    ofx = po.OracleSim(cs, types, rate, H.initial_v(net, 0), net.lifconst, stdpconst=sc, stdp=True, seed=0,
+                       acc=po.ACC_FIXED, scale_exc=info["fix_scale_exc"], scale_inh=info["fix_scale_inh"])
+    o64 = po.OracleSim(cs, types, rate, H.initial_v(net, 0), net.lifconst, stdpconst=sc, stdp=True, seed=0,
+                       acc=po.ACC_F64)
+    done, diverged, verr = 0, None, 0.0
+    while done < EXACT and diverged is None:
+        a, b = ofx.run(10), o64.run(10)
+        if H.first_mismatch(a, b) >= 0:
+            diverged = done
+        else:
+            verr = max(verr, float(np.max(np.abs(ofx.state(po.F_V).astype(np.float64) - o64.state(po.F_V)))))
+        done += 10
+    if EXACT > done:
+        ofx.run(EXACT - done)
+        o64.run(EXACT - done)
+    cf, c64 = ofx.state(po.F_CNT), o64.state(po.F_CNT)
+    rows = []
+    for lo, hi in zip(bounds_of(net)[1:-1], bounds_of(net)[2:]):
+        nf, n64 = float(cf[lo:hi].sum()), float(c64[lo:hi].sum())
+        rows.append((n64 / (hi - lo) / (EXACT * 1e-4), abs(nf - n64) / max(n64, 1.0),
+                     max(0.02, 3.0 * np.sqrt(2.0 * max(n64, 1.0)) / max(n64, 1.0))))
+    _say(f"  config 4 exact vs FP64: first spike divergence at step {diverged}, max |V - V64| before it "
+         f"{verr * 1e3:.3g} mV; rates over "
+         f"{EXACT} steps (FP64 Hz, rate error %, bound %) "
+         f"{[(round(r, 3), round(e * 100, 2), round(t * 100, 2)) for r, e, t in rows]}")
+    assert verr < 1e-6
+    for _r, e, t in rows:
+        assert e <= t
+    ofx.close()
+    o64.close()
+
+
+def bounds_of(net):
+    return np.cumsum([0] + [p[0] for p in net.pops])
