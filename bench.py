@@ -14,7 +14,8 @@ steps; a --long run beside it gives the steady-state figures and the roofline.
 Prints one JSON line: synaptic events/s (value), realtime factor, the HBM
 roofline of the dominant kernel (the fused window kernel), the CPU baseline
 (the oracle's FP32 restatement of the reference step on the host's cores), and
-the config-1 and batched-sweep legs.  --config 1..5 runs one workload alone;
+the config-1, batched-sweep, conductance-network (config 3) and STDP (config 4)
+legs, 1 s of simulated time each.  --config 1..5 runs one workload alone;
 --tune runs the autotuner.
 """
 from __future__ import annotations
@@ -541,6 +542,8 @@ def main(argv=None):
                          "from it); 0 = none")
     ap.add_argument("--no-config1", action="store_true", help="skip the config-1 leg of the default run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg of the default run")
+    ap.add_argument("--no-config3", action="store_true", help="skip the conductance-network leg of the default run")
+    ap.add_argument("--no-config4", action="store_true", help="skip the STDP leg of the default run")
     ap.add_argument("--no-batch-bf16", action="store_true", help="skip the BF16 run of the batched leg")
     ap.add_argument("--no-sequential", action="store_true",
                     help="skip the batched leg's one-network-at-a-time comparison (profiling runs)")
@@ -651,6 +654,16 @@ def main(argv=None):
     batched = None
     if args.config == 2 and not args.no_batch:
         batched = batched_leg(args, batch_shapes, log)
+    # the conductance-based network (config 3) and STDP (config 4), 1 s each
+    coba = stdp = None
+    if args.config == 2 and not args.no_config3:
+        a3 = argparse.Namespace(**vars(args))
+        a3.steps, a3.warmup = 10000, 100
+        coba = coba_leg(a3, log)
+    if args.config == 2 and not args.no_config4:
+        a4 = argparse.Namespace(**vars(args))
+        a4.steps, a4.warmup = 10000, 1000
+        stdp = stdp_leg(a4, log)
 
     out = {
         "metric": "syn. events/s & realtime factor, 1 MI355X; plain vs batched-BF16 @<=2% rate err",
@@ -675,6 +688,8 @@ def main(argv=None):
         "cpu_baseline": r["cpu_baseline"],
         "config1": side,
         "batched": batched,
+        "config3": coba,
+        "config4": stdp,
     }
     print(json.dumps(out))
     return 0
