@@ -132,6 +132,7 @@ struct mgb_sim {
     uint64_t spin_limit = 0;           // wall-clock ticks
     bool failed = false;               // a persistent launch timed out: results void
     uint32_t* h_err = nullptr;         // pinned host copy of *bar_err, written at the end of each run
+    uint32_t* h_err_dev = nullptr;     // its address on the device (the run's last kernel writes it)
     bool err_pending = false;          // h_err is due with the stream's completion
     int xcd_map = -1;                  // fused launches: XCD-contiguous tile order (-1: not computed yet)
     uint64_t tile_hash = 0;            // FNV-1a of the tile boundaries (the ring slices' layout)
@@ -1157,8 +1158,9 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
             khz = 100000;
         const uint64_t us = (o && o->spin_limit_us > 0) ? (uint64_t)o->spin_limit_us : 2000000ull;
         s->spin_limit = us * (uint64_t)khz / 1000ull;
-        MGB_HIP_TRY(hipHostMalloc((void**)&s->h_err, 4, hipHostMallocDefault));
+        MGB_HIP_TRY(hipHostMalloc((void**)&s->h_err, 4, hipHostMallocMapped));
         *s->h_err = 0u;
+        MGB_HIP_TRY(hipHostGetDevicePointer((void**)&s->h_err_dev, s->h_err, 0));
     }
     if (s->tiled) update_persist(s);
     MGB_HIP_TRY(hipStreamSynchronize(st));
@@ -1345,7 +1347,8 @@ mgb_status issue_windows(mgb_sim* s, int32_t nsteps, bool finalize, bool capturi
             ++wk;
         }
         if (finalize) {
-            MGB_HIP_TRY(launch_finalize_tiled(s->ctl, nsteps, wk, s->stream));
+            MGB_HIP_TRY(launch_finalize_tiled(s->ctl, nsteps, wk, s->persist ? s->bar_err : nullptr,
+                                              s->persist ? s->h_err_dev : nullptr, s->stream));
             s->launches += 1;
         }
         return MGB_OK;
@@ -1827,11 +1830,9 @@ mgb_status mgb_run(mgb_sim* s, int32_t nsteps) {
         }
     }
     if (nsteps - done > 0) MGB_TRY(issue_windows(s, nsteps - done, true));
-    if (s->persist && s->h_err) {
-        // the wait flag's error word comes back with the run (read by mgb_sync)
-        MGB_HIP_TRY(hipMemcpyAsync(s->h_err, s->bar_err, 4, hipMemcpyDeviceToHost, s->stream));
-        s->err_pending = true;
-    }
+    // the waits' error word comes back with the run (the finalize kernel writes it to
+    // pinned host memory; mgb_sync reads it)
+    if (s->persist && s->h_err) s->err_pending = true;
     MGB_HIP_TRY(hipEventRecord(s->ev1, s->stream));
     s->timed = true;
     return MGB_OK;
@@ -2148,6 +2149,8 @@ mgb_status mgb_checkpoint_load(mgb_sim* s, const void* buf, int64_t bytes) {
         if (g.bytes) MGB_HIP_TRY(hipMemcpy(g.p, in + off, g.bytes, hipMemcpyHostToDevice));
         off += g.bytes;
     }
+    // the persistent waits' flags hold window indices of the run before (epochs)
+    if (s->bar) MGB_HIP_TRY(hipMemset(s->bar, 0, 4 * (size_t)s->G * kBarStride));
     return MGB_OK;
 }
 

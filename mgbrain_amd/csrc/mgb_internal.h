@@ -279,12 +279,16 @@ hipError_t launch_tile_lean(const int32_t* tile_lo, int32_t G, const uint8_t* ty
 int tiled_blocks_per_cu(const TiledArgs& a, int block, bool persistent = false);
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st);
-// Persistent form (not kTStdp): nwin windows of nsteps steps in one launch, after a
-// reset of the tiles' wait flags; every workgroup must be resident
-// (tiled_blocks_per_cu(a, block, true) x CUs >= G).
+// Persistent form (not kTStdp): nwin windows of nsteps steps in one launch; every
+// workgroup must be resident (tiled_blocks_per_cu(a, block, true) x CUs >= G).  The
+// wait after window k publishes and waits for epoch k + 1 (the window's index over
+// the simulator's life), so flags need no reset between launches, only when the
+// window index goes back (a restored checkpoint).
 hipError_t launch_tiled_persistent(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int32_t nwin,
                                    int block, hipStream_t st);
-hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st);
+// err, h_err: the persistent waits' error word and its pinned host copy (or null)
+hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, const uint32_t* err, uint32_t* h_err,
+                                 hipStream_t st);
 hipError_t set_tiled_lds_limit(int kind, size_t bytes);
 // kTStdp: brings the plastic in-synapses of the neurons the fused kernel listed
 // in window base_window + wk up to date (flush of deferred dendrite updates).

@@ -569,11 +569,6 @@ __device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t G, uint32_t g,
     return __syncthreads_or(bad) == 0;
 }
 
-__global__ void k_bar_reset(uint32_t* bar, uint32_t G) {
-    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < G; h += gridDim.x * blockDim.x)
-        st_agent(&bar[(size_t)h * kBarStride], 0u);
-}
-
 // One window of the fused tiled step (CLASS format).  Workgroup g (targets
 // [tile_lo[g], tile_lo[g+1]) of segment tile_seg[g]):
 //   1. loads its ring slice ring_t[g] (D x B words) into LDS;
@@ -1258,7 +1253,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         }
         if constexpr (PW) {
             if (kw + 1 < nw &&
-                !grid_wait(a.bar, (uint32_t)a.G, (uint32_t)g, (uint32_t)(kw + 1), a.bar_err, a.spin_limit))
+                !grid_wait(a.bar, (uint32_t)a.G, (uint32_t)g, (uint32_t)(kwin + 1), a.bar_err, a.spin_limit))
                 return;
         }
     }   // windows
@@ -1350,9 +1345,12 @@ __global__ __launch_bounds__(256) void k_stdp_flush(TiledArgs a, int32_t off, in
     }
 }
 
-__global__ void k_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows) {
+// End of a run: the step and window counters move on, and the persistent waits'
+// error word goes to the host (pinned memory the host reads after the run).
+__global__ void k_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, const uint32_t* err, uint32_t* h_err) {
     ctl->base_step += steps;
     ctl->base_window += windows;
+    if (err && h_err) __hip_atomic_store(h_err, *err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace
@@ -1494,9 +1492,6 @@ hipError_t launch_tiled_persistent(const TiledArgs& a, int32_t off, int32_t wk, 
                                    int block, hipStream_t st) {
     if (a.kind == kTStdp || !a.bar || !a.bar_err) return hipErrorInvalidValue;
     const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, 0);
-    hipLaunchKernelGGL(k_bar_reset, dim3((a.G + 255) / 256), dim3(256), 0, st, a.bar, (uint32_t)a.G);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     return launch_fn(tiled_fn_of(a.kind, a.chunk, true), a, off, wk, nsteps, nwin, block, lds, st);
 }
 
@@ -1528,8 +1523,9 @@ hipError_t launch_stdp_flush(const TiledArgs& a, int32_t off, int32_t wk, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t st) {
-    hipLaunchKernelGGL(k_finalize_tiled, dim3(1), dim3(1), 0, st, ctl, steps, windows);
+hipError_t launch_finalize_tiled(Ctl* ctl, int32_t steps, int32_t windows, const uint32_t* err, uint32_t* h_err,
+                                 hipStream_t st) {
+    hipLaunchKernelGGL(k_finalize_tiled, dim3(1), dim3(1), 0, st, ctl, steps, windows, err, h_err);
     return hipGetLastError();
 }
 
