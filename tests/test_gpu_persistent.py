@@ -141,19 +141,22 @@ def test_persistent_run_against_the_oracle():
     o.close()
 
 
-def test_wait_beyond_its_bound_fails_the_run():
+@pytest.mark.parametrize("graph", [0, 1])
+def test_wait_beyond_its_bound_fails_the_run(graph):
     # 8192 Poisson neurons firing every step, all projecting to 512 LIFB neurons: each
     # LIFB tile adds millions of synapses per window, the Poisson tiles none, so the
     # Poisson tiles wait far longer than the 1 us bound at the first window's end
-    _say("case: unequal tiles, spin_limit_us=1")
+    # (graph: the run's persistent launch and the kernel that reports the error word
+    # replayed from a captured graph)
+    _say(f"case: unequal tiles, spin_limit_us=1, graph={graph}")
     pops = [dict(num=8192, type=N.POISSON, rate=1.0), dict(num=512, type=N.LIFB, rate=0.0)]
     projs = [dict(src=0, tar=1, kind=N.CONN_BERNOULLI, p=1.0, wmin=1e-6, wmax=1e-6, dmin=1e-3, dmax=1e-3,
                   index=0, seed=1)]
     g = N.Simulator.generated(pops, projs, W.lifb_consts(), seed=1, propagation=N.PROP_TILED, tile_size=512,
-                              spin_limit_us=1)
+                              spin_limit_us=1, use_graph=graph)
     info = g.info()
     assert info["persistent"] == 1 and info["tiles"] >= 2
-    g.run(100)
+    g.run(600 if graph else 100)
     with pytest.raises(N.MGBError) as e:
         g.sync()
     assert e.value.status == N.ERR_HIP and "bound" in str(e.value)
