@@ -271,15 +271,16 @@ def batched_leg(args, shapes=None, log=None):
 
 
 def stdp_leg(args, log):
-    """Brunel 100k (K = 10,000) with pair-based STDP on the E->E synapses.  --stdp-path
-    fused (default): k_tiled<CH, kTStdp> ("k_tiled_stdp" below) + k_stdp_flush, one launch
-    each per window of min(min_delay, shortest inter-spike interval of E neurons) = 10
-    steps, all weights FP32.  direct: a window of one step (k_neurons + k_prop + k_stdp per
-    step), static weights in the format chosen by --precision, plastic weights FP32."""
+    """Brunel 100k (K = 10,000) with pair-based STDP on the E->E synapses, static weights
+    (E->I, I->*, Poisson->*) in the format chosen by --precision (BF16 by default), plastic
+    weights FP32.  --stdp-path fused (default): k_tiled<CH, kTStdpB> (kTStdp with
+    --precision f32; "k_tiled_stdp" below) + k_stdp_flush, one launch each per window of
+    min(min_delay, shortest inter-spike interval of E neurons) = 10 steps.  direct: a window
+    of one step (k_neurons + k_prop + k_stdp per step)."""
     pops, projs, desc, ntot = W.brunel_fixed_k(args.scale, seed=args.seed, plastic_ee=True)
     sc, lc = W.stdp_consts(), W.lifb_consts()
     fused = args.stdp_path == "fused"
-    wf = N.WF_F32 if fused or args.precision != "bf16" else N.WF_BF16
+    wf = N.WF_BF16 if args.precision == "bf16" else N.WF_F32
     prop = N.PROP_TILED if fused else N.PROP_DIRECT
     t0 = time.perf_counter()
     shape = {}
@@ -477,13 +478,14 @@ def tune_coba(args, log):
 
 
 def tune_stdp(args, log):
-    """Autotunes the fused STDP path's launch shapes (FP32 words)."""
+    """Autotunes the fused STDP path's launch shapes (static weights as --precision)."""
     pops, projs, _desc, _ntot = W.brunel_fixed_k(args.scale, seed=args.seed, plastic_ee=True)
     sc, lc = W.stdp_consts(), W.lifb_consts()
 
     def make(tile):
         return N.Simulator.generated(pops, projs, lc, step=DT, dt=DT, seed=args.seed, stdp=True, stdpconst=sc,
-                                     weight_format=N.WF_F32, propagation=N.PROP_TILED, tile_size=tile)
+                                     weight_format=N.WF_BF16 if args.precision == "bf16" else N.WF_F32,
+                                     propagation=N.PROP_TILED, tile_size=tile)
     probe = make(0)
     pinfo = probe.info()
     probe.close()
@@ -533,7 +535,8 @@ def main(argv=None):
                     help="static weight storage of --config 4")
     ap.add_argument("--nets", type=int, default=64, help="networks in the batched sweep")
     ap.add_argument("--stdp-path", default="fused", choices=("fused", "direct"),
-                    help="--config 4: fused window kernel (FP32 weights) or the direct kernels")
+                    help="--config 4: fused window kernel or the direct kernels (static weights as "
+                         "--precision, plastic weights FP32 on both)")
     ap.add_argument("--settle", type=int, default=2000,
                     help="untimed steps run first, so that the timed steps see the network past its start-up "
                          "transient (default 2000 = 200 ms)")

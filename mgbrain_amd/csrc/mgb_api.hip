@@ -62,7 +62,7 @@ struct mgb_sim {
     uint8_t *type = nullptr, *cset = nullptr;
     uint16_t *tbl = nullptr, *stream_idx = nullptr;
     float *rate = nullptr, *V = nullptr, *Ie = nullptr, *Ii = nullptr;
-    int32_t *refr = nullptr, *last = nullptr, *cnt = nullptr;   // kTStdp: last is [3][npad]
+    int32_t *refr = nullptr, *last = nullptr, *cnt = nullptr;   // fused STDP: last is [3][npad]
     CSet* csets = nullptr;
     int32_t* stream_base = nullptr;
     uint64_t* stream_seed = nullptr;
@@ -84,7 +84,7 @@ struct mgb_sim {
     float* tbl_w = nullptr;
     float *ltp = nullptr, *ltd = nullptr;
     int32_t lut_n = 1;
-    int32_t lut_lds = 0;               // kTStdp: table entries the fused kernel keeps in LDS
+    int32_t lut_lds = 0;               // fused STDP: table entries the fused kernel keeps in LDS
     uint32_t* win_buf = nullptr;
     uint32_t win_stride = 0;
     uint64_t* rec = nullptr;
@@ -105,21 +105,23 @@ struct mgb_sim {
     uint64_t n_offs = 0, n_keys = 0;
     uint32_t* tcnt = nullptr;          // [3][nseg]
     int tkind = kTClass;               // synapse word kind of the fused kernel
-    uint32_t* tperm = nullptr;         // kTStdp: CSR slot of each tile-order position (~0u: padding)
+    uint32_t* tperm = nullptr;         // fused STDP: CSR slot of each tile-order position (~0u: padding)
     uint64_t n_tslots = 0;             // length of the (padded) tile-order arrays
     float2* seg_scale = nullptr;       // fixed-point formats, several segments: (scale, 1 / scale)
     uint16_t* seg_of = nullptr;        // segment of each neuron (with seg_scale)
-    int32_t *hist = nullptr, *tf = nullptr, *cntf = nullptr;   // kTStdp: deferred dendrite updates
-    int32_t* flist = nullptr;          // kTStdp: [2][npad][3] neurons to flush
+    int32_t *hist = nullptr, *tf = nullptr, *cntf = nullptr;   // fused STDP: deferred dendrite updates
+    int32_t* flist = nullptr;          // fused STDP: [2][npad][3] neurons to flush
     uint32_t* fcnt = nullptr;          // [2]
-    int32_t stdp_isi = 1;              // kTStdp: shortest inter-spike interval of plastic synapses' ends
+    int32_t stdp_isi = 1;              // fused STDP: shortest inter-spike interval of plastic synapses' ends
     uint32_t* syn_t = nullptr;
     uint16_t* syn16 = nullptr;         // kTClass16 words in tile order (padded sub-rows)
     int64_t* prow = nullptr;           // first slot of each padded row of syn16 / syn_t / w_t
     float* tile_cc = nullptr;          // CLASS kinds: class cache of each tile
     uint32_t* tile_ccls = nullptr;
     uint8_t* tile_lean = nullptr;      // every neuron of the tile an LIFB neuron of its two classes
-    float* w_t = nullptr;              // kTF32 / kTStdp weights in tile order
+    float* w_t = nullptr;              // kTF32 / kTStdp / kTStdpB weights in tile order
+    uint32_t* row_aux = nullptr;       // kTStdpB: row scale bits | has-plastic << 31 per source
+    uint8_t* tile_pl = nullptr;        // kTStdpB: 1 per tile with a plastic in-synapse
     uint16_t* offs16 = nullptr;
     uint32_t* offs32 = nullptr;
     uint32_t* ring_t = nullptr;
@@ -923,17 +925,18 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
         g.col = s->col; g.dly = s->dly; g.w32 = s->w32; g.w16 = s->w16; g.w8 = s->w8;
         g.row_scale = s->row_scale; g.pos_of = pos_of;
         MGB_HIP_TRY(gather_wide(g, st));
-        // fused path for delays that fit the 5-bit field; with STDP, FP32 weights
+        // fused path for delays that fit the 5-bit field; with STDP, FP32 weights or
+        // BF16 static ones
         const int prop = o ? o->propagation : MGB_PROP_AUTO;
-        const bool tiled_ok = max_d <= kClassMaxDelay && (!h.stdp || s->wide_kind == kWideF32);
+        const bool tiled_ok = max_d <= kClassMaxDelay && (!h.stdp || s->wide_kind != kWideFP8);
         if (prop == MGB_PROP_TILED && !tiled_ok) {
-            set_error("tiled propagation of the wide formats needs delays <= %d steps (and FP32 weights with "
-                      "STDP)", kClassMaxDelay);
+            set_error("tiled propagation of the wide formats needs delays <= %d steps (and FP32 or BF16 weights "
+                      "with STDP)", kClassMaxDelay);
             return MGB_ERR_UNSUPPORTED;
         }
         s->tiled = (tiled_ok && prop != MGB_PROP_DIRECT) ? 1 : 0;
         if (s->tiled) {
-            s->tkind = h.stdp ? kTStdp
+            s->tkind = h.stdp ? (s->wide_kind == kWideBF16 ? kTStdpB : kTStdp)
                               : (s->wide_kind == kWideBF16 ? kTBF16 : (s->wide_kind == kWideFP8 ? kTFP8 : kTF32));
             // one fixed-point ring when every integrating neuron is LIFB (it integrates
             // the sum of both sides)
@@ -957,14 +960,14 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
                 MGB_HIP_TRY(sort_pairs(temp, &tb2, kin, kout, vin, vout, S, tbits, st));
             }
             MGB_TRY(dalloc_n(s, &s->syn_t, S));
-            if (s->tkind == kTF32 || s->tkind == kTStdp) MGB_TRY(dalloc_n(s, &s->w_t, S));
+            if (s->tkind == kTF32 || stdp_kind(s->tkind)) MGB_TRY(dalloc_n(s, &s->w_t, S));
             MGB_HIP_TRY(tile_gather_wide(vout, S, s->col, s->dly, s->w16, s->w8, s->w32, tt, s->tkind, s->syn_t,
                                          s->w_t, st));
             MGB_TRY(build_tile_offsets(s, n, kout, tt));
-            uint32_t* pmap = nullptr;          // kTStdp: padded position of each tile-order position
-            if (s->tkind == kTStdp) MGB_TRY(X.get(&pmap, std::max<size_t>(S, 1)));
+            uint32_t* pmap = nullptr;          // STDP: padded position of each tile-order position
+            if (stdp_kind(s->tkind)) MGB_TRY(X.get(&pmap, std::max<size_t>(S, 1)));
             MGB_TRY(pad_subrows(s, n, pmap));
-            if (s->tkind == kTStdp) {
+            if (stdp_kind(s->tkind)) {
                 // plastic weights live in (padded) tile order: keep the CSR slot of each
                 // position (read-back; ~0u at padding) and map the CSR slots of the
                 // in-synapse lists below to padded tile positions
@@ -995,7 +998,13 @@ mgb_status build_network(mgb_sim* s, const HostNeurons& h, const DevList& L, int
             MGB_TRY(dalloc_n(s, &s->in_pos, (size_t)s->n_plastic));
             MGB_TRY(dalloc_n(s, &s->in_src, (size_t)s->n_plastic));
             MGB_HIP_TRY(in_fill(vout, pos_of, L.src, (uint64_t)s->n_plastic, s->in_pos, s->in_src, st));
-            if (s->tkind == kTStdp) {
+            if (s->tiled && s->tkind == kTStdpB) {
+                MGB_TRY(dalloc_n(s, &s->row_aux, (size_t)n));
+                MGB_TRY(dalloc_n(s, &s->tile_pl, (size_t)s->G));
+                MGB_HIP_TRY(row_aux(s->row_scale, n, L.src, L.plastic, S, s->row_aux, st));
+                MGB_HIP_TRY(tile_plastic(s->tile_lo, s->G, s->in_ptr, s->tile_pl, st));
+            }
+            if (s->tiled && stdp_kind(s->tkind)) {
                 // window cap: the shortest inter-spike interval of any end of a plastic
                 // synapse (1 step for Poisson neurons, refractory steps + 1 for the
                 // integrating types; neurons of other types never fire)
@@ -1042,7 +1051,7 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     MGB_TRY(dalloc_n(s, &s->Ie, np));
     MGB_TRY(dalloc_n(s, &s->Ii, np));
     MGB_TRY(dalloc_n(s, &s->refr, np));
-    const size_t nlast = (s->tiled && s->tkind == kTStdp) ? 3 : 1;   // kTStdp: three rotating buffers
+    const size_t nlast = (s->tiled && stdp_kind(s->tkind)) ? 3 : 1;   // fused STDP: three rotating buffers
     MGB_TRY(dalloc_n(s, &s->last, nlast * np));
     MGB_TRY(dalloc_n(s, &s->cnt, np));
     MGB_HIP_TRY(hipMemsetAsync(s->ctl, 0, sizeof(Ctl), st));
@@ -1130,7 +1139,7 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
         MGB_TRY(dalloc_n(s, &s->ltd, ltd.size()));
         MGB_HIP_TRY(hipMemcpyAsync(s->ltp, ltp.data(), 4 * ltp.size(), hipMemcpyHostToDevice, st));
         MGB_HIP_TRY(hipMemcpyAsync(s->ltd, ltd.data(), 4 * ltd.size(), hipMemcpyHostToDevice, st));
-        if (s->tiled && s->tkind == kTStdp) {
+        if (s->tiled && stdp_kind(s->tkind)) {
             // the LDS left beside the tile's data holds the tables' leading entries
             int lds_max = 65536;
             (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
@@ -1146,7 +1155,7 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
     s->profile = o ? o->profile : 0;
     s->prop_block = (o && o->prop_block >= 64 && o->prop_block <= 1024) ? (o->prop_block / 64) * 64 : 256;
     s->prop_grid = (o && o->prop_grid > 0) ? o->prop_grid : 0;
-    if (s->tiled && s->tkind != kTStdp) {
+    if (s->tiled && !stdp_kind(s->tkind)) {
         // persistent launches: wait flags (reset before each launch) and the wait's bound
         MGB_TRY(dalloc_n(s, &s->bar, (size_t)s->G * kBarStride));
         MGB_TRY(dalloc_n(s, &s->bar_err, 1));
@@ -1226,7 +1235,7 @@ void update_persist(mgb_sim* s) {
     if (!s->tiled) return;
     const TiledArgs a = tiled_args(s);
     s->xcd_map = a.xcd_map;
-    if (s->tkind == kTStdp || s->persist_req != MGB_PERSIST_AUTO || !s->bar) return;
+    if (stdp_kind(s->tkind) || s->persist_req != MGB_PERSIST_AUTO || !s->bar) return;
     s->persist = resident(s, a, true) ? 1 : 0;
 }
 
@@ -1238,6 +1247,7 @@ TiledArgs tiled_args(const mgb_sim* s) {
     a.inv_e = 1.0f / s->scale_e; a.inv_i = 1.0f / s->scale_i;
     a.seg_scale = s->seg_scale;
     a.w_t = s->w_t; a.row_scale = s->row_scale;
+    a.row_aux = s->row_aux; a.tile_pl = s->tile_pl;
     a.nseg = s->nseg; a.window = s->window;
     a.tile_lo = s->tile_lo; a.tile_seg = s->tile_seg; a.seg_lo = s->seg_lo;
     a.seg_tile0 = s->seg_tile0; a.seg_obase = s->seg_obase; a.tcnt = s->tcnt;
@@ -1252,7 +1262,7 @@ TiledArgs tiled_args(const mgb_sim* s) {
     a.ring_t = s->ring_t; a.ring_slab = s->ring_slab; a.rw = s->trw;
     a.ctl = s->ctl; a.win_buf = s->win_buf; a.win_stride = s->win_stride;
     a.rec = s->rec; a.rec_cap = (uint64_t)s->rec_cap;
-    if (s->tkind == kTStdp) {
+    if (stdp_kind(s->tkind)) {
         a.last3 = s->last; a.last_stride = (size_t)s->npad;
         a.hist = s->hist; a.tf = s->tf; a.cntf = s->cntf;
         a.flist = s->flist; a.fcnt = s->fcnt; a.flist_cap = (uint32_t)s->npad;
@@ -1336,7 +1346,7 @@ mgb_status issue_windows(mgb_sim* s, int32_t nsteps, bool finalize, bool capturi
             MGB_HIP_TRY(launch_tiled(ta, off, wk, nst, s->tblock, s->stream));
             MGB_TRY(g1.end());
             s->launches += 1;
-            if (s->tkind == kTStdp) {
+            if (stdp_kind(s->tkind)) {
                 SpanGuard g2{s, kSpanStdp, prof};
                 MGB_TRY(g2.begin());
                 MGB_HIP_TRY(launch_stdp_flush(ta, off, wk, s->stream));
@@ -1911,6 +1921,7 @@ mgb_status mgb_get_info(mgb_sim* s, mgb_info* out) {
     out->tiles = s->G;
     out->block = s->tiled ? s->tblock : s->prop_block;
     out->segments = std::max<int32_t>(1, (int32_t)s->h_seg_lo.size() - 1);
+    // (kTStdpB: 4, plus the FP32 weight of a plastic synapse)
     out->word_bytes = !s->tiled ? 0 : (s->tkind == kTClass16 ? 2 : (s->tkind == kTF32 || s->tkind == kTStdp ? 8 : 4));
     out->persistent = s->persist;
     out->last_run_ms = 0.0;
@@ -1994,7 +2005,7 @@ mgb_status mgb_read_state(mgb_sim* s, int32_t field, void* out) {
             return MGB_ERR_INVALID;
     }
     MGB_TRY(mgb_sync(s));
-    if (field == MGB_F_LAST && s->tiled && s->tkind == kTStdp) {
+    if (field == MGB_F_LAST && s->tiled && stdp_kind(s->tkind)) {
         // the next window reads buffer base_window % 3
         Ctl c;
         MGB_HIP_TRY(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
@@ -2028,7 +2039,7 @@ struct CkptSeg {
 // Every device array a run changes, in a fixed order.
 std::vector<CkptSeg> ckpt_segments(const mgb_sim* s, uint64_t n_rec) {
     const size_t np = (size_t)s->npad;
-    const size_t nlast = (s->tiled && s->tkind == kTStdp) ? 3 : 1;
+    const size_t nlast = (s->tiled && stdp_kind(s->tkind)) ? 3 : 1;
     std::vector<CkptSeg> v = {{s->V, 4 * np}, {s->Ie, 4 * np}, {s->Ii, 4 * np},
                               {s->refr, 4 * np}, {s->last, 4 * nlast * np}, {s->cnt, 4 * np},
                               {s->ctl, sizeof(Ctl)},
@@ -2043,7 +2054,7 @@ std::vector<CkptSeg> ckpt_segments(const mgb_sim* s, uint64_t n_rec) {
         v.push_back({s->ring_i, 4 * (size_t)s->D * np});
     }
     if (s->stdp) {
-        if (s->tkind == kTStdp && s->tiled) v.push_back({s->w_t, 4 * (size_t)s->n_tslots});
+        if (stdp_kind(s->tkind) && s->tiled) v.push_back({s->w_t, 4 * (size_t)s->n_tslots});
         else v.push_back({s->w32, 4 * (size_t)s->S});
     }
     if (s->hist) {
@@ -2173,7 +2184,7 @@ mgb_status mgb_read_tiles(mgb_sim* s, int32_t* tile_lo) {
 
 namespace {
 
-// kTStdp: the fused kernel applies a window's STDP in the next window's launch.
+// Fused STDP: the kernel applies a window's STDP in the next window's launch.
 // Writes to w32 (CSR order) the weights the reference holds after the last
 // step run: a copy of w_t (in tmp, n_tslots floats) with the pending window's
 // updates applied.
@@ -2266,7 +2277,7 @@ mgb_status mgb_read_csr(mgb_sim* s, int64_t* row_ptr, int32_t* col, int32_t* del
     if (delay)
         for (size_t k = 0; k < S; ++k) delay[k] = (int32_t)(dl[k] & 127u);
     if (weight && S) {
-        if (s->tiled && s->tkind == kTStdp) MGB_TRY(current_weights(s));
+        if (s->tiled && stdp_kind(s->tkind)) MGB_TRY(current_weights(s));
         MGB_HIP_TRY(hipMemcpy(weight, s->w32, 4 * S, hipMemcpyDeviceToHost));
         if (s->wide_kind == kWideBF16) {
             std::vector<uint16_t> w16(S);
@@ -2329,7 +2340,7 @@ mgb_status mgb_read_weights_noalloc(mgb_sim* s, float* out) {
     MGB_TRY(mgb_sync(s));
     const uint64_t S = (uint64_t)s->S;
     if (S == 0) return MGB_OK;
-    const bool pending = s->tiled && s->tkind == kTStdp;   // fused STDP: updates still to apply
+    const bool pending = s->tiled && stdp_kind(s->tkind);   // fused STDP: updates still to apply
     const bool decode = s->mode == kRingClass || s->wide_kind != kWideF32;
     uint64_t need = 0;
     if (pending) need = s->n_tslots;

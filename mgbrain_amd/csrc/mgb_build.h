@@ -92,6 +92,12 @@ hipError_t tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col
 hipError_t invert_slots(const uint32_t* slot, uint64_t S, uint32_t* inv, hipStream_t st);
 hipError_t remap_u32(uint32_t* a, const uint32_t* map, uint64_t np, hipStream_t st);
 hipError_t scatter_weights(const uint32_t* slot, const float* w_t, uint64_t S, float* w32, hipStream_t st);
+// kTStdpB: out[i] = bits of row_scale[i] | 1 << 31 when source i has a plastic
+// synapse (src / plastic: the synapse list); pl[g] = 1 when a plastic synapse
+// (in_ptr: their in-CSR) targets tile g.
+hipError_t row_aux(const float* row_scale, int32_t n, const int32_t* src, const uint8_t* plastic, uint64_t S,
+                   uint32_t* out, hipStream_t st);
+hipError_t tile_plastic(const int32_t* tile_lo, int32_t G, const int64_t* in_ptr, uint8_t* pl, hipStream_t st);
 // Weights of every CSR slot as mgb_read_csr decodes them: CLASS words through the
 // targets' class tables; BF16 (w16) or FP8 (w8) codes times the row scales, plastic
 // slots from w32.

@@ -155,7 +155,8 @@ __global__ void k_tile_keys_csr(const uint32_t* src, const int32_t* col, TileTab
 
 // Fused-path words of the wide formats in tile order (slot[k]: the CSR slot at
 // tile-order position k): local target | delay-1 << 11 | BF16 / FP8 bits << 16
-// (kTStdp: the plastic flag << 16); FP32 weights go to w_t.
+// (kTStdp: the plastic flag << 16; kTStdpB: 0xFFFF for a plastic synapse, else its
+// BF16 bits); FP32 weights go to w_t.
 __global__ void k_tile_gather_wide(const uint32_t* slot, uint64_t S, const int32_t* col, const uint8_t* dly,
                                    const uint16_t* w16, const uint8_t* w8, const float* w32, TileTabs tt,
                                    int kind, uint32_t* syn_t, float* w_t) {
@@ -169,9 +170,29 @@ __global__ void k_tile_gather_wide(const uint32_t* slot, uint64_t S, const int32
         if (kind == kTBF16) payload = w16[p];
         else if (kind == kTFP8) payload = w8[p];
         else if (kind == kTStdp) payload = (uint32_t)(dly[p] >> 7);
+        else if (kind == kTStdpB) payload = (dly[p] & 128u) ? 0xFFFFu : (uint32_t)w16[p];
         syn_t[k] = loc | ((d - 1u) << 11) | (payload << 16);
-        if (kind == kTF32 || kind == kTStdp) w_t[k] = w32[p];
+        if (kind == kTF32 || kind == kTStdp || kind == kTStdpB) w_t[k] = w32[p];
     }
+}
+
+// kTStdpB: out[i] = row scale bits of source i, then bit 31 set where the source's
+// row holds a plastic synapse (list order: src, plastic).
+__global__ void k_row_aux(const float* row_scale, int32_t n, uint32_t* out) {
+    const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) out[i] = __float_as_uint(row_scale[i]) & 0x7FFFFFFFu;
+}
+
+__global__ void k_row_aux_pl(const int32_t* src, const uint8_t* plastic, uint64_t S, uint32_t* out) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S;
+         k += (uint64_t)gridDim.x * blockDim.x)
+        if (plastic[k]) atomicOr(&out[src[k]], 0x80000000u);
+}
+
+// kTStdpB: pl[g] = 1 when a plastic synapse targets a neuron of tile g.
+__global__ void k_tile_pl(const int32_t* tile_lo, int32_t G, const int64_t* in_ptr, uint8_t* pl) {
+    const int32_t g = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g < G) pl[g] = in_ptr[tile_lo[g + 1]] > in_ptr[tile_lo[g]] ? 1 : 0;
 }
 
 // inv[slot[k]] = k: the tile-order position of every CSR slot.
@@ -524,6 +545,20 @@ hipError_t invert_slots(const uint32_t* slot, uint64_t S, uint32_t* inv, hipStre
 hipError_t remap_u32(uint32_t* a, const uint32_t* map, uint64_t np, hipStream_t st) {
     if (np == 0) return hipSuccess;
     hipLaunchKernelGGL(k_remap, dim3(blocks_for(np, 256)), dim3(256), 0, st, a, map, np);
+    return hipGetLastError();
+}
+
+hipError_t row_aux(const float* row_scale, int32_t n, const int32_t* src, const uint8_t* plastic, uint64_t S,
+                   uint32_t* out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_row_aux, dim3((n + 255) / 256), dim3(256), 0, st, row_scale, n, out);
+    if (S) hipLaunchKernelGGL(k_row_aux_pl, dim3(blocks_for(S, 256)), dim3(256), 0, st, src, plastic, S, out);
+    return hipGetLastError();
+}
+
+hipError_t tile_plastic(const int32_t* tile_lo, int32_t G, const int64_t* in_ptr, uint8_t* pl, hipStream_t st) {
+    if (G <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_pl, dim3((G + 255) / 256), dim3(256), 0, st, tile_lo, G, in_ptr, pl);
     return hipGetLastError();
 }
 

@@ -152,7 +152,12 @@ struct StdpArgs {
 // tiling with B <= 1024: tile-local target (10 bits) | delay-1 << 10 | sign << 15.
 // kTStdp: the kTF32 word with a plastic flag at bit 16, FP32 weights in tile
 // order (w_t, updated in place), pair-based STDP inside the kernel.
-enum TiledKind { kTClass = 0, kTBF16 = 1, kTFP8 = 2, kTF32 = 3, kTClass16 = 4, kTStdp = 5 };
+// kTStdpB: kTStdp with BF16 static weights: bits 16-31 of the word hold the BF16
+// code of a static synapse's w / row scale, or 0xFFFF for a plastic synapse,
+// whose FP32 weight is in w_t (static positions of w_t are never read).
+enum TiledKind { kTClass = 0, kTBF16 = 1, kTFP8 = 2, kTF32 = 3, kTClass16 = 4, kTStdp = 5, kTStdpB = 6 };
+// The fused kinds with STDP.
+inline bool stdp_kind(int k) { return k == kTStdp || k == kTStdpB; }
 
 struct TiledArgs {
     int32_t n, D, B, G;
@@ -160,8 +165,13 @@ struct TiledArgs {
     int32_t kind;           // TiledKind
     float scale_e, scale_i, inv_e, inv_i;   // fixed-point scales of the wide kinds
     const float2* seg_scale;                // per segment (scale, 1 / scale); null: the above
-    float* w_t;             // kTF32 / kTStdp: weights in tile order
-    const float* row_scale; // kTFP8: per-source scale
+    float* w_t;             // kTF32 / kTStdp / kTStdpB: weights in tile order
+    const float* row_scale; // kTFP8 / kTBF16: per-source scale
+    // kTStdpB: per source, its row scale's bits | 1 << 31 when its row holds a
+    // plastic synapse; per tile, 1 when a plastic synapse targets it (the kernel
+    // loads w_t only for spikes of such rows into such tiles)
+    const uint32_t* row_aux;
+    const uint8_t* tile_pl;
     int32_t nseg;
     int32_t window;         // spike-list entries reserved per neuron (steps per window)
     const int32_t* tile_lo;     // [G+1]

@@ -589,7 +589,10 @@ __device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t G, uint32_t g,
 // arrivals add rintf(w * scale) into int32 exc / inh rings, as k_prop does.
 //
 // kTStdp: the kTF32 word with a plastic flag at bit 16 and pair-based STDP
-// (mgcore.cu:346-412) on w_t.  A synapse k -> j lives in the tile of its
+// (mgcore.cu:346-412) on w_t.  kTStdpB: the same with BF16 static weights
+// (times the source row's scale) in the word and 0xFFFF there for a plastic
+// synapse; w_t is read only for spikes of rows with a plastic synapse into
+// tiles with a plastic in-synapse (row_aux, tile_pl).  A synapse k -> j lives in the tile of its
 // target, so every read and update of it happens in one workgroup.  The host
 // caps the window at the shortest inter-spike interval of the plastic
 // synapses' ends: k and j fire at most once in a window.  Axon updates (k's
@@ -615,11 +618,11 @@ __device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t G, uint32_t g,
 // slice stays in LDS and, for tiles of at most one neuron per thread, the
 // neuron state in registers, from the first window to the last; the window
 // lists pass between workgroups through agent-scope accesses and a grid_wait
-// between windows.  Not for kTStdp.
+// between windows.  Not with STDP.
 template <int CH, int WK, bool PW>
 __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_t wk, int32_t nsteps,
                                                 int32_t nwin) {
-    static_assert(!(PW && WK == kTStdp), "the persistent form has no STDP");
+    static_assert(!(PW && (WK == kTStdp || WK == kTStdpB)), "the persistent form has no STDP");
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ uint32_t wsum[16];
     __shared__ unsigned long long se_blk, lsp_r;
@@ -643,7 +646,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         return x * qx + min(x, rx) + b / kXcds;
     }();
     constexpr bool kCls = WK == kTClass || WK == kTClass16;  // CLASS arrival counts
-    constexpr bool kPl = WK == kTStdp;                      // STDP on w_t
+    constexpr bool kMix = WK == kTStdpB;                    // BF16 static weights beside plastic ones
+    constexpr bool kPl = WK == kTStdp || kMix;              // STDP on w_t
     constexpr bool kAux = WK == kTFP8 || WK == kTBF16 || kPl;
     const int RW = kCls ? 1 : a.rw;                         // ring words per (slot, target)
     uint32_t* ring = lds;                                   // [RW][D][B]
@@ -652,10 +656,12 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     uint32_t* sp_start = sp_len + kTiledBatch;              // exclusive prefix of work items
     constexpr uint32_t kItems = kTiledItems;   // work items staged per round
     uint16_t* itm = reinterpret_cast<uint16_t*>(sp_start + kTiledBatch);   // [kItems] spike of item
-    // [kTiledBatch] BF16/FP8: row scale bits; kTStdp: the source's last spike before the window
+    // [kTiledBatch] BF16/FP8: row scale bits; STDP: the source's last spike before the window
     uint32_t* sp_aux = reinterpret_cast<uint32_t*>(itm + kItems);
-    // kTStdp, per tile neuron: spike history [B][kStdpHist], fire count, tf
-    int32_t* hL = reinterpret_cast<int32_t*>(sp_aux + kTiledBatch);
+    // [kTiledBatch] kTStdpB: row scale bits | weights-to-load << 31
+    uint32_t* sp_aux2 = sp_aux + kTiledBatch;
+    // STDP, per tile neuron: spike history [B][kStdpHist], fire count, tf
+    int32_t* hL = reinterpret_cast<int32_t*>(sp_aux + (kMix ? 2 : 1) * kTiledBatch);
     int32_t* cL = hL + (size_t)B * kStdpHist;
     int32_t* tfL = cL + B;
     // the first lut_lds entries of the LTP and LTD tables
@@ -673,6 +679,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const float2 fsc = a.seg_scale ? a.seg_scale[a.tile_seg[g]] : make_float2(a.scale_e, a.inv_e);
     const int32_t t0 = a.tile_lo[g];
     const int32_t tn = a.tile_lo[g + 1] - t0;
+    // kTStdpB: a tile without plastic in-synapses never loads w_t
+    const uint32_t plmask = (kMix && a.tile_pl[g]) ? 0xFFFFFFFFu : 0x7FFFFFFFu;
     // the tile's segment (one segment: all of it known without loads)
     const bool one = a.nseg == 1;
     const int32_t q = one ? 0 : (int32_t)a.tile_seg[g];
@@ -763,9 +771,10 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         const int32_t nprev = (int32_t)min(s0 - sprev, (int64_t)D);
         const int32_t rprev = (int32_t)(sprev % D);
         // metadata of list entry e: first slot of its sub-row in this tile, length |
-        // ring slot of its spike step << 26, BF16 / FP8 row scale bits or (kTStdp)
-        // the source's last spike before the window
-        auto meta = [&](uint32_t ent, uint32_t& beg, uint32_t& lm, uint32_t& aux) {
+        // ring slot of its spike step << 26, BF16 / FP8 row scale bits or (STDP)
+        // the source's last spike before the window; kTStdpB: row scale bits |
+        // (the row holds a plastic synapse and the tile a plastic in-synapse) << 31
+        auto meta = [&](uint32_t ent, uint32_t& beg, uint32_t& lm, uint32_t& aux, uint32_t& aux2) {
             const uint32_t src = ent & kIdMask;
             const size_t ob = obase + (size_t)(src - (uint32_t)slo) * G1;
             uint32_t o0, o1;
@@ -780,13 +789,14 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             if (WK == kTFP8 || WK == kTBF16) aux = __float_as_uint(a.row_scale[src]);
             else if (kPl) aux = (uint32_t)lastA[src];
             else aux = 0u;
+            aux2 = kMix ? (a.row_aux[src] & plmask) : 0u;
         };
         // the first batch's list entries are loaded before their count is known (the
         // index clamped into the segment's part of the list, which holds nq * window
         // entries), and their metadata loads before the ring copy, so that these
         // latencies overlap; entries past the count are never used
-        uint32_t pbeg = 0u, plm = 0u, paux = 0u;
-        if ((uint32_t)tid < min((uint32_t)bd, E)) meta(ent0, pbeg, plm, paux);
+        uint32_t pbeg = 0u, plm = 0u, paux = 0u, paux2 = 0u;
+        if ((uint32_t)tid < min((uint32_t)bd, E)) meta(ent0, pbeg, plm, paux, paux2);
         if (full && kw == 0) ring_slice_in(ring, gring, D, tn, rprev, nprev, RW);
         NState pst;
         int ptype = T_RAW;
@@ -806,11 +816,12 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             const uint32_t nb = min((uint32_t)bd, E - b0);
             uint32_t items = 0u;
             if ((uint32_t)tid < nb) {
-                uint32_t beg = pbeg, lm = plm, aux = paux;
-                if (b0 > 0) meta(PW ? ld_agent(&L[b0 + (uint32_t)tid]) : L[b0 + (uint32_t)tid], beg, lm, aux);
+                uint32_t beg = pbeg, lm = plm, aux = paux, aux2 = paux2;
+                if (b0 > 0) meta(PW ? ld_agent(&L[b0 + (uint32_t)tid]) : L[b0 + (uint32_t)tid], beg, lm, aux, aux2);
                 sp_beg[tid] = beg;
                 sp_len[tid] = lm;
                 if (kAux) sp_aux[tid] = aux;
+                if (kMix) sp_aux2[tid] = aux2;
                 items = ((lm & kIdMask) + CH - 1) / CH;
             }
             const uint32_t incl = block_incl_scan(items, wsum);
@@ -826,7 +837,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                 __syncthreads();
                 // U items per thread in flight: U * CH synapse words loaded before the
                 // first LDS atomic
-                // (kTStdp: at most 8 synapses, whose STDP state then fits in the registers)
+                // (STDP: at most 8 synapses, whose STDP state then fits in the registers)
                 constexpr int U = CH >= 16 ? 1 : (kPl ? (CH >= 8 ? 1 : (CH == 1 ? 4 : 8 / CH)) : 16 / CH);
                 for (uint32_t it = (uint32_t)tid; it < rn; it += (uint32_t)(U * bd)) {
                     // branch-free body, so that all U * CH loads are in flight before the
@@ -835,6 +846,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                     uint32_t cnt[U], kb[U];
                     int32_t sD[U], t2[U], la[U];
                     float rs[U];
+                    bool lw[U];              // kTStdpB: the item's weights are loaded
     #pragma unroll
                     for (int h = 0; h < U; ++h) {
                         const uint32_t q0 = it + (uint32_t)(h * bd);
@@ -847,6 +859,12 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                         kb[h] = sp_beg[j] + c * CH;
                         cnt[h] = (q0 < rn) ? min((uint32_t)CH, len - c * CH) : 0u;
                         rs[h] = (WK == kTFP8 || WK == kTBF16) ? __uint_as_float(sp_aux[j]) : 1.0f;
+                        lw[h] = true;
+                        if (kMix) {
+                            const uint32_t a2 = sp_aux2[j];
+                            rs[h] = __uint_as_float(a2 & 0x7FFFFFFFu);
+                            lw[h] = (a2 >> 31) != 0u;
+                        }
                         if (kPl) {
                             // the spike's step: the window is shorter than the ring, so its
                             // slot determines it
@@ -893,7 +911,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                         }
                         if constexpr (WK != kTClass16 && CH >= 2) {
                             // 32-bit words and FP32 weights: aligned vector loads likewise
-                            // (kTStdp rewrites its weights below: cached loads)
+                            // (STDP rewrites its weights below: cached loads; kTStdpB loads
+                            // them only for rows and tiles with plastic synapses)
                             const uint32_t* p32 = a.syn_t + kb[h];
                             const u32x2* f2 = reinterpret_cast<const u32x2*>(a.w_t + kb[h]);
                             const u32x4* f4 = reinterpret_cast<const u32x4*>(a.w_t + kb[h]);
@@ -902,7 +921,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                                 x[h][0] = v.x;
                                 x[h][1] = v.y;
                                 if (WK == kTF32 || kPl) {
-                                    const u32x2 f = kPl ? *f2 : __builtin_nontemporal_load(f2);
+                                    u32x2 f = {0u, 0u};
+                                    if (lw[h]) f = kPl ? *f2 : __builtin_nontemporal_load(f2);
                                     wv[h][0] = __uint_as_float(f.x);
                                     wv[h][1] = __uint_as_float(f.y);
                                 }
@@ -916,7 +936,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                                     x[h][4 * v4 + 2] = v.z;
                                     x[h][4 * v4 + 3] = v.w;
                                     if (WK == kTF32 || kPl) {
-                                        const u32x4 f = kPl ? f4[v4] : __builtin_nontemporal_load(f4 + v4);
+                                        u32x4 f = {0u, 0u, 0u, 0u};
+                                        if (lw[h]) f = kPl ? f4[v4] : __builtin_nontemporal_load(f4 + v4);
                                         wv[h][4 * v4 + 0] = __uint_as_float(f.x);
                                         wv[h][4 * v4 + 1] = __uint_as_float(f.y);
                                         wv[h][4 * v4 + 2] = __uint_as_float(f.z);
@@ -933,10 +954,10 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                             if (WK == kTClass16) x[h][u] = __builtin_nontemporal_load(&a.syn16[k]);
                             else x[h][u] = __builtin_nontemporal_load(&a.syn_t[k]);
                             if (WK == kTF32) wv[h][u] = __builtin_nontemporal_load(&a.w_t[k]);
-                            else if (kPl) wv[h][u] = a.w_t[k];           // rewritten below: cached load
+                            else if (kPl) wv[h][u] = lw[h] ? a.w_t[k] : 0.0f;   // rewritten below: cached load
                         }
                     }
-                    // kTStdp, per plastic element: j's history newest first gives its last spike
+                    // STDP, per plastic element: j's history newest first gives its last spike
                     // at or before t2 (for the axon update) and the range [mlo, mhi] of
                     // deferred dendrite updates in (lo, t2); the axon table loads of all
                     // elements are then issued together
@@ -953,7 +974,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                                 int32_t lj = 0, mlo = kStdpHist, mhi = 0;
                                 rng[h][u] = -1;
                                 // (words past the item's end are padding or another tile's)
-                                if ((uint32_t)u < cnt[h] && ((xw >> 16) & 1u)) {
+                                const bool plw = kMix ? (xw >> 16) == 0xFFFFu : ((xw >> 16) & 1u) != 0u;
+                                if ((uint32_t)u < cnt[h] && plw) {
                                     const int32_t* hj = hL + loc * kStdpHist;
                                     const int32_t c = cL[loc];
                                     const int32_t lo = max(la[h] == 0 ? -1 : la[h], tfL[loc]);
@@ -1009,6 +1031,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                                 float w;
                                 if (WK == kTBF16) w = bf16_to_f32((uint16_t)(xw >> 16)) * rs[h];
                                 else if (WK == kTFP8) w = fp8e4m3_to_f32((uint8_t)(xw >> 16)) * rs[h];
+                                else if (kMix && (xw >> 16) != 0xFFFFu) w = bf16_to_f32((uint16_t)(xw >> 16)) * rs[h];
                                 else w = wv[h][u];
                                 const bool pl = kPl && rng[h][u] >= 0;
                                 if (pl) {
@@ -1259,7 +1282,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     }   // windows
 }
 
-// Flush of deferred dendrite updates (kTStdp): for each neuron j the fused
+// Flush of deferred dendrite updates (fused STDP): for each neuron j the fused
 // kernel listed in window k, every plastic in-synapse k -> j replays j's spikes
 // in (max(k's last spike, old tf[j]), end of window k-1], oldest first, with
 // dt = k's last spike - t1 (k's last spike 0 counts as "never", as in the fused
@@ -1395,10 +1418,11 @@ hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t
 }
 
 size_t tiled_lds_bytes(int D, int B, int kind, int rw, int lut_lds) {
-    const bool pl = kind == kTStdp;
+    const bool pl = stdp_kind(kind);
     return sizeof(uint32_t) * ((size_t)rw * (size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
            sizeof(uint16_t) * (size_t)kTiledItems +
            ((kind == kTFP8 || kind == kTBF16 || pl) ? sizeof(uint32_t) * kTiledBatch : 0) +
+           (kind == kTStdpB ? sizeof(uint32_t) * kTiledBatch : 0) +
            (pl ? (kStdpHist + 2) * sizeof(int32_t) * (size_t)B + 2 * sizeof(float) * (size_t)lut_lds : 0);
 }
 
@@ -1419,7 +1443,7 @@ const void* tiled_fn_chunk(int chunk) {
     }
 }
 
-// The fused kernel for (kind, chunk); the persistent form exists for every kind but kTStdp.
+// The fused kernel for (kind, chunk); the persistent form exists for every kind without STDP.
 const void* tiled_fn_of(int kind, int chunk, bool pw) {
     switch (kind) {
         case kTBF16: return pw ? tiled_fn_chunk<kTBF16, true>(chunk) : tiled_fn_chunk<kTBF16, false>(chunk);
@@ -1428,6 +1452,7 @@ const void* tiled_fn_of(int kind, int chunk, bool pw) {
         case kTClass16:
             return pw ? tiled_fn_chunk<kTClass16, true>(chunk) : tiled_fn_chunk<kTClass16, false>(chunk);
         case kTStdp: return pw ? nullptr : tiled_fn_chunk<kTStdp, false>(chunk);
+        case kTStdpB: return pw ? nullptr : tiled_fn_chunk<kTStdpB, false>(chunk);
         default: return pw ? tiled_fn_chunk<kTClass, true>(chunk) : tiled_fn_chunk<kTClass, false>(chunk);
     }
 }
@@ -1484,19 +1509,19 @@ hipError_t launch_tile_lean(const int32_t* tile_lo, int32_t G, const uint8_t* ty
 
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st) {
-    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, a.kind == kTStdp ? a.lut_lds : 0);
+    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, stdp_kind(a.kind) ? a.lut_lds : 0);
     return launch_fn(tiled_fn_of(a.kind, a.chunk, false), a, off, wk, nsteps, 1, block, lds, st);
 }
 
 hipError_t launch_tiled_persistent(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int32_t nwin,
                                    int block, hipStream_t st) {
-    if (a.kind == kTStdp || !a.bar || !a.bar_err) return hipErrorInvalidValue;
+    if (stdp_kind(a.kind) || !a.bar || !a.bar_err) return hipErrorInvalidValue;
     const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, 0);
     return launch_fn(tiled_fn_of(a.kind, a.chunk, true), a, off, wk, nsteps, nwin, block, lds, st);
 }
 
 int tiled_blocks_per_cu(const TiledArgs& a, int block, bool persistent) {
-    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, a.kind == kTStdp ? a.lut_lds : 0);
+    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, stdp_kind(a.kind) ? a.lut_lds : 0);
     const void* f = tiled_fn_of(a.kind, a.chunk, persistent);
     if (!f) return 0;
     int nb = 0;

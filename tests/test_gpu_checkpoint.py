@@ -94,15 +94,17 @@ def test_restore_rejects_other_layout():
     b.close()
 
 
-@pytest.mark.parametrize("kind", ["tiled_class", "direct_f32", "tiled_stdp"])
+@pytest.mark.parametrize("kind", ["tiled_class", "direct_f32", "tiled_stdp", "tiled_stdp_bf16"])
 def test_resumed_run_against_the_oracle(kind):
     _say(f"case: checkpointed run against the oracle, {kind}")
     sc, stdp, acc = None, False, po.ACC_F32
-    if kind == "tiled_stdp":
+    if kind.startswith("tiled_stdp"):
+        # (tiled_stdp_bf16: BF16 static weights; this network's rows of equal static
+        # weights are stored exactly)
         net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
         syn = H.host_synapses(net, seed=6)
         sc, stdp, acc = M.stdp_consts_config4(J=1e-4), True, po.ACC_FIXED
-        opts = dict(weight_format=N.WF_F32, propagation=N.PROP_TILED)
+        opts = dict(weight_format=N.WF_BF16 if kind.endswith("bf16") else N.WF_F32, propagation=N.PROP_TILED)
     else:
         net = M.config1(scale=0.3, poisson_rate=0.01)
         syn = H.host_synapses(net, seed=31)

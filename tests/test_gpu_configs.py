@@ -303,16 +303,18 @@ def test_config5_sweep_corners_batched_bf16_against_fp64():
 
 def test_config4_brunel_100k_stdp_full_size():
     """Config 2 with pair-based STDP on the E->E synapses (A+ 0.01, A- 0.0105 x W_max,
-    tau 20 ms, W in [0, 2J]) on the fused path (FP32 weights): 1000 steps against the
-    oracle in fixed point on the read-back CSR -- every spike and all 1e9 weights."""
-    _say("case: config 4 at full size (1e9 synapses, 5.8e8 plastic), fused STDP, 1000 steps")
+    tau 20 ms, W in [0, 2J]) on the fused path with BF16 static weights (E->I, I->*,
+    Poisson->*) and FP32 plastic ones: 1000 steps against the oracle in fixed point on
+    the read-back CSR -- every spike and all 1e9 weights."""
+    _say("case: config 4 at full size (1e9 synapses, 5.8e8 plastic), fused STDP, BF16 static, 1000 steps")
     net = M.config2(scale=1.0, stdp=True)
     sc = M.stdp_consts_config4(J=1e-4)
-    g = H.gpu_generated(net, seed=0, stdp=True, stdpconst=sc, weight_format=N.WF_F32,
+    g = H.gpu_generated(net, seed=0, stdp=True, stdpconst=sc, weight_format=N.WF_BF16,
                         propagation=N.PROP_TILED, record_capacity=4_000_000)
     info = g.info()
     n = info["n_neurons"]
-    assert info["propagation"] == N.PROP_TILED and info["window"] == 10
+    # (4 B words: BF16 static weights in the word, plastic ones beside it)
+    assert info["propagation"] == N.PROP_TILED and info["window"] == 10 and info["word_bytes"] == 4
     nP, nE = net.pops[0][0], net.pops[1][0]
     rp, col, dl, w = g.csr()
     plastic = np.zeros(col.size, np.uint8)

@@ -63,7 +63,8 @@ def test_reference_global_stdp(prop):
 
 
 @pytest.mark.parametrize("wf,prop", [(N.WF_F32, N.PROP_DIRECT), (N.WF_BF16, N.PROP_DIRECT),
-                                     (N.WF_F32, N.PROP_TILED), (N.WF_F32, N.PROP_AUTO)])
+                                     (N.WF_F32, N.PROP_TILED), (N.WF_BF16, N.PROP_TILED),
+                                     (N.WF_F32, N.PROP_AUTO)])
 def test_ee_stdp_benchmark_constants(wf, prop):
     print(f"case: fixed in-degree net, E->E STDP, weight_format={wf}, propagation={prop}", flush=True)
     sys.stdout.flush()
@@ -72,18 +73,51 @@ def test_ee_stdp_benchmark_constants(wf, prop):
     assert 0 < syn.plastic.sum() < syn.n
     sc = M.stdp_consts_config4(J=1e-4)
     if wf == N.WF_BF16:
-        # static weights are stored as BF16 of w / s (s = largest |w| of the source row,
-        # plastic synapses included) and read as bf16 * s; give the oracle the same values
-        stat = syn.plastic == 0
-        smax = np.zeros(net.n, np.float32)
-        np.maximum.at(smax, syn.src, np.abs(syn.weight))
-        sc_row = np.where(smax > 0, smax, np.float32(1.0)).astype(np.float32)[syn.src]
-        x = (syn.weight / sc_row).astype(np.float32)
-        u = x.view(np.uint32).astype(np.uint64)
-        r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
-        syn.weight[stat] = (r * sc_row).astype(np.float32)[stat]
+        _bf16_static(net, syn)
     window = 1 if prop == N.PROP_DIRECT else 10     # min_delay 10, refractory interval 20
     changed, n = _run(net, syn, sc, 800, 6, wf, prop=prop, window=window)
+    assert changed > 100 and n > 1000
+
+
+def _bf16_static(net, syn):
+    """Static weights are stored as BF16 (round to nearest even) of w / s, s the largest
+    |w| of the source row (plastic synapses included), and read as bf16 * s: gives the
+    oracle the same values."""
+    stat = syn.plastic == 0
+    smax = np.zeros(net.n, np.float32)
+    np.maximum.at(smax, syn.src, np.abs(syn.weight))
+    sc_row = np.where(smax > 0, smax, np.float32(1.0)).astype(np.float32)[syn.src]
+    x = (syn.weight / sc_row).astype(np.float32)
+    u = x.view(np.uint32).astype(np.uint64)
+    r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+    syn.weight[stat] = (r * sc_row).astype(np.float32)[stat]
+    return stat
+
+
+@pytest.mark.parametrize("prop,chunk", [(N.PROP_DIRECT, 0), (N.PROP_TILED, 1), (N.PROP_TILED, 2),
+                                        (N.PROP_TILED, 16)])
+def test_stdp_bf16_static_weights_spread(prop, chunk):
+    """BF16 static weights beside FP32 plastic ones (the storage of the STDP benchmark):
+    static weights spread over [0.5, 1.5) x their value, so that the BF16 codes and the
+    per-row scales matter (rows of equal weights are stored exactly).  On the fused path
+    the word holds the BF16 code of a static synapse and marks a plastic one, whose
+    weight is read beside it; chunks 1, 2 and 16 take the scalar, 2-word and 4-word
+    vector loads."""
+    print(f"case: fixed in-degree net, E->E STDP, BF16 static weights spread, propagation={prop}, "
+          f"chunk={chunk}", flush=True)
+    sys.stdout.flush()
+    net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
+    syn = H.host_synapses(net, seed=6)
+    stat = syn.plastic == 0
+    rng = np.random.default_rng(5)
+    syn.weight[stat] = (syn.weight[stat] * rng.uniform(0.5, 1.5, int(stat.sum()))).astype(np.float32)
+    before = syn.weight.copy()
+    _bf16_static(net, syn)
+    assert np.count_nonzero(syn.weight != before) > stat.sum() // 2     # the rounding changed most of them
+    sc = M.stdp_consts_config4(J=1e-4)
+    kw = {} if chunk == 0 else {"chunk": chunk}
+    window = 1 if prop == N.PROP_DIRECT else 10
+    changed, n = _run(net, syn, sc, 800, 6, N.WF_BF16, prop=prop, window=window, **kw)
     assert changed > 100 and n > 1000
 
 

@@ -82,15 +82,16 @@ def test_brunel_shapes(name):
     _against_oracle(g, net, syn, seed=21)
 
 
-def test_stdp_shape():
+@pytest.mark.parametrize("wf", [N.WF_F32, N.WF_BF16])
+def test_stdp_shape(wf):
     name = "brunel100k_K10000_stdp_ee"
     if name not in SHAPES:
         pytest.skip(f"{name} is not in the shapes table")
-    _say(f"case: {name} shape {SHAPES[name]}, E->E STDP, F32 words")
+    _say(f"case: {name} shape {SHAPES[name]}, E->E STDP, weight_format={wf}")
     net = M.config2(scale=0.02, K=200, poisson_rate=0.3, stdp=True)
-    syn = H.host_synapses(net, seed=6)
+    syn = H.host_synapses(net, seed=6)     # (rows of equal static weights: exact in BF16)
     sc = M.stdp_consts_config4(J=1e-4)
-    g = H.gpu_from_list(net, syn, seed=6, stdp=True, stdpconst=sc, weight_format=N.WF_F32,
+    g = H.gpu_from_list(net, syn, seed=6, stdp=True, stdpconst=sc, weight_format=wf,
                         propagation=N.PROP_TILED, record_capacity=4_000_000, **_shape(name))
     assert g.info()["window"] == 10
     _against_oracle(g, net, syn, seed=6, acc=po.ACC_FIXED, stdp=True, stdpconst=sc)
