@@ -110,7 +110,7 @@ struct mgb_sim {
     float2* seg_scale = nullptr;       // fixed-point formats, several segments: (scale, 1 / scale)
     uint16_t* seg_of = nullptr;        // segment of each neuron (with seg_scale)
     int32_t *hist = nullptr, *tf = nullptr, *cntf = nullptr;   // fused STDP: deferred dendrite updates
-    int32_t* flist = nullptr;          // fused STDP: [2][npad][3] neurons to flush
+    int32_t* flist = nullptr;          // fused STDP: [2][npad][2] neurons to flush
     uint32_t* fcnt = nullptr;          // [2]
     int32_t stdp_isi = 1;              // fused STDP: shortest inter-spike interval of plastic synapses' ends
     uint32_t* syn_t = nullptr;
@@ -363,10 +363,12 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D, con
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
     lds_max = std::min(lds_max, 160 * 1024);
     // the kernel's static LDS (block scan sums, event counter) shares the limit
+    // (per target: the difference over 4 targets, which no alignment rounds)
     const int64_t per_b =
-        (int64_t)(tiled_lds_bytes(D, 1, s->tkind, s->trw) - tiled_lds_bytes(D, 0, s->tkind, s->trw));
+        (int64_t)(tiled_lds_bytes(D, 4, s->tkind, s->trw) - tiled_lds_bytes(D, 0, s->tkind, s->trw)) / 4;
     int32_t bmax =
         (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0, s->tkind, s->trw) - kTiledStaticLds) / per_b);
+    while (bmax > 1 && (int64_t)tiled_lds_bytes(D, bmax, s->tkind, s->trw) + kTiledStaticLds > lds_max) --bmax;
     // wide words hold an 11-bit tile-local target; CLASS tiles of at most 1024
     // targets take 16-bit words (else 32-bit words with the global id)
     bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, s->tkind == kTClass ? 1024 : 2048));
@@ -1073,10 +1075,10 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
         MGB_TRY(dalloc_n(s, &s->hist, np * kStdpHist));
         MGB_TRY(dalloc_n(s, &s->tf, np));
         MGB_TRY(dalloc_n(s, &s->cntf, np));
-        MGB_HIP_TRY(hipMemsetAsync(s->hist, 0, 4 * np * kStdpHist, st));
+        MGB_HIP_TRY(hipMemsetAsync(s->hist, 0x80, 4 * np * kStdpHist, st));   // kHistNone
         MGB_HIP_TRY(hipMemsetAsync(s->tf, 0xFF, 4 * np, st));     // -1: no update deferred yet
         MGB_HIP_TRY(hipMemsetAsync(s->cntf, 0, 4 * np, st));
-        MGB_TRY(dalloc_n(s, &s->flist, 2 * 3 * np));
+        MGB_TRY(dalloc_n(s, &s->flist, 2 * 2 * np));
         MGB_TRY(dalloc_n(s, &s->fcnt, 2));
         MGB_HIP_TRY(hipMemsetAsync(s->fcnt, 0, 8, st));
     }

@@ -213,20 +213,20 @@ struct TiledArgs {
     uint32_t win_stride;
     uint64_t* rec;
     uint64_t rec_cap;
-    // kTStdp.  Window k reads its neurons' last spike steps from last3 buffer
+    // kTStdp / kTStdpB.  Window k reads its neurons' last spike steps from last3 buffer
     // k % 3 and writes them to (k + 1) % 3, so buffers (k - 1) % 3 and k % 3
     // hold window k-1's start and end values for the whole launch.  Dendrite
     // updates are deferred: neuron j keeps its last kStdpHist spike steps
-    // (hist, ring indexed by the fire count), its in-synapses are up to date
+    // (hist, newest first: a spike moves the row down one entry), its in-synapses are up to date
     // through step tf[j], and cntf[j] is its fire count at tf[j].
     int32_t* last3;             // [3][last_stride]
     size_t last_stride;
-    int32_t* hist;              // [n][kStdpHist]
+    int32_t* hist;              // [n][kStdpHist] newest first, kHistNone past the spikes
     int32_t* tf;                // [n]
     int32_t* cntf;              // [n]
-    // neurons to flush after window k: list k % 2 of (neuron, fire count, old tf)
-    // triples, fcnt[k % 2] entries; window k clears fcnt[(k + 1) % 2]
-    int32_t* flist;             // [2][flist_cap][3]
+    // neurons to flush after window k: list k % 2 of (neuron, old tf) pairs,
+    // fcnt[k % 2] entries; window k clears fcnt[(k + 1) % 2]
+    int32_t* flist;             // [2][flist_cap][2]
     uint32_t* fcnt;             // [2]
     uint32_t flist_cap;
     const int64_t* din_ptr;     // [n+1] plastic in-synapses of each target
@@ -269,6 +269,9 @@ constexpr int kTiledItems = 16384;  // work items staged in LDS per propagation 
 constexpr int kTiledStaticLds = 1024; // bytes reserved for the fused kernel's static LDS
 constexpr int kClassCache = 32;     // k-fold class sums cached in LDS (tables hold >= this many)
 constexpr int kStdpHist = 16;       // spike steps kept per neuron for deferred dendrite updates
+// A history entry past the neuron's spikes (memset byte 0x80): below every step and
+// every lower bound of a history scan (>= -1).
+constexpr int32_t kHistNone = (int32_t)0x80808080;
 // Every (source, tile) sub-row of the tile-order copy starts at a multiple of
 // kSubAlign words (16 B of CLASS16, 32 B of 32-bit words), so that a
 // work item of 2-16 words is aligned vector loads

@@ -599,13 +599,13 @@ __device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t G, uint32_t g,
 // spike at t2: dt = t2 - j's last spike at t2) are made where the weight is
 // read for propagation.  Dendrite updates (j's spike at t1: dt = k's last
 // spike at t1 - t1) are deferred: neuron j keeps its last kStdpHist spike
-// steps, and a synapse is brought up to date, replaying j's spikes in step
+// steps (newest first), and a synapse is brought up to date, replaying j's spikes in step
 // order, (a) where it is read at k's next spike t2 — j's spikes after
 // max(k's previous spike, tf[j]) and before t2, before the weight is
 // propagated, which is the reference's order (each step: propagation, then
 // STDP) — and (b) when j is flushed: after j has fired kStdpHist - 1 times since
 // tf[j], all of j's plastic in-synapses are brought up to date through the
-// previous window and tf[j] moves there, so the ring always holds the spikes
+// previous window and tf[j] moves there, so the history always holds the spikes
 // needed.  k's last spike 0 counts as "never" (a spike at step 0 then still
 // replays; dt = 0 only clips, which is idempotent).  Window k's neurons read
 // their last spike steps from last3 buffer k % 3 and write them to (k+1) % 3.
@@ -651,7 +651,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     constexpr bool kAux = WK == kTFP8 || WK == kTBF16 || kPl;
     const int RW = kCls ? 1 : a.rw;                         // ring words per (slot, target)
     uint32_t* ring = lds;                                   // [RW][D][B]
-    uint32_t* sp_beg = ring + (size_t)RW * D * B;           // [kTiledBatch] first CSR slot
+    // (STDP: rounded up to 16 bytes, for the history rows' vector reads below)
+    uint32_t* sp_beg = ring + (kPl ? (((size_t)RW * D * B + 3) & ~(size_t)3) : (size_t)RW * D * B);
     uint32_t* sp_len = sp_beg + kTiledBatch;                // len | slot-of-spike << 26
     uint32_t* sp_start = sp_len + kTiledBatch;              // exclusive prefix of work items
     constexpr uint32_t kItems = kTiledItems;   // work items staged per round
@@ -660,7 +661,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     uint32_t* sp_aux = reinterpret_cast<uint32_t*>(itm + kItems);
     // [kTiledBatch] kTStdpB: row scale bits | weights-to-load << 31
     uint32_t* sp_aux2 = sp_aux + kTiledBatch;
-    // STDP, per tile neuron: spike history [B][kStdpHist], fire count, tf
+    // STDP, per tile neuron: spike history [B][kStdpHist] (newest first, kHistNone past
+    // the neuron's spikes; 64-byte rows), fire count, tf
     int32_t* hL = reinterpret_cast<int32_t*>(sp_aux + (kMix ? 2 : 1) * kTiledBatch);
     int32_t* cL = hL + (size_t)B * kStdpHist;
     int32_t* tfL = cL + B;
@@ -976,23 +978,23 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                                 // (words past the item's end are padding or another tile's)
                                 const bool plw = kMix ? (xw >> 16) == 0xFFFFu : ((xw >> 16) & 1u) != 0u;
                                 if ((uint32_t)u < cnt[h] && plw) {
-                                    const int32_t* hj = hL + loc * kStdpHist;
-                                    const int32_t c = cL[loc];
+                                    // j fires at most once a window: only its newest spike can be
+                                    // at or after t2 (then the one before is j's last spike before
+                                    // t2), and the entries after lo are a prefix of the history,
+                                    // read four at a time
+                                    const int4* hv = reinterpret_cast<const int4*>(hL + loc * kStdpHist);
                                     const int32_t lo = max(la[h] == 0 ? -1 : la[h], tfL[loc]);
-                                    const int32_t nm = min(c, kStdpHist);
-                                    bool found = false;
-                                    for (int32_t m = 0; m < nm; ++m) {
-                                        const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
-                                        if (e > t2[h]) continue;           // j fired after t2 in that window
-                                        if (!found) {
-                                            lj = e;
-                                            found = true;
-                                        }
-                                        if (e == t2[h]) continue;
-                                        if (e <= lo) break;
-                                        mlo = min(mlo, m);
-                                        mhi = m;
+                                    const int4 q = hv[0];
+                                    lj = q.x <= t2[h] ? q.x : q.y;
+                                    if (lj == kHistNone) lj = 0;       // none before t2: never fired
+                                    int32_t ngt = (q.x > lo) + (q.y > lo) + (q.z > lo) + (q.w > lo);
+                                    for (int b4 = 1; b4 < kStdpHist / 4 && ngt == 4 * b4; ++b4) {
+                                        const int4 r = hv[b4];
+                                        ngt += (r.x > lo) + (r.y > lo) + (r.z > lo) + (r.w > lo);
                                     }
+                                    mlo = q.x >= t2[h] ? 1 : 0;        // (a spike at t2: dt = 0)
+                                    if (ngt > mlo) mhi = ngt - 1;
+                                    else mlo = kStdpHist;
                                     rng[h][u] = mlo | (mhi << 8);
                                 }
                                 adt[h][u] = t2[h] - lj;
@@ -1037,10 +1039,9 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                                 if (pl) {
                                     // deferred dendrite updates before t2, oldest first
                                     const int32_t* hj = hL + loc * kStdpHist;
-                                    const int32_t c = cL[loc];
                                     const int32_t mlo = rng[h][u] & 0xFF, mhi = rng[h][u] >> 8;
                                     for (int32_t m = mhi; m >= mlo; --m) {
-                                        const int32_t e = hj[(c - 1 - m) & (kStdpHist - 1)];
+                                        const int32_t e = hj[m];
                                         w = clipw(w + dw(la[h] - e), a.w_max, a.w_min);
                                     }
                                 }
@@ -1062,14 +1063,13 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             // 2b. neurons that fired kStdpHist - 1 times since tf (for apply_only: all that
             // fired since tf) go to this window's flush list (k_stdp_flush, launched next)
             uint32_t* fc = a.fcnt + (kwin & 1);
-            int32_t* fli = a.flist + (size_t)(kwin & 1) * a.flist_cap * 3;
+            int32_t* fli = a.flist + (size_t)(kwin & 1) * a.flist_cap * 2;
             for (int32_t j = tid; j < tn; j += bd) {
                 const int32_t n_since = cL[j] - a.cntf[t0 + j];
                 if (full ? n_since >= kStdpHist - 1 : n_since > 0) {
                     const uint32_t e = atomicAdd(fc, 1u);
-                    fli[3 * e] = t0 + j;
-                    fli[3 * e + 1] = cL[j];
-                    fli[3 * e + 2] = tfL[j];
+                    fli[2 * e] = t0 + j;
+                    fli[2 * e + 1] = tfL[j];
                     if (full) {
                         a.tf[t0 + j] = (int32_t)(s0 - 1);
                         a.cntf[t0 + j] = cL[j];
@@ -1231,7 +1231,16 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                         ring[slot * tn + j] = 0u;
                         if (!kCls && RW == 2) ring[((size_t)D + slot) * tn + j] = 0u;
                     }
-                    if (kPl && fired) a.hist[(size_t)i * kStdpHist + ((st.cnt - 1) & (kStdpHist - 1))] = s;
+                    if (kPl && fired) {
+                        // the history moves down one entry (the oldest drops out), in LDS and HBM
+                        int4* hq = reinterpret_cast<int4*>(hL + (size_t)j * kStdpHist);
+                        const int4 q0 = hq[0], q1 = hq[1], q2 = hq[2], q3 = hq[3];
+                        const int4 n0 = make_int4(s, q0.x, q0.y, q0.z), n1 = make_int4(q0.w, q1.x, q1.y, q1.z);
+                        const int4 n2 = make_int4(q1.w, q2.x, q2.y, q2.z), n3 = make_int4(q2.w, q3.x, q3.y, q3.z);
+                        hq[0] = n0; hq[1] = n1; hq[2] = n2; hq[3] = n3;
+                        int4* hg = reinterpret_cast<int4*>(a.hist + (size_t)i * kStdpHist);
+                        hg[0] = n0; hg[1] = n1; hg[2] = n2; hg[3] = n3;
+                    }
                     append(fired, w, s);
                     slot = (slot + 1 == D) ? 0 : slot + 1;
                 }
@@ -1289,7 +1298,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
 // kernel).  The list is staged in LDS 256 entries at a time; their in-synapses
 // are cut into units of one per thread, dealt round-robin over the grid.
 __global__ __launch_bounds__(256) void k_stdp_flush(TiledArgs a, int32_t off, int32_t wk) {
-    __shared__ int32_t ej[256], ec[256], etf[256];
+    __shared__ int32_t ej[256], etf[256];
     __shared__ int64_t epb[256], epe[256];
     __shared__ uint32_t eui[256], wsum[16];                // inclusive prefix of units
     const Ctl* ctl = a.ctl;
@@ -1298,9 +1307,9 @@ __global__ __launch_bounds__(256) void k_stdp_flush(TiledArgs a, int32_t off, in
     // trip instead of three); the index's parity then picks one
     const uint32_t nf0 = a.fcnt[0], nf1 = a.fcnt[1];
     const uint32_t i0 = min(tid0, a.flist_cap - 1u);
-    const int32_t* fl1 = a.flist + (size_t)a.flist_cap * 3;
-    const int32_t j0 = a.flist[3 * i0], c0 = a.flist[3 * i0 + 1], tf0 = a.flist[3 * i0 + 2];
-    const int32_t j1 = fl1[3 * i0], c1 = fl1[3 * i0 + 1], tf1 = fl1[3 * i0 + 2];
+    const int32_t* fl1 = a.flist + (size_t)a.flist_cap * 2;
+    const int32_t j0 = a.flist[2 * i0], tf0 = a.flist[2 * i0 + 1];
+    const int32_t j1 = fl1[2 * i0], tf1 = fl1[2 * i0 + 1];
     const int64_t kwin = ctl->base_window + wk;
     const bool par = (kwin & 1) != 0;
     const uint32_t nf = par ? nf1 : nf0;
@@ -1314,10 +1323,9 @@ __global__ __launch_bounds__(256) void k_stdp_flush(TiledArgs a, int32_t off, in
         uint32_t nu = 0;
         if (tid < nc) {
             const bool first = f0 == 0;
-            const int32_t j = first ? (par ? j1 : j0) : fli[3 * (f0 + tid)];
+            const int32_t j = first ? (par ? j1 : j0) : fli[2 * (f0 + tid)];
             ej[tid] = j;
-            ec[tid] = first ? (par ? c1 : c0) : fli[3 * (f0 + tid) + 1];
-            etf[tid] = first ? (par ? tf1 : tf0) : fli[3 * (f0 + tid) + 2];
+            etf[tid] = first ? (par ? tf1 : tf0) : fli[2 * (f0 + tid) + 1];
             const int64_t pb = a.din_ptr[j], pe = a.din_ptr[j + 1];
             epb[tid] = pb;
             epe[tid] = pe;
@@ -1338,21 +1346,27 @@ __global__ __launch_bounds__(256) void k_stdp_flush(TiledArgs a, int32_t off, in
             const uint32_t uf = u - (f ? eui[f - 1] : 0u);
             const int64_t p = epb[f] + (int64_t)uf * bd + tid;
             if (p >= epe[f]) continue;
-            const int32_t j = ej[f], c = ec[f];
+            const int32_t j = ej[f];
             const uint32_t kk = a.din_pos[p];
             float w = a.w_t[kk];                            // (with lk: one round trip)
             const int32_t lk = lastB[a.din_src[p]];
             const int32_t lo = max(lk == 0 ? -1 : lk, etf[f]);
-            const int32_t* hj = a.hist + (size_t)j * kStdpHist;
-            const int32_t nm = min(c, kStdpHist);
-            // the newest nr entries are after lo; their table values are loaded together,
-            // then applied oldest first
+            // j's history newest first (kHistNone past its spikes; a spike of window k, after
+            // hz, leads it): the newest nr entries are after lo; their table values are
+            // loaded together, then applied oldest first
+            const int4* hj = reinterpret_cast<const int4*>(a.hist + (size_t)j * kStdpHist);
             int32_t hv[kStdpHist];
 #pragma unroll
-            for (int m = 0; m < kStdpHist; ++m) hv[m] = hj[(c - 1 - m) & (kStdpHist - 1)];
+            for (int b4 = 0; b4 < kStdpHist / 4; ++b4) {
+                const int4 q = hj[b4];
+                hv[4 * b4] = q.x;
+                hv[4 * b4 + 1] = q.y;
+                hv[4 * b4 + 2] = q.z;
+                hv[4 * b4 + 3] = q.w;
+            }
             int32_t nr = 0;
 #pragma unroll
-            for (int m = 0; m < kStdpHist; ++m) nr += (m < nm && hv[m] > lo && m == nr) ? 1 : 0;
+            for (int m = 0; m < kStdpHist; ++m) nr += (hv[m] > lo && m == nr) ? 1 : 0;
             if (nr == 0) continue;
             float dv[kStdpHist];
 #pragma unroll
@@ -1419,7 +1433,9 @@ hipError_t launch_finalize(Ctl* ctl, int32_t steps, int32_t windows, hipStream_t
 
 size_t tiled_lds_bytes(int D, int B, int kind, int rw, int lut_lds) {
     const bool pl = stdp_kind(kind);
-    return sizeof(uint32_t) * ((size_t)rw * (size_t)D * (size_t)B + 3 * (size_t)kTiledBatch) +
+    size_t rwords = (size_t)rw * (size_t)D * (size_t)B;
+    if (pl) rwords = (rwords + 3) & ~(size_t)3;             // 16-byte aligned history rows
+    return sizeof(uint32_t) * (rwords + 3 * (size_t)kTiledBatch) +
            sizeof(uint16_t) * (size_t)kTiledItems +
            ((kind == kTFP8 || kind == kTBF16 || pl) ? sizeof(uint32_t) * kTiledBatch : 0) +
            (kind == kTStdpB ? sizeof(uint32_t) * kTiledBatch : 0) +
