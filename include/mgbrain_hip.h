@@ -82,7 +82,8 @@ enum {
 
 /* Spike propagation strategy. */
 enum {
-    MGB_PROP_AUTO = 0,    /* TILED for the CLASS format without STDP, else DIRECT */
+    MGB_PROP_AUTO = 0,    /* TILED where the fused kernel takes the network (delays <= 32 steps;
+                             with STDP, FP32 or BF16 weights), else DIRECT */
     MGB_PROP_DIRECT = 1,  /* one wave per spike over its CSR row, integer atomics to the ring in HBM */
     MGB_PROP_TILED = 2    /* one fused kernel per window: each workgroup owns a tile of targets,
                              keeps their ring slice in LDS, adds the arriving synapses from a

@@ -700,7 +700,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // a tile of at most one neuron per thread: the neuron's state and out-degree are
     // loaded at the start of each window (after the ring copy, which they would
     // delay), their latency hidden by the propagation phase
-    const bool pre = tn <= bd && full && !(kPl && CH >= 16);  // (not where registers would spill)
+    const bool pre = tn <= bd && full && !(kPl && (CH >= 16 || kMix));  // (not where registers would spill)
     uint32_t* gring = a.ring_t + (size_t)g * a.ring_slab;
     for (int32_t kw = 0; kw < nw; ++kw) {
         // (persistent form: the thread index is opaque to the compiler in each window,
