@@ -363,12 +363,15 @@ mgb_status choose_tiles(mgb_sim* s, const mgb_opts* o, int32_t n, int32_t D, con
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
     lds_max = std::min(lds_max, 160 * 1024);
     // the kernel's static LDS (block scan sums, event counter) shares the limit
-    // (per target: the difference over 4 targets, which no alignment rounds)
+    // (per target: the difference over 4 targets, which no alignment rounds; STDP keeps at
+    // least one entry of each table in LDS)
+    const int lut1 = stdp_kind(s->tkind) ? 1 : 0;
     const int64_t per_b =
         (int64_t)(tiled_lds_bytes(D, 4, s->tkind, s->trw) - tiled_lds_bytes(D, 0, s->tkind, s->trw)) / 4;
     int32_t bmax =
-        (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0, s->tkind, s->trw) - kTiledStaticLds) / per_b);
-    while (bmax > 1 && (int64_t)tiled_lds_bytes(D, bmax, s->tkind, s->trw) + kTiledStaticLds > lds_max) --bmax;
+        (int32_t)((lds_max - (int64_t)tiled_lds_bytes(D, 0, s->tkind, s->trw, lut1) - kTiledStaticLds) / per_b);
+    while (bmax > 1 && (int64_t)tiled_lds_bytes(D, bmax, s->tkind, s->trw, lut1) + kTiledStaticLds > lds_max)
+        --bmax;
     // wide words hold an 11-bit tile-local target; CLASS tiles of at most 1024
     // targets take 16-bit words (else 32-bit words with the global id)
     bmax = std::max<int32_t>(1, std::min<int32_t>(bmax, s->tkind == kTClass ? 1024 : 2048));
@@ -1137,8 +1140,10 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
         std::vector<float> ltp, ltd;
         MGB_TRY(build_stdp_lut(h.stdpconst, ltp, ltd));
         s->lut_n = (int32_t)ltp.size();
-        MGB_TRY(dalloc_n(s, &s->ltp, ltp.size()));
-        MGB_TRY(dalloc_n(s, &s->ltd, ltd.size()));
+        // one allocation, LTD after LTP (the fused kernel reads both through one buffer
+        // resource)
+        MGB_TRY(dalloc_n(s, &s->ltp, 2 * ltp.size()));
+        s->ltd = s->ltp + ltp.size();
         MGB_HIP_TRY(hipMemcpyAsync(s->ltp, ltp.data(), 4 * ltp.size(), hipMemcpyHostToDevice, st));
         MGB_HIP_TRY(hipMemcpyAsync(s->ltd, ltd.data(), 4 * ltd.size(), hipMemcpyHostToDevice, st));
         if (s->tiled && stdp_kind(s->tkind)) {
@@ -1148,7 +1153,8 @@ mgb_status setup_state(mgb_sim* s, const HostNeurons& h, const mgb_opts* o) {
             lds_max = std::min(lds_max, 160 * 1024);
             const int64_t room =
                 (int64_t)lds_max - kTiledStaticLds - (int64_t)tiled_lds_bytes(s->D, s->B, s->tkind, s->trw);
-            s->lut_lds = (int32_t)std::max<int64_t>(0, std::min<int64_t>(s->lut_n, room / 8));
+            // (at least one entry: the tiles leave room for it)
+            s->lut_lds = (int32_t)std::max<int64_t>(1, std::min<int64_t>(s->lut_n, room / 8));
             const size_t lds = tiled_lds_bytes(s->D, s->B, s->tkind, s->trw, s->lut_lds);
             if (lds > 65536) MGB_HIP_TRY(set_tiled_lds_limit(s->tkind, lds));
         }

@@ -233,9 +233,9 @@ struct TiledArgs {
     const uint32_t* din_pos;    // their tile-order positions
     const int32_t* din_src;     // their sources
     const float* ltp;           // ltp[k] = A_LTP * expf(-k / TAU_LTP)
-    const float* ltd;           // ltd[k] = A_LTD * expf(-k / TAU_LTD)
+    const float* ltd;           // ltd[k] = A_LTD * expf(-k / TAU_LTD); ltd == ltp + lut_n
     int32_t lut_n;
-    int32_t lut_lds;            // leading entries of each table copied to LDS
+    int32_t lut_lds;            // leading entries of each table copied to LDS (>= 1)
     float w_max, w_min;
     // kTStdp read-back: the launch applies the pending window's updates to w_t
     // (a copy) and lists every neuron with deferred dendrite updates for the
@@ -278,6 +278,9 @@ constexpr int32_t kHistNone = (int32_t)0x80808080;
 constexpr int kSubAlign = 8;
 constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 each
 constexpr int kBarStride = 32;      // words between the flags of a persistent launch's grid wait
+// Word 3 of a raw buffer resource on gfx9 (CDNA): data format 32, no swizzle; loads past
+// the resource's byte count return 0.
+constexpr int kBufferDword3 = 0x00020000;
 // Dynamic LDS of the fused kernel (rw ring words per slot and target); kTStdp also
 // caches lut_lds entries of each STDP table.
 size_t tiled_lds_bytes(int D, int B, int kind, int rw, int lut_lds = 0);

@@ -669,11 +669,18 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // the first lut_lds entries of the LTP and LTD tables
     float* lutL = reinterpret_cast<float*>(tfL + B);
     // STDP weight change for dt = t_pre - t_post (stdp_dw) from the LDS copy where it holds |dt|
+    // (lut_lds >= 1; entry 0 of both tables is 0, so dt = 0 gives 0), else from HBM.  The HBM
+    // read is a buffer load over both tables (ltd follows ltp), so that the compiler keeps an
+    // LDS load and a buffer load instead of one flat load through a selected pointer
+    const __amdgpu_buffer_rsrc_t lut_rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(kPl ? a.ltp : nullptr), (short)0, kPl ? 8 * a.lut_n : 0, kBufferDword3);
     auto dw = [&](int32_t dt) -> float {
-        if (dt == 0) return 0.0f;
         const int32_t k = dt < 0 ? -dt : dt;
-        if (k < a.lut_lds) return lutL[(dt < 0 ? 0 : a.lut_lds) + k];
-        return (dt < 0 ? a.ltp : a.ltd)[min(k, a.lut_n - 1)];
+        float v = lutL[(dt < 0 ? 0 : a.lut_lds) + min(k, a.lut_lds - 1)];
+        if (k >= a.lut_lds)
+            v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                lut_rs, 4 * ((dt < 0 ? 0 : a.lut_n) + min(k, a.lut_n - 1)), 0, 0));
+        return v;
     };
     const bool full = !kPl || a.apply_only == 0;
     Ctl* ctl = a.ctl;
