@@ -189,7 +189,8 @@ typedef struct mgb_info {
     int32_t tiles;
     int32_t block;             /* workgroup size of the propagation / fused kernel */
     int32_t segments;          /* independent segments (networks of a batch) */
-    int32_t word_bytes;        /* bytes per synapse the fused kernel reads (0: direct path) */
+    int32_t word_bytes;        /* bytes per synapse the fused kernel reads (0: direct path; fused
+                                  STDP with BF16 static weights: 4, and 4 more for a plastic one) */
     int32_t persistent;        /* 1: runs use the persistent launch (MGB_PERSIST_AUTO and resident) */
 } mgb_info;
 
