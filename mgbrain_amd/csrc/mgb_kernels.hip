@@ -806,6 +806,13 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         // latencies overlap; entries past the count are never used
         uint32_t pbeg = 0u, plm = 0u, paux = 0u, paux2 = 0u;
         if ((uint32_t)tid < min((uint32_t)bd, E)) meta(ent0, pbeg, plm, paux, paux2);
+        // windows of more than one batch (not with STDP, whose registers are spent): the
+        // entries of batch b + 2 load during batch b and their metadata during batch b + 1,
+        // so neither latency is waited for between batches (ent1: the next batch's entries)
+        constexpr bool kAhead = !kPl;
+        uint32_t ent1 = 0u;
+        if (kAhead && E > (uint32_t)bd && (uint32_t)tid < E - (uint32_t)bd)
+            ent1 = PW ? ld_agent(&L[(uint32_t)bd + (uint32_t)tid]) : L[(uint32_t)bd + (uint32_t)tid];
         if (full && kw == 0) ring_slice_in(ring, gring, D, tn, rprev, nprev, RW);
         NState pst;
         int ptype = T_RAW;
@@ -826,12 +833,21 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             uint32_t items = 0u;
             if ((uint32_t)tid < nb) {
                 uint32_t beg = pbeg, lm = plm, aux = paux, aux2 = paux2;
-                if (b0 > 0) meta(PW ? ld_agent(&L[b0 + (uint32_t)tid]) : L[b0 + (uint32_t)tid], beg, lm, aux, aux2);
+                if (!kAhead && b0 > 0)
+                    meta(PW ? ld_agent(&L[b0 + (uint32_t)tid]) : L[b0 + (uint32_t)tid], beg, lm, aux, aux2);
                 sp_beg[tid] = beg;
                 sp_len[tid] = lm;
                 if (kAux) sp_aux[tid] = aux;
                 if (kMix) sp_aux2[tid] = aux2;
                 items = ((lm & kIdMask) + CH - 1) / CH;
+            }
+            if constexpr (kAhead) {
+                // the next batch's metadata from its entries (loaded a batch ago), and the
+                // entries of the batch after it
+                const uint32_t b1 = b0 + (uint32_t)bd, b2 = b1 + (uint32_t)bd;
+                if (b1 < E && (uint32_t)tid < E - b1) meta(ent1, pbeg, plm, paux, paux2);
+                if (b2 < E && (uint32_t)tid < E - b2)
+                    ent1 = PW ? ld_agent(&L[b2 + (uint32_t)tid]) : L[b2 + (uint32_t)tid];
             }
             const uint32_t incl = block_incl_scan(items, wsum);
             const uint32_t total = wsum[(blockDim.x >> 6) - 1];
