@@ -348,7 +348,8 @@ def stdp_leg(args, log):
             k_bytes = {"k_tiled_stdp+k_stdp_flush": sum(k_bytes.values())}
             k_ms = {"k_tiled_stdp+k_stdp_flush": (pb["prop_ms"] + pb["stdp_ms"], pb["prop_launches"])}
             log(f"# stdp kernels: k_tiled_stdp {pb['prop_ms'] * 1e3 / max(1, pb['prop_launches']):.1f} us, "
-                f"k_stdp_flush {pb['stdp_ms'] * 1e3 / max(1, pb['stdp_launches']):.1f} us per window")
+                f"k_stdp_flush {pb['stdp_ms'] * 1e3 / max(1, pb['prop_launches']):.1f} us per window "
+                f"({pb['stdp_launches']} launches in {pb['prop_launches']} windows)")
         kernels = {}
         for k, (ms, nl) in k_ms.items():
             if nl and ms > 0:

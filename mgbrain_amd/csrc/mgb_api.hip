@@ -1349,12 +1349,15 @@ mgb_status issue_windows(mgb_sim* s, int32_t nsteps, bool finalize, bool capturi
         }
         while (off < nsteps) {
             const int32_t nst = std::min(s->window, nsteps - off);
+            // fused STDP: the flush kernel after every second window and after the last
+            TiledArgs tw = ta;
+            tw.flush_next = stdp_kind(s->tkind) && ((wk & 1) != 0 || off + nst >= nsteps) ? 1 : 0;
             SpanGuard g1{s, kSpanProp, prof};
             MGB_TRY(g1.begin());
-            MGB_HIP_TRY(launch_tiled(ta, off, wk, nst, s->tblock, s->stream));
+            MGB_HIP_TRY(launch_tiled(tw, off, wk, nst, s->tblock, s->stream));
             MGB_TRY(g1.end());
             s->launches += 1;
-            if (stdp_kind(s->tkind)) {
+            if (tw.flush_next) {
                 SpanGuard g2{s, kSpanStdp, prof};
                 MGB_TRY(g2.begin());
                 MGB_HIP_TRY(launch_stdp_flush(ta, off, wk, s->stream));

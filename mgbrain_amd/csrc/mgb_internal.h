@@ -237,10 +237,14 @@ struct TiledArgs {
     int32_t lut_n;
     int32_t lut_lds;            // leading entries of each table copied to LDS (>= 1)
     float w_max, w_min;
-    // kTStdp read-back: the launch applies the pending window's updates to w_t
+    // Fused STDP read-back: the launch applies the pending window's updates to w_t
     // (a copy) and lists every neuron with deferred dendrite updates for the
     // flush kernel, touching no ring, neuron state, spike list or tf / cntf
     int32_t apply_only;
+    // Fused STDP: 1 when k_stdp_flush follows this window's launch (the host runs it
+    // after every second window of a run and after its last), else the window lists
+    // no neuron for flushing
+    int32_t flush_next;
     // ring words per (slot, target): 1 for CLASS counts and for the wide kinds when
     // every integrating neuron is LIFB (one fixed-point sum), else 2 (exc, inh)
     int32_t rw;

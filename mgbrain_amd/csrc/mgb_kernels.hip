@@ -1083,13 +1083,18 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             __syncthreads();   // wsum / sp_* are rewritten by the next batch
         }
         if (kPl) {
-            // 2b. neurons that fired kStdpHist - 1 times since tf (for apply_only: all that
-            // fired since tf) go to this window's flush list (k_stdp_flush, launched next)
+            // 2b. when a flush follows this window: neurons that fired kStdpHist - 2 times
+            // since tf (for apply_only: all that fired since tf) go to this window's flush
+            // list (k_stdp_flush, launched next).  A flush follows at least every second
+            // window and a neuron fires at most once a window, so one not listed here
+            // (at most kStdpHist - 3) is listed at the next flush with at most
+            // kStdpHist - 1, and its history then holds every spike since tf plus the one
+            // of that window
             uint32_t* fc = a.fcnt + (kwin & 1);
             int32_t* fli = a.flist + (size_t)(kwin & 1) * a.flist_cap * 2;
             for (int32_t j = tid; j < tn; j += bd) {
                 const int32_t n_since = cL[j] - a.cntf[t0 + j];
-                if (full ? n_since >= kStdpHist - 1 : n_since > 0) {
+                if (full ? (a.flush_next && n_since >= kStdpHist - 2) : n_since > 0) {
                     const uint32_t e = atomicAdd(fc, 1u);
                     fli[2 * e] = t0 + j;
                     fli[2 * e + 1] = tfL[j];
