@@ -137,6 +137,7 @@ struct mgb_sim {
     uint32_t* h_err_dev = nullptr;     // its address on the device (the run's last kernel writes it)
     bool err_pending = false;          // h_err is due with the stream's completion
     int xcd_map = -1;                  // fused launches: XCD-contiguous tile order (-1: not computed yet)
+    int region = 0;                    // persistent launches: per-tile list regions (TiledArgs::region)
     uint64_t tile_hash = 0;            // FNV-1a of the tile boundaries (the ring slices' layout)
     // host mirrors for read-back
     std::vector<float> class_w;        // [n_tbl][2]
@@ -1240,10 +1241,23 @@ TiledArgs tiled_args(const mgb_sim* s);
 void update_persist(mgb_sim* s) {
     s->persist = 0;
     s->xcd_map = -1;
+    s->region = 0;
     if (!s->tiled) return;
-    const TiledArgs a = tiled_args(s);
+    TiledArgs a = tiled_args(s);
     s->xcd_map = a.xcd_map;
     if (stdp_kind(s->tkind) || s->persist_req != MGB_PERSIST_AUTO || !s->bar) return;
+    // per-tile list regions when one thread per tile can scan their counts and the
+    // counts fit in LDS beside the tile's data with every workgroup still resident
+    if (s->G <= kRegionMaxG && s->G <= s->tblock) {
+        a.region = 1;
+        if (set_tiled_pw_lds_limit(a) == hipSuccess && resident(s, a, true)) {
+            s->region = 1;
+            s->persist = 1;
+            return;
+        }
+        (void)hipGetLastError();
+        a.region = 0;
+    }
     s->persist = resident(s, a, true) ? 1 : 0;
 }
 
@@ -1279,6 +1293,7 @@ TiledArgs tiled_args(const mgb_sim* s) {
         a.w_max = s->sc[4]; a.w_min = s->sc[5];
     }
     a.bar = s->bar; a.bar_err = s->bar_err; a.spin_limit = s->spin_limit;
+    a.region = s->region;
     a.xcd_map = s->xcd_map >= 0 ? s->xcd_map : (resident(s, a, false) ? 1 : 0);
     return a;
 }

@@ -251,8 +251,14 @@ struct TiledArgs {
     // 1: workgroup b runs tile (b % kXcds) * (G / kXcds) + ... (a contiguous run of
     // tiles per XCD), set when every workgroup of the launch is resident at once
     int32_t xcd_map;
-    // persistent launches: one flag per tile (kBarStride words apart) for the wait
-    // between windows, its error word, and the wait's bound in wall-clock ticks
+    // persistent launches, 1 when G <= kRegionMaxG and G <= the block size: windows
+    // followed by another window of the same launch write their spikes into the tile's
+    // own part of the list (tile_lo * window on) and publish the count with the wait's
+    // flag, instead of reserving list space with a returning global atomic
+    int32_t region;
+    // persistent launches: one flag per tile (kBarStride words apart; a 64-bit word:
+    // window epoch | spike count << 32) for the wait between windows, its error word,
+    // and the wait's bound in wall-clock ticks
     uint32_t* bar;
     uint32_t* bar_err;
     uint64_t spin_limit;
@@ -282,6 +288,8 @@ constexpr int32_t kHistNone = (int32_t)0x80808080;
 constexpr int kSubAlign = 8;
 constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 each
 constexpr int kBarStride = 32;      // words between the flags of a persistent launch's grid wait
+constexpr int kRegionMaxG = 512;    // persistent launches: most tiles whose lists go to per-tile regions
+constexpr int kRegionMinSpikes = 64; // ... and the fewest spikes of the previous window for it
 // Word 3 of a raw buffer resource on gfx9 (CDNA): data format 32, no swizzle; loads past
 // the resource's byte count return 0.
 constexpr int kBufferDword3 = 0x00020000;
@@ -299,6 +307,10 @@ hipError_t launch_tile_lean(const int32_t* tile_lo, int32_t G, const uint8_t* ty
 int tiled_blocks_per_cu(const TiledArgs& a, int block, bool persistent = false);
 hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int block,
                         hipStream_t st);
+// Dynamic LDS of the persistent form for `a` (with a.region, the list regions' counts),
+// and the persistent function's limit raised to it.
+size_t tiled_pw_lds_bytes(const TiledArgs& a);
+hipError_t set_tiled_pw_lds_limit(const TiledArgs& a);
 // Persistent form (not kTStdp): nwin windows of nsteps steps in one launch; every
 // workgroup must be resident (tiled_blocks_per_cu(a, block, true) x CUs >= G).  The
 // wait after window k publishes and waits for epoch k + 1 (the window's index over

@@ -537,23 +537,35 @@ __device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ uint64_t ld_agent64(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent64(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Grid-wide wait between two windows of a persistent launch.  Each thread first
 // waits for its own stores to complete (s_waitcnt vmcnt(0): the agent-scope
 // stores above are then visible at agent scope), then the workgroup publishes
-// `epoch` in its flag (kBarStride words apart, one cache line each) and waits
-// until every workgroup's flag holds at least `epoch`, one flag per thread.
-// The wait is bounded: past `limit` clock ticks, or once another workgroup has
-// reported a timeout, it sets *err and returns false, so every workgroup leaves
-// the kernel (the host turns *err into an error).
-__device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t G, uint32_t g, uint32_t epoch, uint32_t* err,
-                                          uint64_t limit) {
+// `epoch` and `count` (the window's spikes of the tile, when its list went to
+// the tile's region) in its flag, one 64-bit word per tile kBarStride words apart
+// (a cache line each), and waits until every workgroup's flag holds at least
+// `epoch`, one flag per thread; with `cnt`, the counts of tiles [ct0, ct1) land in
+// cnt[h] (0 for the others).  The wait is bounded: past `limit` clock ticks, or
+// once another workgroup has reported a timeout, it sets *err and returns false,
+// so every workgroup leaves the kernel (the host turns *err into an error).
+__device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t G, uint32_t g, uint32_t epoch, uint32_t count,
+                                          uint32_t* cnt, uint32_t ct0, uint32_t ct1, uint32_t* err, uint64_t limit) {
     __builtin_amdgcn_s_waitcnt(0x0F70);    // vmcnt(0)
     __syncthreads();
-    if (threadIdx.x == 0) st_agent(&bar[(size_t)g * kBarStride], epoch);
+    uint64_t* flag = reinterpret_cast<uint64_t*>(bar);
+    constexpr size_t kStride64 = kBarStride / 2;
+    if (threadIdx.x == 0) st_agent64(&flag[(size_t)g * kStride64], ((uint64_t)count << 32) | epoch);
     int bad = 0;
     for (uint32_t h = threadIdx.x; h < G && !bad; h += blockDim.x) {
         const uint64_t c0 = (uint64_t)wall_clock64();
-        while ((int32_t)(ld_agent(&bar[(size_t)h * kBarStride]) - epoch) < 0) {
+        uint64_t v = ld_agent64(&flag[(size_t)h * kStride64]);
+        while ((int32_t)((uint32_t)v - epoch) < 0) {
             if (ld_agent(err) != 0u) {
                 bad = 1;
                 break;
@@ -564,7 +576,9 @@ __device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t G, uint32_t g,
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
+            v = ld_agent64(&flag[(size_t)h * kStride64]);
         }
+        if (cnt) cnt[h] = (h >= ct0 && h < ct1) ? (uint32_t)(v >> 32) : 0u;
     }
     return __syncthreads_or(bad) == 0;
 }
@@ -704,10 +718,30 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     const int64_t sbase = ctl->base_step + off;
     const int64_t wbase = ctl->base_window + wk;
     const int32_t nw = PW ? nwin : 1;
+    // persistent launches with per-tile list regions (a.region): between windows the
+    // tiles' spike counts, then their exclusive prefix, and the tiles' list offsets
+    // (tile h's part of a list starts at tile_lo[h] * window), [G] words each of LDS
+    // after the staging area (the launch brings them)
+    uint32_t* rg_pre = reinterpret_cast<uint32_t*>(itm + kItems) + (kAux ? kTiledBatch : 0);
+    uint32_t* rg_off = rg_pre + (PW ? a.G : 0);
+    const bool rgn_on = PW && a.region != 0;
+    if (rgn_on)
+        for (int32_t h = tid0; h < a.G; h += bd) rg_off[h] = (uint32_t)a.tile_lo[h] * (uint32_t)a.window;
+    // a window's list goes to the regions when the previous one of its segment held more
+    // than kRegionMinSpikes spikes (every workgroup of the segment knows that count, so
+    // producers and consumers agree); the previous window's choice (thread-uniform)
+    bool rg_prev = false;
+    const uint32_t gt0 = one ? 0u : (uint32_t)a.seg_tile0[q];               // the segment's tiles
+    const uint32_t gt1 = one ? (uint32_t)a.G : (uint32_t)a.seg_tile0[q + 1];
+    // (thread 0) synaptic events and spikes of the windows that wrote regions, added
+    // to the run's counters by the launch's last window
+    unsigned long long se_acc = 0ull, spk_acc = 0ull;
     // a tile of at most one neuron per thread: the neuron's state and out-degree are
     // loaded at the start of each window (after the ring copy, which they would
     // delay), their latency hidden by the propagation phase
-    const bool pre = tn <= bd && full && !(kPl && (CH >= 16 || kMix));  // (not where registers would spill)
+    // (not where registers would spill: with STDP at 16 synapses a work item or with
+    // BF16 static weights, and in the persistent form)
+    const bool pre = !PW && tn <= bd && full && !(kPl && (CH >= 16 || kMix));
     uint32_t* gring = a.ring_t + (size_t)g * a.ring_slab;
     for (int32_t kw = 0; kw < nw; ++kw) {
         // (persistent form: the thread index is opaque to the compiler in each window,
@@ -726,6 +760,23 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         // launch know the index and read the list other workgroups just wrote.
         const uint32_t li = min((uint32_t)tid, (uint32_t)nq * (uint32_t)a.window - 1u);
         const uint32_t* L = a.win_buf + (size_t)bprev * a.win_stride + lbase;
+        // the previous window's list is in the tiles' regions
+        const bool rgc = rgn_on && kw > 0 && rg_prev;
+        // entry e (< the list's count) of the previous window's list: in the regions,
+        // tile h (the last one whose prefix is at most e) holds it at e - prefix
+        auto rentry = [&](uint32_t e) -> uint32_t {
+            uint32_t lo = gt0, hi = gt1 - 1u;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if (rg_pre[mid] <= e) lo = mid;
+                else hi = mid - 1u;
+            }
+            return ld_agent(&a.win_buf[(size_t)bprev * a.win_stride + rg_off[lo] + (e - rg_pre[lo])]);
+        };
+        auto lent = [&](uint32_t e) -> uint32_t {
+            if (rgc) return rentry(e);
+            return PW ? ld_agent(&L[e]) : L[e];
+        };
         int64_t sprev;
         uint32_t E, ent0;
         if (kw == 0) {
@@ -737,11 +788,24 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             sprev = bprev == 0 ? tb0 : (bprev == 1 ? tb1 : tb2);
             E = bprev == 0 ? n0 : (bprev == 1 ? n1 : n2);
             ent0 = bprev == 0 ? l0 : (bprev == 1 ? l1 : l2);
+        } else if (rgc) {
+            // the counts came with the wait (rg_pre); the list is laid out by their prefix
+            sprev = s0 - nsteps;
+            const uint32_t c = (uint32_t)tid < (uint32_t)a.G ? rg_pre[tid] : 0u;
+            const uint32_t incl = block_incl_scan(c, wsum);
+            E = wsum[(blockDim.x >> 6) - 1];
+            if ((uint32_t)tid < (uint32_t)a.G) rg_pre[tid] = incl - c;
+            __syncthreads();
+            ent0 = (uint32_t)tid < E ? rentry((uint32_t)tid) : 0u;
         } else {
             sprev = s0 - nsteps;
             E = ld_agent(&tcnt[bprev * nseg]);
             ent0 = ld_agent(&L[li]);
         }
+        // this window's list goes to the tiles' regions (another window of the launch
+        // follows and the previous list was long enough)
+        const bool rgp = rgn_on && kw + 1 < nw && E > (uint32_t)kRegionMinSpikes;
+        rg_prev = rgp;
         if (full && lt == 0 && tid == 0) {
             if (PW) st_agent(&tcnt[bnext * nseg], 0u);
             else tcnt[bnext * nseg] = 0u;
@@ -811,8 +875,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         // so neither latency is waited for between batches (ent1: the next batch's entries)
         constexpr bool kAhead = !kPl;
         uint32_t ent1 = 0u;
-        if (kAhead && E > (uint32_t)bd && (uint32_t)tid < E - (uint32_t)bd)
-            ent1 = PW ? ld_agent(&L[(uint32_t)bd + (uint32_t)tid]) : L[(uint32_t)bd + (uint32_t)tid];
+        if (kAhead && E > (uint32_t)bd && (uint32_t)tid < E - (uint32_t)bd) ent1 = lent((uint32_t)bd + (uint32_t)tid);
         if (full && kw == 0) ring_slice_in(ring, gring, D, tn, rprev, nprev, RW);
         NState pst;
         int ptype = T_RAW;
@@ -833,8 +896,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             uint32_t items = 0u;
             if ((uint32_t)tid < nb) {
                 uint32_t beg = pbeg, lm = plm, aux = paux, aux2 = paux2;
-                if (!kAhead && b0 > 0)
-                    meta(PW ? ld_agent(&L[b0 + (uint32_t)tid]) : L[b0 + (uint32_t)tid], beg, lm, aux, aux2);
+                if (!kAhead && b0 > 0) meta(lent(b0 + (uint32_t)tid), beg, lm, aux, aux2);
                 sp_beg[tid] = beg;
                 sp_len[tid] = lm;
                 if (kAux) sp_aux[tid] = aux;
@@ -846,8 +908,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                 // entries of the batch after it
                 const uint32_t b1 = b0 + (uint32_t)bd, b2 = b1 + (uint32_t)bd;
                 if (b1 < E && (uint32_t)tid < E - b1) meta(ent1, pbeg, plm, paux, paux2);
-                if (b2 < E && (uint32_t)tid < E - b2)
-                    ent1 = PW ? ld_agent(&L[b2 + (uint32_t)tid]) : L[b2 + (uint32_t)tid];
+                if (b2 < E && (uint32_t)tid < E - b2) ent1 = lent(b2 + (uint32_t)tid);
             }
             const uint32_t incl = block_incl_scan(items, wsum);
             const uint32_t total = wsum[(blockDim.x >> 6) - 1];
@@ -1113,6 +1174,7 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         // end; lanes that find it full append to HBM directly.
         uint32_t* lsp = sp_beg;
         uint32_t* wbuf = a.win_buf + (size_t)bcur * a.win_stride + lbase;
+        uint32_t* rgw = a.win_buf + (size_t)bcur * a.win_stride + (size_t)t0 * (size_t)a.window;   // (rgp)
         const int32_t slot0 = (int32_t)(s0 % D);
         unsigned long long se_local = 0ull;
         for (int32_t j0 = 0; j0 < tn; j0 += bd) {
@@ -1148,7 +1210,24 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
                 lb = __shfl(lb, (int)leader);
                 const uint32_t cap = (uint32_t)kTiledBatch;
                 const uint32_t over0 = max(lb, cap);                 // first index past the LDS list
-                if (lb + pc > cap) {
+                if (rgp) {
+                    // the tile's region holds every entry at its place in the tile's list
+                    // (written now, complete by the wait); the LDS list keeps the first cap
+                    // for the records, and records past them take their places here
+                    const uint32_t r = lb + rank_below(m);
+                    if (fired) st_agent(&rgw[r], (uint32_t)i | ((uint32_t)w << 26));
+                    if (a.rec && lb + pc > cap) {
+                        unsigned long long rb = 0ull;
+                        if (lane_id() == leader)
+                            rb = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)(lb + pc - over0));
+                        rb = __shfl(rb, (int)leader);
+                        if (fired && r >= cap) {
+                            const uint64_t at = rb + (r - over0);
+                            if (at < a.rec_cap) a.rec[at] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)i;
+                            else ctl->rec_overflow = 1;
+                        }
+                    }
+                } else if (lb + pc > cap) {
                     uint32_t gb = 0u;
                     unsigned long long rb = 0ull;
                     if (lane_id() == leader) {
@@ -1289,10 +1368,21 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         }
         const uint32_t nl = min(lsp_n, (uint32_t)kTiledBatch);
         if (tid == 0) {
-            if (se_blk) atomicAdd((unsigned long long*)&ctl->se, se_blk);
-            if (nl) {
-                lsp_g = atomicAdd(&tcnt[bcur * nseg], nl);
-                lsp_r = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)nl);
+            if (rgp) {
+                // region window: no list space to reserve; the counters wait for the
+                // launch's last window (records still take their places now)
+                se_acc += se_blk;
+                if (!a.rec) spk_acc += lsp_n;
+                else if (nl) lsp_r = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)nl);
+            } else {
+                if (se_blk + se_acc) atomicAdd((unsigned long long*)&ctl->se, se_blk + se_acc);
+                if (spk_acc) atomicAdd((unsigned long long*)&ctl->rec_cnt, spk_acc);
+                se_acc = 0ull;
+                spk_acc = 0ull;
+                if (nl) {
+                    lsp_g = atomicAdd(&tcnt[bcur * nseg], nl);
+                    lsp_r = atomicAdd((unsigned long long*)&ctl->rec_cnt, (unsigned long long)nl);
+                }
             }
         }
         // 4. LDS -> ring slice, except the slots this window consumed (cleared above,
@@ -1300,10 +1390,15 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         // offset comes back from the atomic above
         if (!PW || kw + 1 == nw) ring_slice_out(ring, gring, D, tn, (int32_t)(s0 % D), nsteps, RW);
         __syncthreads();
-        for (uint32_t k = (uint32_t)tid; k < nl; k += (uint32_t)bd) {
+        for (uint32_t k = (uint32_t)tid; k < ((rgp && !a.rec) ? 0u : nl); k += (uint32_t)bd) {
             const uint32_t ent = lsp[k];
-            if (PW) st_agent(&wbuf[lsp_g + k], ent);
-            else wbuf[lsp_g + k] = ent;
+            if (rgp) {
+                // (the entry is in the region already)
+            } else if (PW) {
+                st_agent(&wbuf[lsp_g + k], ent);
+            } else {
+                wbuf[lsp_g + k] = ent;
+            }
             if (a.rec) {
                 const uint64_t at = lsp_r + k;
                 if (at < a.rec_cap)
@@ -1313,7 +1408,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
         }
         if constexpr (PW) {
             if (kw + 1 < nw &&
-                !grid_wait(a.bar, (uint32_t)a.G, (uint32_t)g, (uint32_t)(kwin + 1), a.bar_err, a.spin_limit))
+                !grid_wait(a.bar, (uint32_t)a.G, (uint32_t)g, (uint32_t)(kwin + 1), rgp ? lsp_n : 0u,
+                           rgn_on ? rg_pre : nullptr, gt0, gt1, a.bar_err, a.spin_limit))
                 return;
         }
     }   // windows
@@ -1557,15 +1653,29 @@ hipError_t launch_tiled(const TiledArgs& a, int32_t off, int32_t wk, int32_t nst
     return launch_fn(tiled_fn_of(a.kind, a.chunk, false), a, off, wk, nsteps, 1, block, lds, st);
 }
 
+// Dynamic LDS of the persistent form: the window's, plus the list regions' counts.
+size_t tiled_pw_lds_bytes(const TiledArgs& a) {
+    return tiled_lds_bytes(a.D, a.B, a.kind, a.rw, 0) + (a.region ? 2 * sizeof(uint32_t) * (size_t)a.G : 0);
+}
+
+hipError_t set_tiled_pw_lds_limit(const TiledArgs& a) {
+    const void* f = tiled_fn_of(a.kind, a.chunk, true);
+    if (!f) return hipErrorInvalidDeviceFunction;
+    const size_t lds = tiled_pw_lds_bytes(a);
+    return lds > 65536 ? hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) : hipSuccess;
+}
+
 hipError_t launch_tiled_persistent(const TiledArgs& a, int32_t off, int32_t wk, int32_t nsteps, int32_t nwin,
                                    int block, hipStream_t st) {
     if (stdp_kind(a.kind) || !a.bar || !a.bar_err) return hipErrorInvalidValue;
-    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, 0);
-    return launch_fn(tiled_fn_of(a.kind, a.chunk, true), a, off, wk, nsteps, nwin, block, lds, st);
+    if (a.region && (a.G > kRegionMaxG || a.G > block)) return hipErrorInvalidValue;
+    return launch_fn(tiled_fn_of(a.kind, a.chunk, true), a, off, wk, nsteps, nwin, block, tiled_pw_lds_bytes(a),
+                     st);
 }
 
 int tiled_blocks_per_cu(const TiledArgs& a, int block, bool persistent) {
-    const size_t lds = tiled_lds_bytes(a.D, a.B, a.kind, a.rw, stdp_kind(a.kind) ? a.lut_lds : 0);
+    const size_t lds = persistent ? tiled_pw_lds_bytes(a)
+                                  : tiled_lds_bytes(a.D, a.B, a.kind, a.rw, stdp_kind(a.kind) ? a.lut_lds : 0);
     const void* f = tiled_fn_of(a.kind, a.chunk, persistent);
     if (!f) return 0;
     int nb = 0;
@@ -1582,7 +1692,14 @@ hipError_t set_tiled_lds_limit(int kind, size_t bytes) {
             const void* f = tiled_fn_of(kind, chunk, pw != 0);
             if (!f) continue;
             const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-            if (e != hipSuccess) return e;
+            if (e == hipSuccess) continue;
+            // the persistent form's own static LDS (the list regions' tables) may not leave
+            // room for these tiles: it is then never resident, and never launched
+            if (pw) {
+                (void)hipGetLastError();
+                continue;
+            }
+            return e;
         }
     return hipSuccess;
 }
