@@ -1248,7 +1248,7 @@ void update_persist(mgb_sim* s) {
     if (stdp_kind(s->tkind) || s->persist_req != MGB_PERSIST_AUTO || !s->bar) return;
     // per-tile list regions when one thread per tile can scan their counts and the
     // counts fit in LDS beside the tile's data with every workgroup still resident
-    if (s->G <= kRegionMaxG && s->G <= s->tblock) {
+    if (s->tkind == kTClass16 && s->G <= kRegionMaxG && s->G <= s->tblock) {
         a.region = 1;
         if (set_tiled_pw_lds_limit(a) == hipSuccess && resident(s, a, true)) {
             s->region = 1;

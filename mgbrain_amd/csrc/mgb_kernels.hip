@@ -724,7 +724,10 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // after the staging area (the launch brings them)
     uint32_t* rg_pre = reinterpret_cast<uint32_t*>(itm + kItems) + (kAux ? kTiledBatch : 0);
     uint32_t* rg_off = rg_pre + (PW ? a.G : 0);
-    const bool rgn_on = PW && a.region != 0;
+    // (the 16-bit CLASS words of the persistent form only: in the other kinds' persistent
+    // forms the code alone, never run, cost their windows 5 %)
+    constexpr bool kRg = PW && WK == kTClass16;
+    const bool rgn_on = kRg && a.region != 0;
     if (rgn_on)
         for (int32_t h = tid0; h < a.G; h += bd) rg_off[h] = (uint32_t)a.tile_lo[h] * (uint32_t)a.window;
     // a window's list goes to the regions when the previous one of its segment held more
@@ -740,8 +743,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // loaded at the start of each window (after the ring copy, which they would
     // delay), their latency hidden by the propagation phase
     // (not where registers would spill: with STDP at 16 synapses a work item or with
-    // BF16 static weights, and in the persistent form)
-    const bool pre = !PW && tn <= bd && full && !(kPl && (CH >= 16 || kMix));
+    // BF16 static weights, and in the persistent form of the CLASS kinds)
+    const bool pre = !(PW && kCls) && tn <= bd && full && !(kPl && (CH >= 16 || kMix));
     uint32_t* gring = a.ring_t + (size_t)g * a.ring_slab;
     for (int32_t kw = 0; kw < nw; ++kw) {
         // (persistent form: the thread index is opaque to the compiler in each window,
