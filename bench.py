@@ -572,7 +572,7 @@ def main(argv=None):
     if args.copy_probe:
         return copy_probe()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
+        print(json.dumps({"error": "bench.py runs on one GPU only; multi-GPU sharding is a design note, "
                                    "not an implemented path", "n_gpus": args.gpus}))
         return 2
 

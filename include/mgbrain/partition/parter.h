@@ -5,7 +5,7 @@
 // device 0 (mgsim.cpp:341-352) and MGSimulator here simulates them as one
 // network: neurons are then split into nparts contiguous id ranges (the
 // partition does not change a single-device run).  Graph partitioning (METIS,
-// MODEL, BSIM) for several GPUs is not part of this build; nparts > 1 without
+// MODEL, BSIM) for several GPUs is not part of this library; nparts > 1 without
 // SINGLE_GPU throws std::invalid_argument, as MGSimulator's constructor does.
 #pragma once
 #include <stdexcept>

@@ -33,7 +33,7 @@ typedef enum mgb_status {
     MGB_ERR_INVALID = 1,      /* bad argument or network (e.g. a delay below one step) */
     MGB_ERR_HIP = 2,          /* HIP runtime error */
     MGB_ERR_OOM = 3,          /* device or host allocation failed */
-    MGB_ERR_UNSUPPORTED = 4,  /* valid but outside this build's limits */
+    MGB_ERR_UNSUPPORTED = 4,  /* valid but outside this library's limits */
     MGB_ERR_CAPACITY = 5      /* spike record capacity exceeded (data kept up to it) */
 } mgb_status;
 
@@ -255,7 +255,7 @@ mgb_status mgb_fire_counts(mgb_sim* sim, int32_t* out);
 
 /* The fused path's tiles: tile_lo[0..tiles] (mgb_info.tiles + 1 entries), tile g
  * holding neurons [tile_lo[g], tile_lo[g + 1]).  MGB_ERR_UNSUPPORTED on the direct
- * path.  No reference counterpart (introspection of this build's tiling). */
+ * path.  No reference counterpart (introspection of the library's tiling). */
 mgb_status mgb_read_tiles(mgb_sim* sim, int32_t* tile_lo);
 
 /* Device CSR in the reference's axon order: row_ptr[n+1] (axon_offs), target,

@@ -65,7 +65,7 @@ void MGSimulator::open(const mgb_net_desc& d) {
 
 MGSimulator::MGSimulator(Network& net, std::vector<int>& part, int nparts) {
     if (nparts > 1 && !Config::SINGLE_GPU)
-        throw std::invalid_argument("MGSimulator: more than one GPU partition is not supported by this build");
+        throw std::invalid_argument("MGSimulator: more than one GPU partition is not supported by this library");
     if (part.size() != net.neurons.size() && nparts > 1)
         throw std::invalid_argument("MGSimulator: partition vector size does not match the network");
     std::cout << (nparts <= 1 ? "run single-net on single-GPU" : "run multi-net on single-GPU") << std::endl;
