@@ -76,8 +76,15 @@ typedef struct mgb_opts {
 /* Launch form of the fused kernel. */
 enum {
     MGB_PERSIST_AUTO = 0,  /* the whole windows of a run (or of a graph block) in one launch when
-                              every workgroup fits on the device at once, else one launch per window */
-    MGB_PERSIST_OFF = 1    /* one launch per window */
+                              every workgroup fits on the device at once, else one launch per window;
+                              16-bit CLASS words in networks of 32768 neurons or more (and at most
+                              512 tiles): list regions after windows of more than 256 spikes */
+    MGB_PERSIST_OFF = 1,   /* one launch per window */
+    MGB_PERSIST_REGIONS = 2, /* as AUTO, with list regions in every window of a launch but its last
+                              whatever the size (where the words and tiles allow them): each
+                              workgroup writes its tile's spikes at the tile's own place of the
+                              window's list and passes the count with its wait flag */
+    MGB_PERSIST_LISTS = 3  /* as AUTO, without list regions: every window's list by a global atomic */
 };
 
 /* Spike propagation strategy. */
@@ -191,7 +198,8 @@ typedef struct mgb_info {
     int32_t segments;          /* independent segments (networks of a batch) */
     int32_t word_bytes;        /* bytes per synapse the fused kernel reads (0: direct path; fused
                                   STDP with BF16 static weights: 4, and 4 more for a plastic one) */
-    int32_t persistent;        /* 1: runs use the persistent launch (MGB_PERSIST_AUTO and resident) */
+    int32_t persistent;        /* 1: runs use the persistent launch (not MGB_PERSIST_OFF, and every
+                                  workgroup resident); 2: the persistent launch with list regions */
 } mgb_info;
 
 typedef struct mgb_sim mgb_sim;

@@ -126,9 +126,9 @@ struct mgb_sim {
     uint32_t* offs32 = nullptr;
     uint32_t* ring_t = nullptr;
     size_t ring_slab = 0;              // words per tile in ring_t (RW * D * B rounded up to 16 bytes)
-    // persistent launches of the fused kernel (MGB_PERSIST_AUTO, every workgroup resident)
+    // persistent launches of the fused kernel (not MGB_PERSIST_OFF, every workgroup resident)
     int persist_req = MGB_PERSIST_AUTO;
-    int persist = 0;
+    int persist = 0;                   // 1: persistent launches, 2: with list regions
     uint32_t* bar = nullptr;           // [G * kBarStride] grid-wait flags
     uint32_t* bar_err = nullptr;       // set by a wait that timed out
     uint64_t spin_limit = 0;           // wall-clock ticks
@@ -1236,7 +1236,7 @@ TiledArgs tiled_args(const mgb_sim* s);
 
 // The launch form of the current launch shape, computed once per shape: the
 // XCD-contiguous tile order when every workgroup is resident, and persistent launches
-// when asked for (MGB_PERSIST_AUTO), the kernel has the form (not STDP) and every
+// when asked for (not MGB_PERSIST_OFF), the kernel has the form (not STDP) and every
 // workgroup of its persistent form is resident.
 void update_persist(mgb_sim* s) {
     s->persist = 0;
@@ -1245,14 +1245,19 @@ void update_persist(mgb_sim* s) {
     if (!s->tiled) return;
     TiledArgs a = tiled_args(s);
     s->xcd_map = a.xcd_map;
-    if (stdp_kind(s->tkind) || s->persist_req != MGB_PERSIST_AUTO || !s->bar) return;
-    // per-tile list regions when one thread per tile can scan their counts and the
-    // counts fit in LDS beside the tile's data with every workgroup still resident
-    if (s->tkind == kTClass16 && s->G <= kRegionMaxG && s->G <= s->tblock) {
-        a.region = 1;
+    if (stdp_kind(s->tkind) || s->persist_req == MGB_PERSIST_OFF || !s->bar) return;
+    // per-tile list regions (the kernel's variant with them) when asked for, or by
+    // default in networks large enough for windows of hundreds of spikes (in small ones
+    // the variant without them, which preloads the neuron state, is the faster), when one
+    // thread per tile can scan their counts and the counts fit in LDS beside the tile's
+    // data with every workgroup still resident
+    const bool want = s->persist_req == MGB_PERSIST_REGIONS ||
+                      (s->persist_req == MGB_PERSIST_AUTO && s->n >= kRegionMinNeurons);
+    if (want && s->tkind == kTClass16 && s->G <= kRegionMaxG && s->G <= s->tblock) {
+        a.region = s->persist_req == MGB_PERSIST_REGIONS ? 2 : 1;
         if (set_tiled_pw_lds_limit(a) == hipSuccess && resident(s, a, true)) {
-            s->region = 1;
-            s->persist = 1;
+            s->region = a.region;
+            s->persist = 2;
             return;
         }
         (void)hipGetLastError();
@@ -1452,7 +1457,7 @@ mgb_status validate_opts(const mgb_opts* o) {
     if (!o) return MGB_OK;
     if (o->weight_format < MGB_WF_AUTO || o->weight_format > MGB_WF_FP8 || o->window < 0 ||
         o->record_capacity < 0 || o->propagation < MGB_PROP_AUTO || o->propagation > MGB_PROP_TILED ||
-        o->tile_size < 0 || o->device < 0 || o->persistent < MGB_PERSIST_AUTO || o->persistent > MGB_PERSIST_OFF ||
+        o->tile_size < 0 || o->device < 0 || o->persistent < MGB_PERSIST_AUTO || o->persistent > MGB_PERSIST_LISTS ||
         o->spin_limit_us < 0) {
         set_error("invalid mgb_opts");
         return MGB_ERR_INVALID;

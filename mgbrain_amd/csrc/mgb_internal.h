@@ -251,11 +251,12 @@ struct TiledArgs {
     // 1: workgroup b runs tile (b % kXcds) * (G / kXcds) + ... (a contiguous run of
     // tiles per XCD), set when every workgroup of the launch is resident at once
     int32_t xcd_map;
-    // persistent launches of 16-bit CLASS words, 1 when G <= kRegionMaxG and G <= the
-    // block size (and the counts fit in LDS): windows
-    // followed by another window of the same launch write their spikes into the tile's
-    // own part of the list (tile_lo * window on) and publish the count with the wait's
-    // flag, instead of reserving list space with a returning global atomic
+    // persistent launches of 16-bit CLASS words with G <= kRegionMaxG and G <= the block
+    // size (and the counts fitting in LDS), launched as the kernel's list-region variant:
+    // windows followed by another window of the same launch write their spikes into the
+    // tile's own part of the list (tile_lo * window on) and publish the count with the
+    // wait's flag, instead of reserving list space with a returning global atomic;
+    // 1: windows after one of more than kRegionMinSpikes spikes, 2: every such window
     int32_t region;
     // persistent launches: one flag per tile (kBarStride words apart; a 64-bit word:
     // window epoch | spike count << 32) for the wait between windows, its error word,
@@ -291,6 +292,7 @@ constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 
 constexpr int kBarStride = 32;      // words between the flags of a persistent launch's grid wait
 constexpr int kRegionMaxG = 512;    // persistent launches: most tiles whose lists go to per-tile regions
 constexpr int kRegionMinSpikes = 256; // ... and the fewest spikes of the previous window for it
+constexpr int kRegionMinNeurons = 32768; // MGB_PERSIST_AUTO: list regions from this many neurons on
 // Word 3 of a raw buffer resource on gfx9 (CDNA): data format 32, no swizzle; loads past
 // the resource's byte count return 0.
 constexpr int kBufferDword3 = 0x00020000;

@@ -19,7 +19,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_CAPACITY = range(6)
 RAW, LIF0, POISSON, LIF, LIFE, LIFB = range(6)
 WF_AUTO, WF_CLASS, WF_F32, WF_BF16, WF_FP8 = range(5)
 PROP_AUTO, PROP_DIRECT, PROP_TILED = range(3)
-PERSIST_AUTO, PERSIST_OFF = range(2)
+PERSIST_AUTO, PERSIST_OFF, PERSIST_REGIONS, PERSIST_LISTS = range(4)
 CONN_BERNOULLI, CONN_FIXED_INDEGREE = 1, 2
 F_V, F_IEXC, F_IINH, F_REFR, F_LAST, F_CNT = range(6)
 

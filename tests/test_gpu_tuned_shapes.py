@@ -43,7 +43,7 @@ def _shape(name):
 def _against_oracle(g, net, syn, seed, acc=po.ACC_F32, stdp=False, stdpconst=None):
     info = g.info()
     # the fused STDP kernel runs one launch per window; the others are persistent
-    assert info["propagation"] == N.PROP_TILED and info["persistent"] == (0 if stdp else 1)
+    assert info["propagation"] == N.PROP_TILED and (info["persistent"] == 0 if stdp else info["persistent"] >= 1)
     _, o = H.oracle(net, syn, seed=seed, acc=acc, stdp=stdp, stdpconst=stdpconst,
                     scale=(info["fix_scale_exc"], info["fix_scale_inh"]))
     w0 = g.weights() if stdp else None
@@ -121,7 +121,7 @@ def test_batched_shapes(name, wf):
     a.set_launch(s["block"], s["chunk"])
     b = N.Simulator.batch_generated(nets, weight_format=wf, record_capacity=4_000_000, propagation=N.PROP_TILED,
                                     persistent=N.PERSIST_OFF)
-    assert a.info()["persistent"] == 1
+    assert a.info()["persistent"] >= 1
     for n in (STEPS // 2, STEPS // 2):
         a.run(n)
         b.run(n)
