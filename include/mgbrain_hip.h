@@ -78,10 +78,10 @@ enum {
     MGB_PERSIST_AUTO = 0,  /* the whole windows of a run (or of a graph block) in one launch when
                               every workgroup fits on the device at once, else one launch per window;
                               16-bit CLASS words in networks of 32768 neurons or more (and at most
-                              512 tiles): list regions after windows of more than 256 spikes */
+                              512 tiles): with list regions (MGB_PERSIST_REGIONS) */
     MGB_PERSIST_OFF = 1,   /* one launch per window */
-    MGB_PERSIST_REGIONS = 2, /* as AUTO, with list regions in every window of a launch but its last
-                              whatever the size (where the words and tiles allow them): each
+    MGB_PERSIST_REGIONS = 2, /* as AUTO, with list regions whatever the size (where the words and
+                              tiles allow them): in every window of a launch but its last, each
                               workgroup writes its tile's spikes at the tile's own place of the
                               window's list and passes the count with its wait flag */
     MGB_PERSIST_LISTS = 3  /* as AUTO, without list regions: every window's list by a global atomic */

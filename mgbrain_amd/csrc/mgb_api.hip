@@ -1247,14 +1247,14 @@ void update_persist(mgb_sim* s) {
     s->xcd_map = a.xcd_map;
     if (stdp_kind(s->tkind) || s->persist_req == MGB_PERSIST_OFF || !s->bar) return;
     // per-tile list regions (the kernel's variant with them) when asked for, or by
-    // default in networks large enough for windows of hundreds of spikes (in small ones
-    // the variant without them, which preloads the neuron state, is the faster), when one
+    // default in networks large enough for windows of many spikes (in small ones the
+    // variant without them, which preloads the neuron state, is the faster), when one
     // thread per tile can scan their counts and the counts fit in LDS beside the tile's
     // data with every workgroup still resident
     const bool want = s->persist_req == MGB_PERSIST_REGIONS ||
                       (s->persist_req == MGB_PERSIST_AUTO && s->n >= kRegionMinNeurons);
     if (want && s->tkind == kTClass16 && s->G <= kRegionMaxG && s->G <= s->tblock) {
-        a.region = s->persist_req == MGB_PERSIST_REGIONS ? 2 : 1;
+        a.region = 1;
         if (set_tiled_pw_lds_limit(a) == hipSuccess && resident(s, a, true)) {
             s->region = a.region;
             s->persist = 2;

@@ -255,8 +255,7 @@ struct TiledArgs {
     // size (and the counts fitting in LDS), launched as the kernel's list-region variant:
     // windows followed by another window of the same launch write their spikes into the
     // tile's own part of the list (tile_lo * window on) and publish the count with the
-    // wait's flag, instead of reserving list space with a returning global atomic;
-    // 1: windows after one of more than kRegionMinSpikes spikes, 2: every such window
+    // wait's flag, instead of reserving list space with a returning global atomic
     int32_t region;
     // persistent launches: one flag per tile (kBarStride words apart; a 64-bit word:
     // window epoch | spike count << 32) for the wait between windows, its error word,
@@ -291,7 +290,6 @@ constexpr int kSubAlign = 8;
 constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 each
 constexpr int kBarStride = 32;      // words between the flags of a persistent launch's grid wait
 constexpr int kRegionMaxG = 512;    // persistent launches: most tiles whose lists go to per-tile regions
-constexpr int kRegionMinSpikes = 256; // ... and the fewest spikes of the previous window for it
 constexpr int kRegionMinNeurons = 32768; // MGB_PERSIST_AUTO: list regions from this many neurons on
 // Word 3 of a raw buffer resource on gfx9 (CDNA): data format 32, no swizzle; loads past
 // the resource's byte count return 0.

@@ -633,12 +633,11 @@ __device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t G, uint32_t g,
 // neuron state in registers, from the first window to the last; the window
 // lists pass between workgroups through agent-scope accesses and a grid_wait
 // between windows.  Not with STDP.
-template <int CH, int WK, bool PW, int RG = 0>
+template <int CH, int WK, bool PW, bool RG = false>
 __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_t wk, int32_t nsteps,
                                                 int32_t nwin) {
     static_assert(!(PW && (WK == kTStdp || WK == kTStdpB)), "the persistent form has no STDP");
-    static_assert(RG == 0 || (PW && WK == kTClass16 && RG <= 2),
-                  "list regions: persistent form of 16-bit CLASS words only");
+    static_assert(!RG || (PW && WK == kTClass16), "list regions: persistent form of 16-bit CLASS words only");
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ uint32_t wsum[16];
     __shared__ unsigned long long se_blk, lsp_r;
@@ -729,14 +728,12 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
     // (a variant of the persistent form of the 16-bit CLASS words only: in the other
     // kinds' persistent forms the code alone, never run, cost their windows 5 %, and
     // it keeps the neuron-state preload out of the variant, which would spill)
-    constexpr bool kRg = RG != 0;
+    constexpr bool kRg = RG;
     const bool rgn_on = kRg && a.region != 0;
     if (rgn_on)
         for (int32_t h = tid0; h < a.G; h += bd) rg_off[h] = (uint32_t)a.tile_lo[h] * (uint32_t)a.window;
-    // a window's list goes to the regions when the previous one of its segment held more
-    // than kRegionMinSpikes spikes (every workgroup of the segment knows that count, so
-    // producers and consumers agree), or always in the variant RG == 2 (a.region == 2:
-    // every window but a launch's last); the previous window's choice (thread-uniform)
+    // every window's list but a launch's last goes to the regions; whether the previous
+    // window's did (thread-uniform)
     bool rg_prev = false;
     const uint32_t gt0 = one ? 0u : (uint32_t)a.seg_tile0[q];               // the segment's tiles
     const uint32_t gt1 = one ? (uint32_t)a.G : (uint32_t)a.seg_tile0[q + 1];
@@ -810,9 +807,8 @@ __global__ __launch_bounds__(1024) void k_tiled(TiledArgs a, int32_t off, int32_
             E = ld_agent(&tcnt[bprev * nseg]);
             ent0 = ld_agent(&L[li]);
         }
-        // this window's list goes to the tiles' regions (another window of the launch
-        // follows and the previous list was long enough)
-        const bool rgp = rgn_on && kw + 1 < nw && (RG == 2 || E > (uint32_t)kRegionMinSpikes);
+        // this window's list goes to the tiles' regions (another window of the launch follows)
+        const bool rgp = rgn_on && kw + 1 < nw;
         rg_prev = rgp;
         if (full && lt == 0 && tid == 0) {
             if (PW) st_agent(&tcnt[bnext * nseg], 0u);
@@ -1575,12 +1571,12 @@ size_t tiled_lds_bytes(int D, int B, int kind, int rw, int lut_lds) {
 }
 
 namespace {
-template <int CH, int WK, bool PW, int RG>
+template <int CH, int WK, bool PW, bool RG>
 const void* tiled_fn() {
     return reinterpret_cast<const void*>(&k_tiled<CH, WK, PW, RG>);
 }
 
-template <int WK, bool PW, int RG = 0>
+template <int WK, bool PW, bool RG = false>
 const void* tiled_fn_chunk(int chunk) {
     switch (chunk) {
         case 1: return tiled_fn<1, WK, PW, RG>();
@@ -1592,12 +1588,9 @@ const void* tiled_fn_chunk(int chunk) {
 }
 
 // The fused kernel for (kind, chunk); the persistent form exists for every kind without
-// STDP, and for the 16-bit CLASS words also with per-tile list regions (rg: TiledArgs::region).
-const void* tiled_fn_of(int kind, int chunk, bool pw, int rg = 0) {
-    if (rg) {
-        if (!pw || kind != kTClass16) return nullptr;
-        return rg == 2 ? tiled_fn_chunk<kTClass16, true, 2>(chunk) : tiled_fn_chunk<kTClass16, true, 1>(chunk);
-    }
+// STDP, and for the 16-bit CLASS words also with per-tile list regions (rg).
+const void* tiled_fn_of(int kind, int chunk, bool pw, bool rg = false) {
+    if (rg) return pw && kind == kTClass16 ? tiled_fn_chunk<kTClass16, true, true>(chunk) : nullptr;
     switch (kind) {
         case kTBF16: return pw ? tiled_fn_chunk<kTBF16, true>(chunk) : tiled_fn_chunk<kTBF16, false>(chunk);
         case kTFP8: return pw ? tiled_fn_chunk<kTFP8, true>(chunk) : tiled_fn_chunk<kTFP8, false>(chunk);
@@ -1672,7 +1665,7 @@ size_t tiled_pw_lds_bytes(const TiledArgs& a) {
 }
 
 hipError_t set_tiled_pw_lds_limit(const TiledArgs& a) {
-    const void* f = tiled_fn_of(a.kind, a.chunk, true, a.region);
+    const void* f = tiled_fn_of(a.kind, a.chunk, true, a.region != 0);
     if (!f) return hipErrorInvalidDeviceFunction;
     const size_t lds = tiled_pw_lds_bytes(a);
     return lds > 65536 ? hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) : hipSuccess;
@@ -1682,7 +1675,7 @@ hipError_t launch_tiled_persistent(const TiledArgs& a, int32_t off, int32_t wk, 
                                    int block, hipStream_t st) {
     if (stdp_kind(a.kind) || !a.bar || !a.bar_err) return hipErrorInvalidValue;
     if (a.region && (a.G > kRegionMaxG || a.G > block)) return hipErrorInvalidValue;
-    const void* f = tiled_fn_of(a.kind, a.chunk, true, a.region);
+    const void* f = tiled_fn_of(a.kind, a.chunk, true, a.region != 0);
     if (!f) return hipErrorInvalidDeviceFunction;
     return launch_fn(f, a, off, wk, nsteps, nwin, block, tiled_pw_lds_bytes(a), st);
 }
@@ -1690,7 +1683,7 @@ hipError_t launch_tiled_persistent(const TiledArgs& a, int32_t off, int32_t wk, 
 int tiled_blocks_per_cu(const TiledArgs& a, int block, bool persistent) {
     const size_t lds = persistent ? tiled_pw_lds_bytes(a)
                                   : tiled_lds_bytes(a.D, a.B, a.kind, a.rw, stdp_kind(a.kind) ? a.lut_lds : 0);
-    const void* f = tiled_fn_of(a.kind, a.chunk, persistent, persistent ? a.region : 0);
+    const void* f = tiled_fn_of(a.kind, a.chunk, persistent, persistent && a.region != 0);
     if (!f) return 0;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, lds) != hipSuccess) {
@@ -1701,9 +1694,9 @@ int tiled_blocks_per_cu(const TiledArgs& a, int block, bool persistent) {
 }
 
 hipError_t set_tiled_lds_limit(int kind, size_t bytes) {
-    for (int pw = 0; pw < 4; ++pw)     // per window, persistent, persistent with list regions (two)
+    for (int pw = 0; pw < 3; ++pw)     // per window, persistent, persistent with list regions
         for (int chunk : {1, 2, 4, 8, 16}) {
-            const void* f = tiled_fn_of(kind, chunk, pw != 0, pw >= 2 ? pw - 1 : 0);
+            const void* f = tiled_fn_of(kind, chunk, pw != 0, pw == 2);
             if (!f) continue;
             const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
             if (e == hipSuccess) continue;
